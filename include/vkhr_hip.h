@@ -1,0 +1,207 @@
+/*
+ * vkhr_hip.h -- C ABI of the MI355X (gfx950) hybrid-hair hot path.
+ *
+ * Every entry point is a plain C function over caller-owned device pointers,
+ * enqueued on a caller HIP stream (passed as an opaque pointer; NULL is the
+ * default stream), and returns an int status: VKHR_OK (0) or a negative
+ * VKHR_ERR_* code, with a message available from vkhr_last_error().  No C++
+ * exception crosses this boundary.  Calls on distinct buffers are reentrant;
+ * the only hidden state is a small per-device scratch area that the library
+ * allocates on first use (reductions, counters).
+ *
+ * Layouts:
+ *   xyz           float32 x,y,z per vertex, tightly packed (12 B / vertex)
+ *   indices       uint32 line-list pairs (root, tip) as HairStyle::indices
+ *   volumes       x fastest, then y, then z: linear index x + y*nx + z*nx*ny
+ *   matrices      column-major float[16] (glm memory order)
+ *   images        row-major, row 0 at the top (Vulkan framebuffer order),
+ *                 float RGBA interleaved
+ *
+ * Each function names the reference function or shader it replaces
+ * (paths relative to the vkhr source tree).
+ */
+#ifndef VKHR_HIP_H
+#define VKHR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VKHR_ABI_VERSION 1
+
+enum vkhr_status {
+    VKHR_OK = 0,
+    VKHR_ERR_INVALID = -1,     /* bad argument (null pointer, zero size, ...) */
+    VKHR_ERR_HIP = -2,         /* a HIP runtime call failed */
+    VKHR_ERR_OVERFLOW = -3,    /* a size exceeds what the path supports */
+    VKHR_ERR_NOMEM = -4,       /* scratch allocation failed */
+    VKHR_ERR_NO_DEVICE = -5    /* no usable gfx950 device */
+};
+
+typedef void* vkhr_stream; /* hipStream_t */
+
+/* AABB of a hair style: include/vkhr/scene_graph/hair_style.hh:16-21 */
+typedef struct vkhr_aabb {
+    float origin[3];
+    float radius;
+    float size[3];
+    float volume;
+} vkhr_aabb;
+
+/* Camera uniform block: ViewProjection, include/vkhr/scene_graph/camera.hh:15-22
+ * (GLSL twin share/shaders/scene_graph/camera.glsl:4-11). */
+typedef struct vkhr_camera {
+    float view[16];
+    float projection[16];
+    float position[3];
+    float look_at_distance;
+    float znear, zfar;
+    float resolution[2];
+} vkhr_camera;
+
+/* One light: LightSource::Buffer, include/vkhr/scene_graph/light_source.hh:17-23
+ * (GLSL twin share/shaders/scene_graph/lights.glsl:4-13).  matrix is the
+ * biased light view-projection used to sample the shadow map. */
+typedef struct vkhr_light {
+    float vector[4];
+    float intensity[4];
+    float matrix[16];
+    float origin[3];
+    float znear;
+    float zfar;
+    float pad_[3];
+} vkhr_light;
+
+/* Per-style parameters: vulkan::HairStyle::Parameters,
+ * include/vkhr/rasterizer/hair_style.hh:48-56 (GLSL twin
+ * share/shaders/strands/strand.glsl:8-16, where strand_radius is
+ * strand_width, hair_opacity is hair_alpha, hair_shininess is hair_exponent). */
+typedef struct vkhr_strand_params {
+    vkhr_aabb volume_bounds;
+    float volume_resolution[3];
+    float strand_radius;
+    float hair_color[3];
+    float hair_opacity;
+    float hair_shininess;
+    float strand_ratio;
+} vkhr_strand_params;
+
+/* Renderer parameters: Interface::Parameters,
+ * include/vkhr/rasterizer/interface.hh:60-112 (GLSL twin
+ * share/shaders/scene_graph/params.glsl:7-33). */
+typedef struct vkhr_render_params {
+    int32_t shading_model;          /* 0 Kajiya-Kay, 2 ADSM only, 3 LAO only */
+    int32_t adsm_kernel_size;
+    int32_t adsm_sampling_type;
+    int32_t adsm_stride_size;
+    int32_t adsm_on;
+    int32_t ctsm_kernel_size;
+    int32_t ctsm_sampling_type;
+    float ctsm_bias;
+    int32_t ctsm_on;
+    int32_t shadow_technique;
+    float isosurface;
+    float raycast_steps;
+    float occlusion_radius;
+    float ao_exponent;
+    float ao_clamp;
+    float lod_magnified_distance;
+    int32_t renderer;               /* 0 rasterizer, 2 raymarcher, 3 hybrid */
+    float lod_minified_distance;
+    int32_t benchmarking;
+} vkhr_render_params;
+
+/* ---------------------------------------------------------------- library */
+
+int vkhr_abi_version(void);
+/* Message for the last error raised on the calling thread ("" if none). */
+const char* vkhr_last_error(void);
+/* Binds the current device, checks it is gfx950 and sets up the per-device
+ * scratch.  Optional: every entry point calls it on first use. */
+int vkhr_init(void);
+
+/* ------------------------------------------------------ style preparation */
+
+/* HairStyle::generate_bounding_box + get_bounding_box
+ * (src/vkhr/scene_graph/hair_style.cc:215-255): component min/max over the
+ * vertices with both initialised at the origin, origin = min, size = max-min,
+ * radius = |size|, volume = size.x*size.y*size.z.  Writes the host struct
+ * *out; synchronises the stream. */
+int vkhr_aabb_compute(const float* xyz, uint32_t n_vertices, vkhr_aabb* out,
+                      vkhr_stream stream);
+
+/* HairStyle::generate_indices (hair_style.cc:196-213): pairs (v, v+1) within
+ * each strand.  strand_offsets holds n_strands+1 first-vertex offsets;
+ * indices receives 2*(strand_offsets[n]-n_strands) entries. */
+int vkhr_generate_indices(const uint32_t* strand_offsets, uint32_t n_strands,
+                          uint32_t* indices, vkhr_stream stream);
+
+/* HairStyle::generate_tangents (hair_style.cc:171-194): t_v =
+ * normalize(x_{v+1}-x_v), the last vertex of a strand repeats the previous
+ * tangent.  Strands of a single vertex get a zero tangent. */
+int vkhr_generate_tangents(const float* xyz, const uint32_t* strand_offsets,
+                           uint32_t n_strands, float* tangents,
+                           vkhr_stream stream);
+
+/* ------------------------------------------------------------ voxelization */
+
+/* HairStyle::voxelize_vertices (hair_style.cc:257-294): one count per vertex
+ * in voxel min(floor((v-o)/(size/res)), res-1), linear index formed in float.
+ * counts (nx*ny*nz uint32) is zeroed by the call and receives unsaturated
+ * counts; the reference's uint8 view is min(count, 255).  tangent_sums
+ * (nullable, 3 floats per voxel) receives the per-voxel sum of the tangents
+ * of all contributions (tangents must then be non-null).  Requires
+ * nx*ny*nz <= 2^24 (the float index is exact only up to there). */
+int vkhr_voxelize_vertices(const float* xyz, uint32_t n_vertices,
+                           const float* tangents, const vkhr_aabb* bounds,
+                           uint32_t nx, uint32_t ny, uint32_t nz,
+                           uint32_t* counts, float* tangent_sums,
+                           vkhr_stream stream);
+
+/* HairStyle::voxelize_segments (hair_style.cc:296-342): per index pair, the
+ * reference's DDA -- root and tip in voxel units, steps = max |tip-root|,
+ * direction = (tip-root)/steps, ceil(steps) samples at root, root+dir, ...
+ * accumulated sequentially in float; each sample counts voxel
+ * min(floor(p), res-1).  Bit-exact with the reference's float arithmetic
+ * evaluated in IEEE single precision without contraction.  Segments whose
+ * samples leave the volume (possible only with an AABB that does not contain
+ * the vertices) are clamped at 0 from below instead of indexing out of
+ * bounds.  Same outputs and limits as vkhr_voxelize_vertices; the tangent of
+ * a segment is the tangent of its root vertex. */
+int vkhr_voxelize_segments(const float* xyz, uint32_t n_vertices,
+                           const uint32_t* indices, uint32_t n_indices,
+                           const float* tangents, const vkhr_aabb* bounds,
+                           uint32_t nx, uint32_t ny, uint32_t nz,
+                           uint32_t* counts, float* tangent_sums,
+                           vkhr_stream stream);
+
+/* Turns raw counts into the volumes the renderers sample, as the reference's
+ * load path does (rasterizer/hair_style.cc:75-77, hair_style.cc:283-291,
+ * 331-357):
+ *   density_u8   = min(count, 255), then, if normalize != 0,
+ *                  HairStyle::Volume::normalize (min/max stretch to 0..255,
+ *                  float scale, truncation; a constant volume becomes 0);
+ *   tangent_i8x4 = trunc(sum / min(count,255) * 127) per component, w = 0,
+ *                  0 for empty voxels (nullable; needs tangent_sums);
+ *   density_f32  = density_u8 / 255 (nullable; the R8_UNORM value);
+ *   cells        = corner-packed density (nullable): for every cell
+ *                  (i,j,k), i,j,k in [-1, n-1], the 8 texels of its
+ *                  trilinear footprint as one uint64 (byte b = texel
+ *                  (i+(b&1), j+(b>>1&1), k+(b>>2)), 0 outside the volume);
+ *                  (nx+1)*(ny+1)*(nz+1) entries, index (i+1) + (j+1)*(nx+1)
+ *                  + (k+1)*(nx+1)*(ny+1).  The raymarcher reads this layout:
+ *                  one 8-byte gather per trilinear tap. */
+int vkhr_volume_finalize(const uint32_t* counts, const float* tangent_sums,
+                         uint32_t nx, uint32_t ny, uint32_t nz, int normalize,
+                         uint8_t* density_u8, int8_t* tangent_i8x4,
+                         float* density_f32, uint64_t* cells,
+                         vkhr_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VKHR_HIP_H */

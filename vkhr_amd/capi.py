@@ -1,0 +1,132 @@
+"""ctypes binding of libvkhr_hip.so (the C ABI declared in include/vkhr_hip.h).
+
+This module only loads and describes the library: there is no fallback.  If
+the shared object is missing or does not export a declared symbol, importing
+raises, so nothing can silently run without the HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+from pathlib import Path
+
+PKG_DIR = Path(__file__).resolve().parent
+ROOT = PKG_DIR.parent
+LIB_PATH = PKG_DIR / "libvkhr_hip.so"
+HEADER = ROOT / "include" / "vkhr_hip.h"
+
+VKHR_OK = 0
+STATUS = {0: "VKHR_OK", -1: "VKHR_ERR_INVALID", -2: "VKHR_ERR_HIP", -3: "VKHR_ERR_OVERFLOW",
+          -4: "VKHR_ERR_NOMEM", -5: "VKHR_ERR_NO_DEVICE"}
+
+
+class VkhrError(RuntimeError):
+    def __init__(self, fn: str, code: int, message: str):
+        super().__init__(f"{fn} -> {STATUS.get(code, code)}: {message}")
+        self.code = code
+
+
+class AABB(C.Structure):
+    _fields_ = [("origin", C.c_float * 3), ("radius", C.c_float), ("size", C.c_float * 3), ("volume", C.c_float)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("view", C.c_float * 16), ("projection", C.c_float * 16), ("position", C.c_float * 3),
+                ("look_at_distance", C.c_float), ("znear", C.c_float), ("zfar", C.c_float),
+                ("resolution", C.c_float * 2)]
+
+
+class Light(C.Structure):
+    _fields_ = [("vector", C.c_float * 4), ("intensity", C.c_float * 4), ("matrix", C.c_float * 16),
+                ("origin", C.c_float * 3), ("znear", C.c_float), ("zfar", C.c_float), ("pad_", C.c_float * 3)]
+
+
+class StrandParams(C.Structure):
+    _fields_ = [("volume_bounds", AABB), ("volume_resolution", C.c_float * 3), ("strand_radius", C.c_float),
+                ("hair_color", C.c_float * 3), ("hair_opacity", C.c_float), ("hair_shininess", C.c_float),
+                ("strand_ratio", C.c_float)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("shading_model", C.c_int32), ("adsm_kernel_size", C.c_int32), ("adsm_sampling_type", C.c_int32),
+                ("adsm_stride_size", C.c_int32), ("adsm_on", C.c_int32), ("ctsm_kernel_size", C.c_int32),
+                ("ctsm_sampling_type", C.c_int32), ("ctsm_bias", C.c_float), ("ctsm_on", C.c_int32),
+                ("shadow_technique", C.c_int32), ("isosurface", C.c_float), ("raycast_steps", C.c_float),
+                ("occlusion_radius", C.c_float), ("ao_exponent", C.c_float), ("ao_clamp", C.c_float),
+                ("lod_magnified_distance", C.c_float), ("renderer", C.c_int32),
+                ("lod_minified_distance", C.c_float), ("benchmarking", C.c_int32)]
+
+
+P = C.c_void_p
+U32 = C.c_uint32
+I32 = C.c_int
+
+# name -> (argtypes); every function returns int status except noted.
+SIGNATURES = {
+    "vkhr_abi_version": [],
+    "vkhr_init": [],
+    "vkhr_aabb_compute": [P, U32, C.POINTER(AABB), P],
+    "vkhr_generate_indices": [P, U32, P, P],
+    "vkhr_generate_tangents": [P, P, U32, P, P],
+    "vkhr_voxelize_vertices": [P, U32, P, C.POINTER(AABB), U32, U32, U32, P, P, P],
+    "vkhr_voxelize_segments": [P, U32, P, U32, P, C.POINTER(AABB), U32, U32, U32, P, P, P],
+    "vkhr_volume_finalize": [P, P, U32, U32, U32, I32, P, P, P, P, P],
+}
+
+
+def declared_symbols() -> list[str]:
+    """Every function name declared in include/vkhr_hip.h."""
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vkhr_[a-z0-9_]+)\s*\(", text)))
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C vkhr_amd)")
+    lib = C.CDLL(str(LIB_PATH))
+    lib.vkhr_last_error.restype = C.c_char_p
+    lib.vkhr_last_error.argtypes = []
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    return lib
+
+
+lib = _load()
+
+
+def call(name: str, *args) -> None:
+    """Calls a status-returning entry point and raises VkhrError on failure."""
+    rc = getattr(lib, name)(*args)
+    if rc != VKHR_OK:
+        raise VkhrError(name, rc, lib.vkhr_last_error().decode(errors="replace"))
+
+
+def ptr(t) -> int | None:
+    """Device/host address of a torch tensor (None for None)."""
+    return None if t is None else t.data_ptr()
+
+
+def stream_of(torch_stream=None):
+    import torch
+    s = torch_stream if torch_stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def make_aabb(d: dict) -> AABB:
+    a = AABB()
+    for i in range(3):
+        a.origin[i] = float(d["origin"][i])
+        a.size[i] = float(d["size"][i])
+    a.radius = float(d["radius"])
+    a.volume = float(d["volume"])
+    return a
+
+
+def aabb_dict(a: AABB) -> dict:
+    import numpy as np
+    return dict(origin=np.array(a.origin[:], np.float32), size=np.array(a.size[:], np.float32),
+                radius=np.float32(a.radius), volume=np.float32(a.volume))

@@ -1,0 +1,76 @@
+"""Torch-tensor conveniences over the C ABI (device memory and streams only:
+every computation is a libvkhr_hip entry point).  Used by tests and bench."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import capi
+from .capi import call, ptr, stream_of
+
+
+def to_dev(a: np.ndarray, device="cuda") -> torch.Tensor:
+    """Copies a numpy array to the device; unsigned 32/64-bit data travel as
+    the same-width signed dtype (the kernels read the bits)."""
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint32:
+        a = a.view(np.int32)
+    elif a.dtype == np.uint64:
+        a = a.view(np.int64)
+    return torch.from_numpy(a).to(device)
+
+
+def aabb(xyz: torch.Tensor) -> dict:
+    out = capi.AABB()
+    call("vkhr_aabb_compute", ptr(xyz), xyz.shape[0], C.byref(out), stream_of())
+    return capi.aabb_dict(out)
+
+
+def generate_indices(offsets: torch.Tensor) -> torch.Tensor:
+    n = offsets.shape[0] - 1
+    n_idx = 2 * (int(offsets[-1].item()) - n)
+    out = torch.empty(max(n_idx, 1), dtype=torch.int32, device=offsets.device)
+    call("vkhr_generate_indices", ptr(offsets), n, ptr(out), stream_of())
+    return out[:n_idx]
+
+
+def generate_tangents(xyz: torch.Tensor, offsets: torch.Tensor) -> torch.Tensor:
+    out = torch.zeros_like(xyz)
+    call("vkhr_generate_tangents", ptr(xyz), ptr(offsets), offsets.shape[0] - 1, ptr(out), stream_of())
+    return out
+
+
+def voxelize(xyz: torch.Tensor, bounds: dict, res, indices: torch.Tensor | None = None,
+             tangents: torch.Tensor | None = None, want_sums: bool = False, counts=None, sums=None):
+    """Vertex mode when indices is None, else segment mode.  Returns (counts, sums)."""
+    nx, ny, nz = res
+    n = nx * ny * nz
+    dev = xyz.device
+    if counts is None:
+        counts = torch.empty(n, dtype=torch.int32, device=dev)
+    if want_sums and sums is None:
+        sums = torch.empty(3 * n, dtype=torch.float32, device=dev)
+    b = capi.make_aabb(bounds)
+    if indices is None:
+        call("vkhr_voxelize_vertices", ptr(xyz), xyz.shape[0], ptr(tangents), C.byref(b), nx, ny, nz,
+             ptr(counts), ptr(sums) if want_sums else None, stream_of())
+    else:
+        call("vkhr_voxelize_segments", ptr(xyz), xyz.shape[0], ptr(indices), indices.shape[0], ptr(tangents),
+             C.byref(b), nx, ny, nz, ptr(counts), ptr(sums) if want_sums else None, stream_of())
+    return counts, (sums if want_sums else None)
+
+
+def finalize(counts: torch.Tensor, res, sums: torch.Tensor | None = None, normalize: bool = True,
+             want_f32: bool = False, want_cells: bool = True):
+    nx, ny, nz = res
+    n = nx * ny * nz
+    dev = counts.device
+    dens = torch.empty(n, dtype=torch.uint8, device=dev)
+    tan = torch.empty(4 * n, dtype=torch.int8, device=dev) if sums is not None else None
+    f32 = torch.empty(n, dtype=torch.float32, device=dev) if want_f32 else None
+    cells = torch.empty((nx + 1) * (ny + 1) * (nz + 1), dtype=torch.int64, device=dev) if want_cells else None
+    call("vkhr_volume_finalize", ptr(counts), ptr(sums), nx, ny, nz, int(normalize), ptr(dens), ptr(tan),
+         ptr(f32), ptr(cells), stream_of())
+    return dict(density=dens, tangents=tan, density_f32=f32, cells=cells)
