@@ -200,6 +200,41 @@ int vkhr_volume_finalize(const uint32_t* counts, const float* tangent_sums,
                          float* density_f32, uint64_t* cells,
                          vkhr_stream stream);
 
+/* --------------------------------------------------------------- raymarch */
+
+/* The volumes one hair style is rendered from (device pointers).  Its world
+ * placement is vkhr_strand_params.volume_bounds, as in the Strand uniform
+ * block of the shaders. */
+typedef struct vkhr_volume {
+    const uint8_t* density;      /* nx*ny*nz R8_UNORM texels (nullable if cells given) */
+    const uint64_t* cells;       /* corner-packed density (vkhr_volume_finalize) */
+    const int8_t* tangents;      /* nx*ny*nz R8G8B8A8_SNORM texels */
+    const float* density_f32;    /* nullable: float densities, used instead of cells when set */
+    uint32_t nx, ny, nz;
+    uint32_t reserved_;
+} vkhr_volume;
+
+/* Per-pixel volume raymarch of one hair style: the raymarcher pass
+ * (share/shaders/volumes/volume.frag:31-102 with volume_rendering.glsl:19-52,
+ * sample_volume.glsl:7-9, approximate_deep_shadows.glsl:24-36,
+ * local_ambient_occlusion.glsl:9-30, kajiya-kay.glsl:7-24, scheme.glsl:7-11)
+ * as drawn by vulkan::Volume::draw over the AABB proxy
+ * (src/vkhr/rasterizer/volume.cc:104-203).  A pixel gets a fragment when its
+ * centre ray from camera->position enters the AABB in front of the eye
+ * (CW front faces, back faces culled: no fragments with the eye inside).
+ * Trilinear filtering is computed exactly (not by a texture unit).
+ *   depth_buffer  nullable per-pixel depth read by the march's early break
+ *                 (the subpass depth input); NULL reads 1.0, the cleared value
+ *   rgba          W*H float4: the fragment colour clamped to [0,1] (what
+ *                 packUnorm4x8 consumes); 0 where no fragment
+ *   depth         nullable W*H NDC depth of the surface point; 1 where none
+ *   surface_info  nullable W*H int4 {fragment, surface step (-1: the shader's
+ *                 vec3(0) surface), entry step (-1: none), found} */
+int vkhr_raymarch(const vkhr_volume* volume, const vkhr_camera* camera, const vkhr_light* light,
+                  const vkhr_strand_params* strand, const vkhr_render_params* params, uint32_t width,
+                  uint32_t height, const float* depth_buffer, float* rgba, float* depth,
+                  int32_t* surface_info, vkhr_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

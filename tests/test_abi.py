@@ -33,7 +33,7 @@ def _header_struct_fields(name):
         decl = decl.strip()
         if not decl:
             continue
-        words = decl.replace(",", " , ").split()
+        words = decl.replace(",", " , ").replace("*", " ").replace("const ", " ").split()
         names = [w for w in words[1:] if w != ","]
         for n in names:
             base = n.split("[")[0]
@@ -43,8 +43,9 @@ def _header_struct_fields(name):
 
 
 def test_struct_layouts_match_header():
-    pairs = [("vkhr_aabb", capi.AABB), ("vkhr_camera", capi.Camera), ("vkhr_light", capi.Light),
-             ("vkhr_strand_params", capi.StrandParams), ("vkhr_render_params", capi.RenderParams)]
+    pairs = [("vkhr_volume", capi.Volume), ("vkhr_aabb", capi.AABB), ("vkhr_camera", capi.Camera),
+             ("vkhr_light", capi.Light), ("vkhr_strand_params", capi.StrandParams),
+             ("vkhr_render_params", capi.RenderParams)]
     for cname, cls in pairs:
         hdr = _header_struct_fields(cname)
         names = [f[0] for f in cls._fields_]

@@ -58,6 +58,12 @@ class RenderParams(C.Structure):
                 ("lod_minified_distance", C.c_float), ("benchmarking", C.c_int32)]
 
 
+class Volume(C.Structure):
+    _fields_ = [("density", C.c_void_p), ("cells", C.c_void_p), ("tangents", C.c_void_p),
+                ("density_f32", C.c_void_p), ("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32),
+                ("reserved_", C.c_uint32)]
+
+
 P = C.c_void_p
 U32 = C.c_uint32
 I32 = C.c_int
@@ -72,6 +78,8 @@ SIGNATURES = {
     "vkhr_voxelize_vertices": [P, U32, P, C.POINTER(AABB), U32, U32, U32, P, P, P],
     "vkhr_voxelize_segments": [P, U32, P, U32, P, C.POINTER(AABB), U32, U32, U32, P, P, P],
     "vkhr_volume_finalize": [P, P, U32, U32, U32, I32, P, P, P, P, P],
+    "vkhr_raymarch": [C.POINTER(Volume), C.POINTER(Camera), C.POINTER(Light), C.POINTER(StrandParams),
+                      C.POINTER(RenderParams), U32, U32, P, P, P, P, P],
 }
 
 
@@ -85,6 +93,11 @@ def declared_symbols() -> list[str]:
 def _load() -> C.CDLL:
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C vkhr_amd)")
+    # torch ships its own HIP runtime under the same soname (libamdhip64.so.7).
+    # Loading torch first makes the library bind to that one instance, so
+    # device pointers and streams are shared; two runtimes in one process
+    # would not see each other's allocations.
+    import torch  # noqa: F401
     lib = C.CDLL(str(LIB_PATH))
     lib.vkhr_last_error.restype = C.c_char_p
     lib.vkhr_last_error.argtypes = []

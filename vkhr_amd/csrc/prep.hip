@@ -49,14 +49,20 @@ aabb_partial_kernel(const float* __restrict__ xyz, uint32_t n, float* __restrict
     }
 }
 
-// Stage 2: one block folds the per-block results.
+// Stage 2: one wave folds the per-block results (lanes stride over blocks).
 __global__ void __launch_bounds__(kWave)
 aabb_final_kernel(const float* __restrict__ partial, int blocks, float* __restrict__ out) {
-    int c = threadIdx.x;
-    if (c >= 6) return;
-    float r = 0.0f;
-    for (int b = 0; b < blocks; ++b) r = c < 3 ? fminf(r, partial[6 * b + c]) : fmaxf(r, partial[6 * b + c]);
-    out[c] = r;
+    float mn[3] = {0.0f, 0.0f, 0.0f}, mx[3] = {0.0f, 0.0f, 0.0f};
+    for (int b = threadIdx.x; b < blocks; b += kWave) {
+        for (int c = 0; c < 3; ++c) {
+            mn[c] = fminf(mn[c], partial[6 * b + c]);
+            mx[c] = fmaxf(mx[c], partial[6 * b + 3 + c]);
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        float a = wave_min(mn[c]), b = wave_max(mx[c]);
+        if (threadIdx.x == 0) { out[c] = a; out[3 + c] = b; }
+    }
 }
 
 __global__ void __launch_bounds__(256)

@@ -4,7 +4,7 @@
 // layout read by the raymarcher.
 //
 // Pass 1 (per voxel): saturating uint8 view, tangent quantization, and the
-//                     min/max of the uint8 view (wave reduce + 1 atomic/wave).
+//                     min/max of the uint8 view (one atomic pair per block).
 // Pass 2 (per voxel): HairStyle::Volume::normalize and the float view.
 // Pass 3 (per cell):  corner packing, 8 texels -> one uint64.
 // Compiled with -ffp-contract=off (quantization and normalize are float
@@ -52,13 +52,21 @@ finalize_counts_kernel(const uint32_t* __restrict__ counts, const float* __restr
             reinterpret_cast<char4*>(tangents)[i] = q;
         }
     }
-    // wave-level min/max, then one atomic pair per wave
+    // wave min/max, then block min/max in LDS, then one atomic pair per block
     for (int o = 32; o > 0; o >>= 1) {
         uint32_t a = __shfl_xor(lo, o, kWave), b = __shfl_xor(hi, o, kWave);
         lo = a < lo ? a : lo;
         hi = b > hi ? b : hi;
     }
-    if (minmax && (threadIdx.x % kWave) == 0) {
+    __shared__ uint32_t red[2][256 / kWave];
+    int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    if (lane == 0) { red[0][wid] = lo; red[1][wid] = hi; }
+    __syncthreads();
+    if (minmax && threadIdx.x == 0) {
+        for (int w = 1; w < int(blockDim.x / kWave); ++w) {
+            lo = red[0][w] < lo ? red[0][w] : lo;
+            hi = red[1][w] > hi ? red[1][w] : hi;
+        }
         atomicMin(&minmax[0], lo);
         atomicMax(&minmax[1], hi);
     }
@@ -146,7 +154,8 @@ extern "C" int vkhr_volume_finalize(const uint32_t* counts, const float* tangent
         minmax_init_kernel<<<1, 64, 0, s>>>(mm);
         VKHR_CHECK_LAUNCH();
     }
-    unsigned grid = grid_for(n, 256, 16384);
+    // A capped grid keeps the min/max atomics at one pair per block.
+    unsigned grid = grid_for(n, 256, 2048);
     finalize_counts_kernel<<<grid, 256, 0, s>>>(counts, tangent_sums, n, density_u8, tangent_i8x4, mm);
     VKHR_CHECK_LAUNCH();
     if (normalize || density_f32) {

@@ -74,3 +74,39 @@ def finalize(counts: torch.Tensor, res, sums: torch.Tensor | None = None, normal
     call("vkhr_volume_finalize", ptr(counts), ptr(sums), nx, ny, nz, int(normalize), ptr(dens), ptr(tan),
          ptr(f32), ptr(cells), stream_of())
     return dict(density=dens, tangents=tan, density_f32=f32, cells=cells)
+
+
+def volume_struct(vol: dict, res) -> capi.Volume:
+    v = capi.Volume()
+    v.density = ptr(vol.get("density"))
+    v.cells = ptr(vol.get("cells"))
+    v.tangents = ptr(vol.get("tangents"))
+    v.density_f32 = ptr(vol.get("density_f32"))
+    v.nx, v.ny, v.nz = res
+    return v
+
+
+def raymarch(vol: dict, res, cam, light, strand, render, width: int, height: int, depth_buffer=None,
+             want_depth=True, want_info=False, out=None):
+    """Returns dict(rgba (H,W,4) float32, depth (H,W) or None, info (H,W,4) int32 or None)."""
+    dev = next(t for t in vol.values() if t is not None).device
+    rgba = out if out is not None else torch.empty((height, width, 4), dtype=torch.float32, device=dev)
+    depth = torch.empty((height, width), dtype=torch.float32, device=dev) if want_depth else None
+    info = torch.empty((height, width, 4), dtype=torch.int32, device=dev) if want_info else None
+    v = volume_struct(vol, res)
+    call("vkhr_raymarch", C.byref(v), C.byref(cam), C.byref(light), C.byref(strand), C.byref(render), width,
+         height, ptr(depth_buffer), ptr(rgba), ptr(depth), ptr(info), stream_of())
+    return dict(rgba=rgba, depth=depth, info=info)
+
+
+def style_volume(style, res, device, segments=True, want_f32=False):
+    """Voxelize + finalize a synthetic style on the device (the reference's
+    load path).  Returns (bounds, volume dict)."""
+    xyz = to_dev(style.vertices, device)
+    off = to_dev(style.strand_offsets, device)
+    tan = generate_tangents(xyz, off)
+    b = aabb(xyz)
+    idx = generate_indices(off) if segments else None
+    counts, sums = voxelize(xyz, b, res, indices=idx, tangents=tan, want_sums=True)
+    vol = finalize(counts, res, sums=sums, normalize=True, want_f32=want_f32, want_cells=True)
+    return b, vol
