@@ -228,8 +228,9 @@ typedef struct vkhr_volume {
  *   rgba          W*H float4: the fragment colour clamped to [0,1] (what
  *                 packUnorm4x8 consumes); 0 where no fragment
  *   depth         nullable W*H NDC depth of the surface point; 1 where none
- *   surface_info  nullable W*H int4 {fragment, surface step (-1: the shader's
- *                 vec3(0) surface), entry step (-1: none), found} */
+ *   surface_info  nullable W*H int4 {state (0: the ray misses the box,
+ *                 1: fragment, 2: marched but discarded), surface step (-1:
+ *                 the shader's vec3(0) surface), entry step (-1: none), found} */
 int vkhr_raymarch(const vkhr_volume* volume, const vkhr_camera* camera, const vkhr_light* light,
                   const vkhr_strand_params* strand, const vkhr_render_params* params, uint32_t width,
                   uint32_t height, const float* depth_buffer, float* rgba, float* depth,

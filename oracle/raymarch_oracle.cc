@@ -308,6 +308,7 @@ void oracle_raymarch(const uint8_t* density, const int8_t* tangent, const float*
             entry = forced[3 * q + 1];
             found = forced[3 * q + 2] != 0;
         }
+        inf[0] = 2;  // the ray marched; 1 once a fragment is written
         inf[1] = surf; inf[2] = entry; inf[3] = found ? 1 : 0;
         // sample index -> position (the shader keeps P itself)
         auto pos_of = [&](int k) {

@@ -28,7 +28,7 @@ def compare_raymarch(gpu: dict, vol_np: dict, res, cam, light, strand, render, w
                     pixels=pixels, density_f32=density_f32, depth_buffer=depth_buffer)
     g_info = info[pixels]
     frag_g, frag_o = g_info[:, 0] == 1, o["info"][:, 0] == 1
-    frag_mismatch = int((frag_g != frag_o).sum())
+    frag_mismatch = int((frag_g != frag_o).sum()) + int(((g_info[:, 0] == 0) != (o["info"][:, 0] == 0)).sum())
     both = frag_g & frag_o
     decision_diff = both & (np.any(g_info[:, 1:] != o["info"][:, 1:], axis=1))
     n_ties = int(decision_diff.sum())

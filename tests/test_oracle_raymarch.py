@@ -177,7 +177,8 @@ def test_empty_volume_discards_everything():
     sc, bounds, dens, tan = axis_scene(n, w, h, c=0)
     o = OR.raymarch(dens, tan, (n, n, n), params.camera(sc), params.light(sc), params.strand(bounds, (n, n, n)),
                     params.render(raycast_steps=32.0), w, h)
-    assert (o["info"][:, 0] == 0).all() and (o["rgba"] == 0).all() and (o["depth"] == 1).all()
+    assert (o["info"][:, 0] != 1).all() and (o["rgba"] == 0).all() and (o["depth"] == 1).all()
+    assert (o["info"][:, 0] == 2).sum() > 100      # rays entered the box and were discarded
 
 
 def test_fragment_mask_matches_box_silhouette():
@@ -192,7 +193,7 @@ def test_fragment_mask_matches_box_silhouette():
     tan = np.zeros(4 * n ** 3, np.int8)
     o = OR.raymarch(dens, tan, (n, n, n), params.camera(sc), params.light(sc), params.strand(b, (n, n, n)),
                     params.render(raycast_steps=16.0), w, h)
-    mask = o["info"][:, 0].reshape(h, w) == 1
+    mask = o["info"][:, 0].reshape(h, w) != 0
     vp = np.array(sc.projection, np.float32).astype(np.float64) @ np.array(sc.view, np.float32).astype(np.float64)
     corners = np.array([[b["origin"][i] + b["size"][i] * ((k >> i) & 1) for i in range(3)] for k in range(8)])
     c4 = (vp @ np.c_[corners, np.ones(8)].T).T

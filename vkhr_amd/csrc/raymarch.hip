@@ -203,7 +203,7 @@ raymarch_kernel(RaymarchArgs a) {
                 if (entry < 0) { entry = i; t_entry = t; }
             }
         }
-        info = make_int4(0, surf, entry, found ? 1 : 0);
+        info = make_int4(2, surf, entry, found ? 1 : 0);  // marched; 1 once a fragment is written
         double sp[3] = {0.0, 0.0, 0.0};  // the shader's vec3(0) surface
         if (found ? (surf >= 0) : (entry >= 0)) {
             double ts = double(found ? t_surf : t_entry);
