@@ -236,6 +236,83 @@ int vkhr_raymarch(const vkhr_volume* volume, const vkhr_camera* camera, const vk
                   uint32_t height, const float* depth_buffer, float* rgba, float* depth,
                   int32_t* surface_info, vkhr_stream stream);
 
+/* ------------------------------------------------- strand rasterizer, PPLL */
+
+/* Per-pixel linked list (vulkan::LinkedList, src/vkhr/rasterizer/
+ * linked_list.hh:16-30; share/shaders/transparency/ppll.glsl:4-48).
+ * nodes holds node_count 12-byte nodes {packUnorm4x8 colour, float depth,
+ * uint32 previous node}; heads holds width*height node indices
+ * (0xffffffff = end of list); counter is one uint32. */
+typedef struct vkhr_ppll {
+    uint32_t* heads;
+    uint32_t* nodes;
+    uint32_t* counter;
+    uint32_t node_count;
+    uint32_t width, height;
+    uint32_t reserved_;
+} vkhr_ppll;
+
+/* Optional per-node capture of what the rasterizer inserted (for tests and
+ * debugging); each array has node_count entries, any may be NULL. */
+typedef struct vkhr_frag_capture {
+    uint32_t* segment;   /* index pair that produced the node */
+    uint32_t* pixel;     /* linear pixel index */
+    float* color;        /* float4 colour before packUnorm4x8 */
+} vkhr_frag_capture;
+
+/* vulkan::LinkedList::clear (linked_list.cc:75-98): heads = 0xffffffff,
+ * counter = 0. */
+int vkhr_ppll_clear(const vkhr_ppll* ppll, vkhr_stream stream);
+
+/* Line rule shared by the shadow-map and colour passes (the reference leaves
+ * wide-line coverage to the Vulkan driver; this is the build's definition).
+ * A segment is clipped to 0 <= z <= w in clip space and mapped to window
+ * coordinates a, b (viewport = the target, depth 0..1); pixel (i, j) with
+ * centre c = (i+0.5, j+0.5) gets a fragment when
+ *   0 <= t < 1,  t = (c-a).(b-a) / |b-a|^2     (ends owned half-open)
+ *   dist(c, line ab) <= line_width / 2
+ * Window depth interpolates linearly in t; varyings perspective-correctly. */
+
+/* Shadow-map depth pass (Rasterizer::draw_depth, src/vkhr/rasterizer.cc:
+ * 293-315; depth_map.vert:9-12): lines of width line_width under the
+ * unbiased light view-projection light_vp (column-major) into a size_w x
+ * size_h float depth map, cleared to 1 and kept at the minimum (depth test
+ * LESS with writes).  Vertices are world-space (identity model matrix). */
+int vkhr_shadow_depth(const float* xyz, uint32_t n_vertices, const uint32_t* indices, uint32_t n_indices,
+                      const float* light_vp, float line_width, uint32_t size_w, uint32_t size_h,
+                      float* depth_map, vkhr_stream stream);
+
+/* Strand colour pass (vulkan::HairStyle::draw, rasterizer/hair_style.cc:
+ * 182-199; strand.vert:21-31; strand.frag:34-87 with gpaa.glsl:33-51,
+ * kajiya-kay.glsl:7-24, approximate_deep_shadows.glsl:12-73 (3x3 Gaussian
+ * PCF of shadow_map through light->matrix), local_ambient_occlusion.glsl:9-30
+ * on the density volume, ppll.glsl:23-48).  Line width is
+ * strand->strand_radius; each surviving fragment becomes a PPLL node
+ * (allocation aggregated per wave: one counter atomic per wave).  Nodes
+ * beyond ppll->node_count are dropped (the reference accepts index ==
+ * node_count, one past its pool).  depth_buffer (nullable, W*H) is the depth
+ * test input (LESS, no writes); NULL reads 1.0.  capture is nullable. */
+int vkhr_raster_strands(const float* xyz, const float* tangents, const float* thickness, uint32_t n_vertices,
+                        const uint32_t* indices, uint32_t n_indices, const vkhr_camera* camera,
+                        const vkhr_light* light, const vkhr_strand_params* strand,
+                        const vkhr_render_params* params, const vkhr_volume* volume, const float* shadow_map,
+                        uint32_t shadow_w, uint32_t shadow_h, const float* depth_buffer, const vkhr_ppll* ppll,
+                        const vkhr_frag_capture* capture, vkhr_stream stream);
+
+/* PPLL resolve (share/shaders/transparency/resolve.comp:15-102): per pixel a
+ * k-buffer of the K=16 newest nodes, up to 1024 further nodes swapped in by
+ * depth and blended unsorted, then the k-buffer blended back to front, all
+ * onto the background (float4 W*H, nullable: opaque white, the cleared
+ * target).  out (float4 W*H) receives the blended colour before the rgba8
+ * store; pixels without nodes receive the background. */
+int vkhr_ppll_resolve(const vkhr_ppll* ppll, const float* background, float* out, vkhr_stream stream);
+
+/* Raymarch fragments into the PPLL (the raymarch pass's ppll_node_data /
+ * ppll_link_node, volume.frag:95-100), alongside the float outputs. */
+int vkhr_raymarch_ppll(const vkhr_volume* volume, const vkhr_camera* camera, const vkhr_light* light,
+                       const vkhr_strand_params* strand, const vkhr_render_params* params, uint32_t width,
+                       uint32_t height, const float* depth_buffer, const vkhr_ppll* ppll, vkhr_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

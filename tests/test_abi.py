@@ -45,7 +45,8 @@ def _header_struct_fields(name):
 def test_struct_layouts_match_header():
     pairs = [("vkhr_volume", capi.Volume), ("vkhr_aabb", capi.AABB), ("vkhr_camera", capi.Camera),
              ("vkhr_light", capi.Light), ("vkhr_strand_params", capi.StrandParams),
-             ("vkhr_render_params", capi.RenderParams)]
+             ("vkhr_render_params", capi.RenderParams), ("vkhr_ppll", capi.Ppll),
+             ("vkhr_frag_capture", capi.FragCapture)]
     for cname, cls in pairs:
         hdr = _header_struct_fields(cname)
         names = [f[0] for f in cls._fields_]

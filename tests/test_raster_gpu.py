@@ -1,0 +1,203 @@
+"""GPU parity of the strand rasterizer against the float64 oracle, in the
+two stages the blend-order-sensitive k-buffer calls for:
+  1. fragments: the set of (segment, pixel) pairs, their colour before
+     packUnorm4x8 (max abs error 1e-4) and depth (1e-6), read through the
+     per-node capture; the packed node colour is packUnorm4x8 of it;
+  2. resolve: given the identical node lists (atomic insertion order is not
+     deterministic), the k-buffer resolve within 2e-3 of float64.
+Also: the shadow-map depth pass, PPLL bookkeeping, pool overflow without
+out-of-bounds writes, and the full 1080p / 1024^2 / 32M-node configuration
+on a sampled subset of segments."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import raster as ORs
+from vkhr_amd import params, synth
+from vkhr_amd import device as D
+
+pytestmark = pytest.mark.gpu
+
+FRAG_TOL = 1e-4
+RESOLVE_TOL = 2e-3
+NULL = 0xFFFFFFFF
+
+
+def _cm(m):
+    return np.asarray(m, np.float64).T.reshape(-1).astype(np.float32)
+
+
+class Scene:
+    def __init__(self, style, res, w, h, device, azimuth=0.0, shadow=1024, segments=True, line_width=1.0):
+        self.style, self.res, self.w, self.h = style, res, w, h
+        self.xyz = D.to_dev(style.vertices, device)
+        off = D.to_dev(style.strand_offsets, device)
+        self.idx = D.generate_indices(off)
+        self.tan = D.generate_tangents(self.xyz, off)
+        self.th = D.to_dev(style.thickness, device)
+        self.bounds, self.vol = D.style_volume(style, res, device, segments=segments)
+        self.sc = synth.make_scene(self.bounds, w, h, azimuth_deg=azimuth)
+        self.cam, self.light = params.camera(self.sc), params.light(self.sc)
+        self.sp = params.strand(self.bounds, res, strand_radius=line_width)
+        self.rp = params.render(renderer=0)
+        self.shadow = shadow
+        self.smap = D.shadow_depth(self.xyz, self.idx, _cm(self.sc.light_vp), line_width, (shadow, shadow))
+
+    def np_inputs(self):
+        return dict(xyz=self.style.vertices, tan=self.tan.cpu().numpy(), th=self.style.thickness,
+                    idx=self.idx.cpu().numpy().view(np.uint32), dens=self.vol["density"].cpu().numpy(),
+                    smap=self.smap.cpu().numpy())
+
+    def raster(self, node_count, capture=True):
+        dev = self.xyz.device
+        ppll = D.PPLL(self.w, self.h, node_count, dev)
+        ppll.clear()
+        cap = None
+        if capture:
+            cap = dict(segment=torch.full((node_count,), -1, dtype=torch.int32, device=dev),
+                       pixel=torch.empty(node_count, dtype=torch.int32, device=dev),
+                       color=torch.empty((node_count, 4), dtype=torch.float32, device=dev))
+        D.raster_strands(self.xyz, self.tan, self.th, self.idx, self.cam, self.light, self.sp, self.rp, self.vol,
+                         self.res, self.smap, ppll, capture=cap)
+        return ppll, cap
+
+
+def _fragment_parity(s: Scene, ppll, cap, pairs=None):
+    n = min(ppll.used(), ppll.node_count)
+    seg = cap["segment"][:n].cpu().numpy().view(np.uint32)
+    pix = cap["pixel"][:n].cpu().numpy().view(np.uint32)
+    col = cap["color"][:n].cpu().numpy().astype(np.float64)
+    nodes = ppll.nodes.cpu().numpy().view(np.uint32).reshape(-1, 3)[:n]
+    ni = s.np_inputs()
+    o = ORs.raster(ni["xyz"], ni["tan"], ni["th"], ni["idx"], s.cam, s.light, s.sp, s.rp, density=ni["dens"],
+                   res=s.res, shadow_map=ni["smap"], pairs=pairs)
+    kept = o["kept"]
+    o_key = o["segment"][kept].astype(np.uint64) << np.uint64(32) | o["pixel"][kept].astype(np.uint64)
+    sel = np.ones(n, bool) if pairs is None else np.isin(seg, np.asarray(pairs, np.uint32))
+    g_key = seg[sel].astype(np.uint64) << np.uint64(32) | pix[sel].astype(np.uint64)
+    assert len(np.unique(g_key)) == len(g_key), "a (segment, pixel) fragment inserted twice"
+    common, gi, oi = np.intersect1d(g_key, o_key, return_indices=True)
+    missing, extra = len(o_key) - len(common), len(g_key) - len(common)
+    assert missing + extra <= max(2, 1e-4 * len(o_key)), (missing, extra, len(o_key))
+    g_col = col[sel][gi]
+    o_col = o["rgba"][kept][oi]
+    err = float(np.abs(g_col - o_col).max()) if len(common) else 0.0
+    assert err <= FRAG_TOL, err
+    derr = float(np.abs(nodes[sel][gi, 1].view(np.float32).astype(np.float64) - o["depth"][kept][oi]).max())
+    assert derr <= 1e-6, derr
+    # the node colour is packUnorm4x8 of the captured colour
+    assert np.array_equal(nodes[sel][:, 0], ORs.pack_unorm4x8(col[sel]))
+    return dict(fragments=len(o_key), matched=len(common), missing=missing, extra=extra, max_err=err,
+                max_depth_err=derr)
+
+
+def _resolve_parity(ppll, background=None):
+    out = ppll.resolve(background=background).cpu().numpy().astype(np.float64)
+    heads = ppll.heads.cpu().numpy().view(np.uint32)
+    nodes = ppll.nodes.cpu().numpy().view(np.uint32)
+    used = min(ppll.used(), ppll.node_count)
+    want = ORs.resolve(heads, nodes[:3 * max(used, 1)], ppll.width, ppll.height,
+                       background=None if background is None else background.cpu().numpy())
+    err = float(np.abs(out - want).max())
+    assert err <= RESOLVE_TOL, err
+    return err
+
+
+@pytest.fixture(scope="module")
+def c0(device):
+    return Scene(synth.config0(), (128, 128, 128), 512, 512, device, azimuth=30.0)
+
+
+def test_shadow_map_matches_oracle(c0):
+    ni = c0.np_inputs()
+    want = ORs.shadow_depth(ni["xyz"], ni["idx"], _cm(c0.sc.light_vp), 1.0, (1024, 1024))
+    got = ni["smap"]
+    differ = got != want
+    assert differ.mean() < 1e-4, differ.sum()
+    assert (got < 1).sum() > 10000
+
+
+def test_config0_fragments_and_resolve(c0):
+    ppll, cap = c0.raster(1 << 22)
+    assert ppll.used() < ppll.node_count
+    rep = _fragment_parity(c0, ppll, cap)
+    print("config0 raster", rep)
+    assert rep["fragments"] > 100000
+    # every node is reachable from exactly one head
+    heads = ppll.heads.cpu().numpy().view(np.uint32)
+    nodes = ppll.nodes.cpu().numpy().view(np.uint32).reshape(-1, 3)
+    seen = np.zeros(ppll.used(), np.int32)
+    for h0 in heads[heads != NULL][:2000]:
+        h = h0
+        while h != NULL:
+            seen[h] += 1
+            h = nodes[h, 2]
+    assert seen.max() <= 1
+    err = _resolve_parity(ppll)
+    print("resolve max err", err)
+
+
+def test_background_and_variants(c0, device):
+    bg = torch.rand((c0.h, c0.w, 4), device=device)
+    for rp in [params.render(renderer=0, shading_model=2), params.render(renderer=0, shading_model=3),
+               params.render(renderer=3, lod_magnified_distance=0.0, lod_minified_distance=100.0)]:
+        c0.rp = rp
+        ppll, cap = c0.raster(1 << 22)
+        rng = np.random.default_rng(2)
+        pairs = rng.choice(c0.idx.shape[0] // 2, 3000, replace=False)
+        _fragment_parity(c0, ppll, cap, pairs=pairs)
+        _resolve_parity(ppll, background=bg)
+    c0.rp = params.render(renderer=0)
+
+
+def test_pool_overflow_drops_without_out_of_bounds(c0, device):
+    n = 5000
+    guard = 64
+    ppll = D.PPLL(c0.w, c0.h, n, device)
+    ppll.nodes = torch.full((3 * (n + guard),), 0x5A5A5A5A, dtype=torch.int32, device=device)
+    ppll.c.nodes = ppll.nodes.data_ptr()
+    ppll.clear()
+    D.raster_strands(c0.xyz, c0.tan, c0.th, c0.idx, c0.cam, c0.light, c0.sp, c0.rp, c0.vol, c0.res, c0.smap, ppll)
+    assert ppll.used() > n
+    nodes = ppll.nodes.cpu().numpy()
+    assert (nodes[3 * n:] == 0x5A5A5A5A).all(), "write past the node pool"
+    heads = ppll.heads.cpu().numpy().view(np.uint32)
+    live = heads[heads != NULL]
+    assert len(live) and (live < n).all()
+    ppll.resolve()
+
+
+def test_hybrid_raymarch_and_strands_share_the_ppll(c0, device):
+    """Raymarched fragments go into the same list as the strands (the
+    reference's subpass 0 strands + subpass 1 raymarch), then one resolve."""
+    ppll, _ = c0.raster(1 << 22, capture=False)
+    n_strand = ppll.used()
+    rp = params.render(raycast_steps=256.0)
+    D.raymarch_ppll(c0.vol, c0.res, c0.cam, c0.light, c0.sp, rp, ppll)
+    rm = D.raymarch(c0.vol, c0.res, c0.cam, c0.light, c0.sp, rp, c0.w, c0.h, want_info=True)
+    frags = (rm["info"][..., 0] == 1)
+    assert ppll.used() - n_strand == int(frags.sum())
+    # the raymarch nodes carry packUnorm4x8 of the raymarch colour
+    nodes = ppll.nodes.cpu().numpy().view(np.uint32).reshape(-1, 3)[n_strand:ppll.used()]
+    rgba = rm["rgba"].reshape(-1, 4).cpu().numpy()
+    heads = ppll.heads.cpu().numpy().view(np.uint32)
+    pix = np.nonzero(frags.reshape(-1).cpu().numpy())[0]
+    head_nodes = heads[pix]
+    is_rm = head_nodes >= n_strand                     # raymarch node inserted last: it is the head
+    assert is_rm.all()
+    assert np.array_equal(nodes[head_nodes - n_strand, 0], ORs.pack_unorm4x8(rgba[pix]))
+    _resolve_parity(ppll)
+
+
+def test_config1_full_scale_sampled(device):
+    """136,320 strands at 1920x1080, 1024^2 shadow map, k=16 and a 32M-node
+    pool; the oracle checks 4,000 random segments and the whole resolve."""
+    s = Scene(synth.config1(), (256, 256, 256), 1920, 1080, device)
+    ppll, cap = s.raster(1 << 25)
+    assert ppll.used() < ppll.node_count
+    rng = np.random.default_rng(5)
+    pairs = rng.choice(s.idx.shape[0] // 2, 4000, replace=False)
+    rep = _fragment_parity(s, ppll, cap, pairs=pairs)
+    print("config1 raster", rep, "nodes", ppll.used())
+    err = _resolve_parity(ppll)
+    print("config1 resolve", err)

@@ -64,6 +64,15 @@ class Volume(C.Structure):
                 ("reserved_", C.c_uint32)]
 
 
+class Ppll(C.Structure):
+    _fields_ = [("heads", C.c_void_p), ("nodes", C.c_void_p), ("counter", C.c_void_p), ("node_count", C.c_uint32),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("reserved_", C.c_uint32)]
+
+
+class FragCapture(C.Structure):
+    _fields_ = [("segment", C.c_void_p), ("pixel", C.c_void_p), ("color", C.c_void_p)]
+
+
 P = C.c_void_p
 U32 = C.c_uint32
 I32 = C.c_int
@@ -80,6 +89,14 @@ SIGNATURES = {
     "vkhr_volume_finalize": [P, P, U32, U32, U32, I32, P, P, P, P, P],
     "vkhr_raymarch": [C.POINTER(Volume), C.POINTER(Camera), C.POINTER(Light), C.POINTER(StrandParams),
                       C.POINTER(RenderParams), U32, U32, P, P, P, P, P],
+    "vkhr_raymarch_ppll": [C.POINTER(Volume), C.POINTER(Camera), C.POINTER(Light), C.POINTER(StrandParams),
+                           C.POINTER(RenderParams), U32, U32, P, C.POINTER(Ppll), P],
+    "vkhr_ppll_clear": [C.POINTER(Ppll), P],
+    "vkhr_shadow_depth": [P, U32, P, U32, P, C.c_float, U32, U32, P, P],
+    "vkhr_raster_strands": [P, P, P, U32, P, U32, C.POINTER(Camera), C.POINTER(Light), C.POINTER(StrandParams),
+                            C.POINTER(RenderParams), C.POINTER(Volume), P, U32, U32, P, C.POINTER(Ppll),
+                            C.POINTER(FragCapture), P],
+    "vkhr_ppll_resolve": [C.POINTER(Ppll), P, P, P],
 }
 
 

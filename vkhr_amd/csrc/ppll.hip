@@ -1,0 +1,150 @@
+// Per-pixel linked list: clear and the k-buffer resolve
+// (share/shaders/transparency/resolve.comp:15-102), one thread per pixel.
+// The K=16 slots live in registers: every slot access is an unrolled
+// compare/select, so no private array spills to scratch memory.
+#include "render.hpp"
+
+namespace vkhr {
+namespace {
+
+constexpr int kK = 16;
+constexpr int kMaxFragments = 1024;
+constexpr float kMaxDepth = 3.e+38f;
+
+__global__ void __launch_bounds__(256)
+ppll_clear_kernel(uint32_t* __restrict__ heads, size_t n, uint32_t* __restrict__ counter) {
+    size_t stride = size_t(gridDim.x) * blockDim.x;
+    size_t i0 = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i0 == 0) *counter = 0u;
+    for (size_t i = i0; i < n; i += stride) heads[i] = kPpllNull;
+}
+
+__device__ __forceinline__ float4 unpack_unorm4x8(uint32_t c) {
+    return make_float4(float(c & 0xffu) / 255.0f, float((c >> 8) & 0xffu) / 255.0f,
+                       float((c >> 16) & 0xffu) / 255.0f, float(c >> 24) / 255.0f);
+}
+
+// mix(x, y, a) = x * (1 - a) + y * a, on all four channels
+__device__ __forceinline__ float4 blend(float4 x, float4 y) {
+    float a = y.w;
+    return make_float4(x.x * (1.0f - a) + y.x * a, x.y * (1.0f - a) + y.y * a, x.z * (1.0f - a) + y.z * a,
+                       x.w * (1.0f - a) + y.w * a);
+}
+
+// Furthest slot: strictly greater than the running maximum, which starts at
+// 0 with index 0 (resolve.comp:48-57, 81-90).
+__device__ __forceinline__ int furthest(const float (&d)[kK], float& best) {
+    best = 0.0f;
+    int idx = 0;
+#pragma unroll
+    for (int k = 0; k < kK; ++k)
+        if (d[k] > best) { best = d[k]; idx = k; }
+    return idx;
+}
+
+__global__ void __launch_bounds__(256)
+ppll_resolve_kernel(const uint32_t* __restrict__ heads, const uint32_t* __restrict__ nodes, uint32_t node_count,
+                    int width, int height, const float4* __restrict__ background, float4* __restrict__ out) {
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    if (px >= width || py >= height) return;
+    const size_t pix = size_t(py) * width + px;
+    float4 resolved = background ? background[pix] : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    uint32_t head = heads[pix];
+    if (head == kPpllNull) {
+        out[pix] = resolved;
+        return;
+    }
+    float depth[kK];
+    uint32_t color[kK];
+#pragma unroll
+    for (int k = 0; k < kK; ++k) {
+        depth[k] = kMaxDepth;
+        color[k] = 0u;
+    }
+    // the K newest nodes (a node index past the pool ends the list)
+#pragma unroll
+    for (int k = 0; k < kK; ++k) {
+        if (head != kPpllNull && head < node_count) {
+            const uint32_t* n = nodes + 3ull * head;
+            color[k] = n[0];
+            depth[k] = __uint_as_float(n[1]);
+            head = n[2];
+        } else {
+            head = kPpllNull;
+        }
+    }
+    // tail: swap closer nodes into the k-buffer, blend the evicted ones
+    for (int f = 0; f < kMaxFragments; ++f) {
+        if (head == kPpllNull || head >= node_count) break;
+        float far_depth;
+        int far_idx = furthest(depth, far_depth);
+        const uint32_t* n = nodes + 3ull * head;
+        uint32_t fc = n[0];
+        float fd = __uint_as_float(n[1]);
+        uint32_t next = n[2];
+        if (fd < far_depth) {
+#pragma unroll
+            for (int k = 0; k < kK; ++k) {
+                if (k == far_idx) {
+                    uint32_t tc = color[k];
+                    float td = depth[k];
+                    color[k] = fc;
+                    depth[k] = fd;
+                    fc = tc;
+                    fd = td;
+                }
+            }
+        }
+        resolved = blend(resolved, unpack_unorm4x8(fc));
+        head = next;
+    }
+    // the k-buffer, back to front
+#pragma unroll
+    for (int r = 0; r < kK; ++r) {
+        float far_depth;
+        int far_idx = furthest(depth, far_depth);
+        uint32_t c = 0u;
+#pragma unroll
+        for (int k = 0; k < kK; ++k) {
+            if (k == far_idx) {
+                depth[k] = 0.0f;
+                c = color[k];
+            }
+        }
+        resolved = blend(resolved, unpack_unorm4x8(c));
+    }
+    out[pix] = resolved;
+}
+
+}  // namespace
+}  // namespace vkhr
+
+using namespace vkhr;
+
+extern "C" int vkhr_ppll_clear(const vkhr_ppll* ppll, vkhr_stream stream) {
+    clear_error();
+    int rc = ensure_device();
+    if (rc) return rc;
+    VKHR_REQUIRE(ppll && ppll->heads && ppll->counter, VKHR_ERR_INVALID, "ppll is incomplete");
+    VKHR_REQUIRE(ppll->width && ppll->height, VKHR_ERR_INVALID, "empty ppll");
+    size_t n = size_t(ppll->width) * ppll->height;
+    ppll_clear_kernel<<<grid_for(n, 256, 8192), 256, 0, as_stream(stream)>>>(ppll->heads, n, ppll->counter);
+    VKHR_CHECK_LAUNCH();
+    return VKHR_OK;
+}
+
+extern "C" int vkhr_ppll_resolve(const vkhr_ppll* ppll, const float* background, float* out, vkhr_stream stream) {
+    clear_error();
+    int rc = ensure_device();
+    if (rc) return rc;
+    VKHR_REQUIRE(ppll && ppll->heads && ppll->nodes && out, VKHR_ERR_INVALID, "null argument");
+    VKHR_REQUIRE(ppll->width && ppll->height, VKHR_ERR_INVALID, "empty ppll");
+    dim3 grid((ppll->width + 15) / 16, (ppll->height + 15) / 16);
+    ppll_resolve_kernel<<<grid, 256, 0, as_stream(stream)>>>(
+        ppll->heads, ppll->nodes, ppll->node_count, int(ppll->width), int(ppll->height),
+        reinterpret_cast<const float4*>(background), reinterpret_cast<float4*>(out));
+    VKHR_CHECK_LAUNCH();
+    return VKHR_OK;
+}

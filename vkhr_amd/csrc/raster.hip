@@ -1,0 +1,434 @@
+// Strand rasterizer: the reference's line passes (shadow-map depth pass and
+// strand colour pass into the PPLL) as gfx950 kernels.
+//
+// The reference draws LINE_LIST primitives with a dynamic line width and
+// lets the Vulkan rasterizer pick the fragments.  Here one thread owns one
+// index pair: it clips the segment to 0 <= z <= w, maps it to window
+// coordinates, and walks candidate pixels along its major axis -- every
+// column the segment's rectangle touches, a few rows around the line --
+// testing each candidate with the exact rule
+//     0 <= (c-a).(b-a) < |b-a|^2   and   cross(b-a, c-a)^2 <= (w/2)^2 |b-a|^2
+// in float64 (declared in include/vkhr_hip.h).  Candidates are enumerated by
+// index so that all lanes of a wave stay in one loop and can ballot together
+// for the PPLL node allocation (one counter atomic per wave and round).
+// Window depth interpolates linearly in t, varyings perspective-correctly.
+// Compiled with -ffp-contract=off so the rule evaluates as the float64
+// oracle evaluates it.
+#include "render.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace vkhr {
+namespace {
+
+constexpr int kMaxPcfTaps = 81;  // kernel sizes up to 9x9
+
+struct LineSetup {
+    double ax, ay, az, bx, by, bz;   // window coordinates of the clipped ends
+    double wa, wb;                   // clip w of the clipped ends
+    double s0, s1;                   // clip interval along the original segment
+    double vx, vy, l2;               // b - a, |b - a|^2
+    double h;                        // half-height of the strip across the minor axis
+    int col0, ncols, nrows;          // candidate walk
+    bool xmajor, valid;
+};
+
+struct RasterArgs {
+    double vp[16];                   // view-projection (column-major)
+    double lm[16];                   // light->matrix (biased light view-projection)
+    double eye[3], light_origin[3], hair_color[3], intensity[3];
+    double alpha, shininess, lod, line_width, res_x, res_y;
+    double ln_transmit;              // log(1 - alpha)
+    double pcf_dx[kMaxPcfTaps], pcf_dy[kMaxPcfTaps], pcf_w[kMaxPcfTaps];
+    int pcf_taps;
+    double ao_scaling, ao_clamp, ao_exponent;
+    double vorigin[3], vsize[3], vres[3];
+    DensityVolume vol;
+    int shading_model, adsm_on;
+    int width, height;               // target
+    const float* shadow_map;
+    int sw, sh;
+    const float* depth_buffer;
+    const float* xyz;
+    const float* tangents;
+    const float* thickness;
+    const uint32_t* indices;
+    uint32_t n_pairs, n_vertices;
+    PpllView ppll;
+    CaptureView cap;
+    uint32_t* depth_bits;            // shadow pass output (float bits, min)
+};
+
+__device__ __forceinline__ void xform4(const double* m, double x, double y, double z, double o[4]) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) o[r] = m[0 * 4 + r] * x + m[1 * 4 + r] * y + m[2 * 4 + r] * z + m[3 * 4 + r];
+}
+
+// Clip, project and plan the candidate walk of one segment.
+__device__ LineSetup setup_line(const double* m, const double pa[3], const double pb[3], int width, int height,
+                                double line_width) {
+    LineSetup L;
+    L.valid = false;
+    double ca[4], cb[4];
+    xform4(m, pa[0], pa[1], pa[2], ca);
+    xform4(m, pb[0], pb[1], pb[2], cb);
+    double s0 = 0.0, s1 = 1.0;
+    // planes z >= 0 and w - z >= 0
+    double fa[2] = {ca[2], ca[3] - ca[2]}, fb[2] = {cb[2], cb[3] - cb[2]};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        if (fa[k] < 0.0 && fb[k] < 0.0) return L;
+        if (fa[k] < 0.0) s0 = fmax(s0, fa[k] / (fa[k] - fb[k]));
+        if (fb[k] < 0.0) s1 = fmin(s1, fa[k] / (fa[k] - fb[k]));
+    }
+    if (!(s0 < s1)) return L;
+    double c0[4], c1[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        c0[r] = ca[r] + s0 * (cb[r] - ca[r]);
+        c1[r] = ca[r] + s1 * (cb[r] - ca[r]);
+    }
+    if (!(c0[3] > 0.0 && c1[3] > 0.0)) return L;
+    L.s0 = s0; L.s1 = s1;
+    L.wa = c0[3]; L.wb = c1[3];
+    L.ax = (c0[0] / c0[3] + 1.0) * (0.5 * width);
+    L.ay = (c0[1] / c0[3] + 1.0) * (0.5 * height);
+    L.az = c0[2] / c0[3];
+    L.bx = (c1[0] / c1[3] + 1.0) * (0.5 * width);
+    L.by = (c1[1] / c1[3] + 1.0) * (0.5 * height);
+    L.bz = c1[2] / c1[3];
+    L.vx = L.bx - L.ax;
+    L.vy = L.by - L.ay;
+    L.l2 = L.vx * L.vx + L.vy * L.vy;
+    if (!(L.l2 > 0.0) || !(L.l2 < 1e30)) return L;
+    L.xmajor = fabs(L.vx) >= fabs(L.vy);
+    double hw = 0.5 * line_width;
+    double major_a = L.xmajor ? L.ax : L.ay, major_b = L.xmajor ? L.bx : L.by;
+    double vmaj = L.xmajor ? L.vx : L.vy;
+    int extent = L.xmajor ? width : height;
+    double lo = fmin(major_a, major_b) - hw, hi = fmax(major_a, major_b) + hw;
+    double c_lo = floor(lo - 0.5), c_hi = ceil(hi - 0.5);
+    c_lo = fmax(c_lo, 0.0);
+    c_hi = fmin(c_hi, double(extent - 1));
+    if (c_lo > c_hi) return L;
+    L.col0 = int(c_lo);
+    L.ncols = int(c_hi - c_lo) + 1;
+    // rows: the strip |minor - line(major)| <= hw * |v| / |v_major|, plus a margin
+    L.h = hw * sqrt(L.l2) / fabs(vmaj);
+    L.nrows = int(floor(2.0 * L.h)) + 3;
+    L.valid = true;
+    return L;
+}
+
+// Candidate k of a segment -> pixel (px, py); false when outside the target.
+__device__ __forceinline__ bool candidate(const LineSetup& L, int k, int width, int height, int& px, int& py) {
+    int col = L.col0 + k / L.nrows;
+    int r = k % L.nrows;
+    double cm = col + 0.5;
+    double line_minor;
+    if (L.xmajor) line_minor = L.ay + (cm - L.ax) * (L.vy / L.vx);
+    else line_minor = L.ax + (cm - L.ay) * (L.vx / L.vy);
+    int row = int(floor(line_minor - L.h - 0.5)) - 1 + r;
+    if (L.xmajor) { px = col; py = row; }
+    else { px = row; py = col; }
+    return px >= 0 && py >= 0 && px < width && py < height;
+}
+
+// The line rule at pixel centre (px+0.5, py+0.5); t returned on success.
+__device__ __forceinline__ bool covers(const LineSetup& L, int px, int py, double line_width, double& t) {
+    double cx = px + 0.5 - L.ax, cy = py + 0.5 - L.ay;
+    double dp = cx * L.vx + cy * L.vy;
+    if (!(dp >= 0.0 && dp < L.l2)) return false;
+    double cr = L.vx * cy - L.vy * cx;
+    double hw = 0.5 * line_width;
+    if (!(cr * cr <= hw * hw * L.l2)) return false;
+    t = dp / L.l2;
+    return true;
+}
+
+// Bilinear, clamp-to-edge sample of the shadow map at uv.
+__device__ __forceinline__ double shadow_tex(const RasterArgs& a, double u, double v) {
+    double x = u * a.sw - 0.5, y = v * a.sh - 0.5;
+    if (!(fabs(x) < 1e9 && fabs(y) < 1e9)) return double(a.shadow_map[0]);
+    double fx = floor(x), fy = floor(y);
+    double wx = x - fx, wy = y - fy;
+    int x0 = int(fx), y0 = int(fy);
+    int x1 = min(max(x0 + 1, 0), a.sw - 1), y1 = min(max(y0 + 1, 0), a.sh - 1);
+    x0 = min(max(x0, 0), a.sw - 1);
+    y0 = min(max(y0, 0), a.sh - 1);
+    double t00 = a.shadow_map[size_t(y0) * a.sw + x0], t10 = a.shadow_map[size_t(y0) * a.sw + x1];
+    double t01 = a.shadow_map[size_t(y1) * a.sw + x0], t11 = a.shadow_map[size_t(y1) * a.sw + x1];
+    double r0 = t00 + wx * (t10 - t00), r1 = t01 + wx * (t11 - t01);
+    return r0 + wy * (r1 - r0);
+}
+
+template <bool kColor>
+__global__ void __launch_bounds__(256)
+raster_kernel(RasterArgs a) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    LineSetup L;
+    L.valid = false;
+    uint32_t ia = 0, ib = 0;
+    double pa[3] = {0, 0, 0}, pb[3] = {0, 0, 0};
+    if (s < a.n_pairs) {
+        ia = a.indices[2u * s];
+        ib = a.indices[2u * s + 1u];
+        if (ia < a.n_vertices && ib < a.n_vertices) {
+            float3 fa = load_xyz(a.xyz, ia), fb = load_xyz(a.xyz, ib);
+            pa[0] = fa.x; pa[1] = fa.y; pa[2] = fa.z;
+            pb[0] = fb.x; pb[1] = fb.y; pb[2] = fb.z;
+            L = setup_line(kColor ? a.vp : a.lm, pa, pb, a.width, a.height, a.line_width);
+        }
+    }
+    const int total = L.valid ? L.ncols * L.nrows : 0;
+    // per-segment attribute ends (clipped)
+    double ta[3] = {0, 0, 0}, tb[3] = {0, 0, 0}, tha = 0, thb = 0;
+    if (kColor && L.valid) {
+        float3 ua = load_xyz(a.tangents, ia), ub = load_xyz(a.tangents, ib);
+        double t0[3] = {ua.x, ua.y, ua.z}, t1[3] = {ub.x, ub.y, ub.z};
+        double h0 = a.thickness[ia], h1 = a.thickness[ib];
+        double q0[3], q1[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            ta[c] = t0[c] + L.s0 * (t1[c] - t0[c]);
+            tb[c] = t0[c] + L.s1 * (t1[c] - t0[c]);
+            q0[c] = pa[c] + L.s0 * (pb[c] - pa[c]);
+            q1[c] = pa[c] + L.s1 * (pb[c] - pa[c]);
+        }
+        tha = h0 + L.s0 * (h1 - h0);
+        thb = h0 + L.s1 * (h1 - h0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) { pa[c] = q0[c]; pb[c] = q1[c]; }
+    }
+    for (int k = 0;; ++k) {
+        const bool live = k < total;
+        if (!__any(live)) break;
+        int px = 0, py = 0;
+        double t = 0.0;
+        bool frag = live && candidate(L, k, a.width, a.height, px, py) && covers(L, px, py, a.line_width, t);
+        const uint32_t pix = uint32_t(py) * uint32_t(a.width) + uint32_t(px);
+        const double z = L.az + t * (L.bz - L.az);
+        if (!kColor) {
+            if (frag) atomicMin(&a.depth_bits[pix], __float_as_uint(float(z)));
+            continue;
+        }
+        // early depth test (LESS against the subpass depth, no writes)
+        if (frag) frag = float(z) < (a.depth_buffer ? a.depth_buffer[pix] : 1.0f);
+        float4 color = make_float4(0, 0, 0, 0);
+        if (frag) {
+            // perspective-correct varyings
+            double iw = (1.0 - t) / L.wa + t / L.wb;
+            double sp = (t / L.wb) / iw;
+            double P[3], T[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                P[c] = pa[c] + sp * (pb[c] - pa[c]);
+                T[c] = ta[c] + sp * (tb[c] - ta[c]);
+            }
+            double thick = tha + sp * (thb - tha);
+            // gpaa (gpaa.glsl:33-51)
+            double cl[4];
+            xform4(a.vp, P[0], P[1], P[2], cl);
+            double sx = (cl[0] / cl[3] + 1.0) * (a.res_x / 2.0), sy = (cl[1] / cl[3] + 1.0) * (a.res_y / 2.0);
+            double ddx = sx - (px + 0.5), ddy = sy - (py + 0.5);
+            double coverage = 1.0 - sqrt(ddx * ddx + ddy * ddy) / (a.line_width / 2.0);
+            coverage *= a.alpha;
+            frag = !(coverage < 0.001);
+            if (frag) {
+                coverage *= 1.0 - a.lod;
+                coverage *= thick * (1.0 / 0.042);
+                double e[3] = {P[0] - a.eye[0], P[1] - a.eye[1], P[2] - a.eye[2]};
+                double l[3] = {a.light_origin[0] - P[0], a.light_origin[1] - P[1], a.light_origin[2] - P[2]};
+                normalize3(e);
+                normalize3(l);
+                double shading[3] = {1.0, 1.0, 1.0};
+                if (a.shading_model == 0) kajiya_kay(a.hair_color, a.intensity, a.shininess, T, l, e, shading);
+                double occlusion = 1.0;
+                if (a.adsm_on == 1 && a.shading_model != 3) {
+                    double ls[4];
+                    xform4(a.lm, P[0], P[1], P[2], ls);
+                    double light_depth = ls[2] / ls[3];
+                    double u0 = ls[0] / ls[3], v0 = ls[1] / ls[3];
+                    double vis = 0.0, tot = 0.0;
+                    for (int q = 0; q < a.pcf_taps; ++q) {
+                        double sd = shadow_tex(a, u0 + a.pcf_dx[q], v0 + a.pcf_dy[q]);
+                        double depth = fmax(light_depth - sd, 0.0);
+                        double count = depth * 15000.0;
+                        if (depth > 1e-5) count += 1.0;
+                        vis += exp(count * a.ln_transmit) * a.pcf_w[q];
+                        tot += a.pcf_w[q];
+                    }
+                    occlusion *= vis / tot;
+                }
+                if (a.shading_model != 2) {
+                    double st[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) st[c] = (P[c] - a.vorigin[c]) / a.vsize[c] * a.vres[c] - 0.5;
+                    occlusion *= local_ambient_occlusion(a.vol, st, a.ao_scaling, a.ao_clamp, a.ao_exponent);
+                }
+                color = make_float4(float(shading[0] * occlusion), float(shading[1] * occlusion),
+                                    float(shading[2] * occlusion), float(coverage));
+            }
+        }
+        uint32_t node = ppll_insert(a.ppll, frag, pix, color, float(z));
+        if (node != kPpllNull) {
+            if (a.cap.segment) a.cap.segment[node] = s;
+            if (a.cap.pixel) a.cap.pixel[node] = pix;
+            if (a.cap.color) a.cap.color[node] = color;
+        }
+    }
+}
+
+void load_cm(const float* m, double* out) {
+    for (int i = 0; i < 16; ++i) out[i] = m[i];
+}
+
+void matmul_cm(const float* x, const float* y, double* out) {
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) {
+            double acc = 0.0;
+            for (int k = 0; k < 4; ++k) acc += double(x[k * 4 + r]) * double(y[c * 4 + k]);
+            out[c * 4 + r] = acc;
+        }
+}
+
+__global__ void fill_depth_kernel(uint32_t* bits, size_t n, uint32_t v) {
+    size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) bits[i] = v;
+}
+
+}  // namespace
+}  // namespace vkhr
+
+using namespace vkhr;
+
+extern "C" int vkhr_shadow_depth(const float* xyz, uint32_t n_vertices, const uint32_t* indices,
+                                 uint32_t n_indices, const float* light_vp, float line_width, uint32_t size_w,
+                                 uint32_t size_h, float* depth_map, vkhr_stream stream) {
+    clear_error();
+    int rc = ensure_device();
+    if (rc) return rc;
+    VKHR_REQUIRE(light_vp && depth_map, VKHR_ERR_INVALID, "null argument");
+    VKHR_REQUIRE(size_w && size_h && size_w <= 16384 && size_h <= 16384, VKHR_ERR_INVALID, "bad map size");
+    VKHR_REQUIRE(line_width > 0.0f && line_width < 64.0f, VKHR_ERR_INVALID, "line width %g", line_width);
+    uint32_t n_pairs = n_indices / 2;
+    VKHR_REQUIRE(n_pairs == 0 || (xyz && indices), VKHR_ERR_INVALID, "null geometry");
+    hipStream_t st = as_stream(stream);
+    size_t n = size_t(size_w) * size_h;
+    fill_depth_kernel<<<grid_for(n, 256, 8192), 256, 0, st>>>(reinterpret_cast<uint32_t*>(depth_map), n,
+                                                             __builtin_bit_cast(uint32_t, 1.0f));
+    VKHR_CHECK_LAUNCH();
+    if (n_pairs == 0) return VKHR_OK;
+    RasterArgs* a = new RasterArgs();
+    std::memset(a, 0, sizeof(RasterArgs));
+    load_cm(light_vp, a->lm);
+    a->line_width = line_width;
+    a->width = int(size_w);
+    a->height = int(size_h);
+    a->xyz = xyz;
+    a->indices = indices;
+    a->n_pairs = n_pairs;
+    a->n_vertices = n_vertices;
+    a->depth_bits = reinterpret_cast<uint32_t*>(depth_map);
+    raster_kernel<false><<<(n_pairs + 255) / 256, 256, 0, st>>>(*a);
+    delete a;
+    VKHR_CHECK_LAUNCH();
+    return VKHR_OK;
+}
+
+extern "C" int vkhr_raster_strands(const float* xyz, const float* tangents, const float* thickness,
+                                   uint32_t n_vertices, const uint32_t* indices, uint32_t n_indices,
+                                   const vkhr_camera* cam, const vkhr_light* light, const vkhr_strand_params* sp,
+                                   const vkhr_render_params* rp, const vkhr_volume* vol, const float* shadow_map,
+                                   uint32_t shadow_w, uint32_t shadow_h, const float* depth_buffer,
+                                   const vkhr_ppll* ppll, const vkhr_frag_capture* capture, vkhr_stream stream) {
+    clear_error();
+    int rc = ensure_device();
+    if (rc) return rc;
+    VKHR_REQUIRE(cam && light && sp && rp && ppll, VKHR_ERR_INVALID, "null parameter struct");
+    VKHR_REQUIRE(ppll->heads && ppll->nodes && ppll->counter, VKHR_ERR_INVALID, "ppll is incomplete");
+    uint32_t n_pairs = n_indices / 2;
+    VKHR_REQUIRE(n_pairs == 0 || (xyz && indices && tangents && thickness), VKHR_ERR_INVALID, "null geometry");
+    const bool adsm = rp->adsm_on == 1 && rp->shading_model != 3;
+    const bool lao = rp->shading_model != 2;
+    VKHR_REQUIRE(!adsm || (shadow_map && shadow_w && shadow_h), VKHR_ERR_INVALID, "ADSM needs a shadow map");
+    VKHR_REQUIRE(!lao || (vol && (vol->cells || vol->density_f32) && vol->nx && vol->ny && vol->nz),
+                 VKHR_ERR_INVALID, "LAO needs the density volume");
+    VKHR_REQUIRE(sp->strand_radius > 0.0f && sp->strand_radius < 64.0f, VKHR_ERR_INVALID, "strand_radius %g",
+                 sp->strand_radius);
+    if (n_pairs == 0) return VKHR_OK;
+    RasterArgs* a = new RasterArgs();
+    std::memset(a, 0, sizeof(RasterArgs));
+    matmul_cm(cam->projection, cam->view, a->vp);
+    load_cm(light->matrix, a->lm);
+    for (int c = 0; c < 3; ++c) {
+        a->eye[c] = cam->position[c];
+        a->light_origin[c] = light->origin[c];
+        a->hair_color[c] = sp->hair_color[c];
+        a->intensity[c] = light->intensity[c];
+        a->vorigin[c] = sp->volume_bounds.origin[c];
+        a->vsize[c] = sp->volume_bounds.size[c];
+    }
+    a->alpha = sp->hair_opacity;
+    a->shininess = sp->hair_shininess;
+    a->ln_transmit = std::log(1.0 - double(sp->hair_opacity));
+    a->lod = rp->renderer == 0 ? 0.0 : rp->renderer == 2 ? 1.0 : [&] {
+        double t = (double(cam->look_at_distance) - rp->lod_magnified_distance) /
+                   (double(rp->lod_minified_distance) - rp->lod_magnified_distance);
+        t = t < 0 ? 0 : (t > 1 ? 1 : t);
+        return t * t * (3.0 - 2.0 * t);
+    }();
+    a->line_width = sp->strand_radius;
+    a->res_x = cam->resolution[0];
+    a->res_y = cam->resolution[1];
+    // approximate_deep_shadows (approximate_deep_shadows.glsl:41-73): taps at
+    // (x, y) / (size / stride) with Gaussian weights; the shader's exponent
+    // reads -(x^2 + y^2) / 2 * sigma^2
+    if (adsm) {
+        double kw = rp->adsm_kernel_size, smoothing = rp->adsm_stride_size;
+        VKHR_REQUIRE(kw >= 1 && kw <= 9 && smoothing > 0, VKHR_ERR_INVALID, "ADSM kernel %g stride %g", kw,
+                     smoothing);
+        float kr = float((kw - 1.0) / 2.0);
+        double sigma = (kw / 2.0) / 2.4, sigma2 = sigma * sigma;
+        int q = 0;
+        for (float y = -kr; y <= kr; y += 1.0f)
+            for (float x = -kr; x <= kr; x += 1.0f) {
+                double exponent = -1.0 * (double(x) * x + double(y) * y) / 2.0 * sigma2;
+                a->pcf_w[q] = 1.0 / (2.0 * M_PI * sigma2) * std::pow(M_E, exponent);
+                a->pcf_dx[q] = double(x) / (double(shadow_w) / smoothing);
+                a->pcf_dy[q] = double(y) / (double(shadow_h) / smoothing);
+                ++q;
+            }
+        a->pcf_taps = q;
+    }
+    a->ao_scaling = double(rp->occlusion_radius) / 0.5;
+    a->ao_clamp = rp->ao_clamp;
+    a->ao_exponent = rp->ao_exponent;
+    if (vol) {
+        a->vol.cells = reinterpret_cast<const unsigned long long*>(vol->cells);
+        a->vol.f32 = vol->density_f32;
+        a->vol.nx = int(vol->nx); a->vol.ny = int(vol->ny); a->vol.nz = int(vol->nz);
+        a->vres[0] = vol->nx; a->vres[1] = vol->ny; a->vres[2] = vol->nz;
+    }
+    a->shading_model = rp->shading_model;
+    a->adsm_on = rp->adsm_on;
+    a->width = int(ppll->width);
+    a->height = int(ppll->height);
+    a->shadow_map = shadow_map;
+    a->sw = int(shadow_w);
+    a->sh = int(shadow_h);
+    a->depth_buffer = depth_buffer;
+    a->xyz = xyz;
+    a->tangents = tangents;
+    a->thickness = thickness;
+    a->indices = indices;
+    a->n_pairs = n_pairs;
+    a->n_vertices = n_vertices;
+    a->ppll = PpllView{ppll->heads, ppll->nodes, ppll->counter, ppll->node_count};
+    if (capture) a->cap = CaptureView{capture->segment, capture->pixel, reinterpret_cast<float4*>(capture->color)};
+    raster_kernel<true><<<(n_pairs + 255) / 256, 256, 0, as_stream(stream)>>>(*a);
+    delete a;
+    VKHR_CHECK_LAUNCH();
+    return VKHR_OK;
+}

@@ -17,7 +17,7 @@
 // Launch: 16x16-pixel blocks of 256 threads, each wave covering an 8x8 tile
 // (neighbouring rays march neighbouring texels, so gathers hit the same
 // cache lines).
-#include "common.hpp"
+#include "render.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -42,13 +42,13 @@ struct RaymarchArgs {
     int shading_model, adsm_on;
     int nx, ny, nz;
     int width, height;
-    const unsigned long long* cells;
-    const float* density_f32;
+    DensityVolume vol;
     const char4* tangents;
     const float* depth_buffer;
     float4* rgba;
     float* depth;
     int4* info;
+    PpllView ppll;             // node insertion when ppll.heads is set
 };
 
 // Texel-space coordinate of a world point along one axis: uvw * n - 0.5.
@@ -59,41 +59,6 @@ __device__ __forceinline__ double to_texel(double p, double o, double s, double 
 struct Texel3 {
     double x, y, z;
 };
-
-// Trilinear density at texel coordinate (x, y, z): exact weights, zero border.
-__device__ __forceinline__ float density_at(const RaymarchArgs& a, double x, double y, double z) {
-    if (!(x >= -1.0 && y >= -1.0 && z >= -1.0 && x < double(a.nx) && y < double(a.ny) && z < double(a.nz)))
-        return 0.0f;
-    double fx = floor(x), fy = floor(y), fz = floor(z);
-    float wx = float(x - fx), wy = float(y - fy), wz = float(z - fz);
-    int ix = int(fx), iy = int(fy), iz = int(fz);
-    float b[8];
-    if (a.density_f32) {
-#pragma unroll
-        for (int n = 0; n < 8; ++n) {
-            int tx = ix + (n & 1), ty = iy + ((n >> 1) & 1), tz = iz + (n >> 2);
-            bool in = tx >= 0 && ty >= 0 && tz >= 0 && tx < a.nx && ty < a.ny && tz < a.nz;
-            b[n] = in ? a.density_f32[size_t(tx) + size_t(a.nx) * (size_t(ty) + size_t(a.ny) * size_t(tz))] : 0.0f;
-        }
-    } else {
-        size_t cx = size_t(a.nx) + 1, cy = size_t(a.ny) + 1;
-        unsigned long long c =
-            a.cells[size_t(ix + 1) + cx * (size_t(iy + 1) + cy * size_t(iz + 1))];
-        uint32_t lo = uint32_t(c), hi = uint32_t(c >> 32);
-        b[0] = float(lo & 0xffu); b[1] = float((lo >> 8) & 0xffu);
-        b[2] = float((lo >> 16) & 0xffu); b[3] = float(lo >> 24);
-        b[4] = float(hi & 0xffu); b[5] = float((hi >> 8) & 0xffu);
-        b[6] = float((hi >> 16) & 0xffu); b[7] = float(hi >> 24);
-    }
-    float c00 = fmaf(wx, b[1] - b[0], b[0]);
-    float c10 = fmaf(wx, b[3] - b[2], b[2]);
-    float c01 = fmaf(wx, b[5] - b[4], b[4]);
-    float c11 = fmaf(wx, b[7] - b[6], b[6]);
-    float c0 = fmaf(wy, c10 - c00, c00);
-    float c1 = fmaf(wy, c11 - c01, c01);
-    float v = fmaf(wz, c1 - c0, c0);
-    return a.density_f32 ? v : v * (1.0f / 255.0f);
-}
 
 // Trilinear SNORM tangent (xyz) at a texel coordinate.
 __device__ __forceinline__ void tangent_at(const RaymarchArgs& a, double x, double y, double z, double out[3]) {
@@ -116,19 +81,14 @@ __device__ __forceinline__ void tangent_at(const RaymarchArgs& a, double x, doub
     }
 }
 
-__device__ __forceinline__ void normalize3(double v[3]) {
-    double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-    double s = l > 0.0 ? 1.0 / l : 0.0;
-    v[0] *= s; v[1] *= s; v[2] *= s;
-}
-
 __global__ void __launch_bounds__(256)
 raymarch_kernel(RaymarchArgs a) {
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
     const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
-    if (px >= a.width || py >= a.height) return;
-    const size_t pix = size_t(py) * a.width + px;
+    // Out-of-image lanes stay converged (the PPLL insert ballots the wave).
+    const bool inside = px < a.width && py < a.height;
+    const size_t pix = inside ? size_t(py) * a.width + px : 0;
 
     float4 color = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float out_depth = 1.0f;
@@ -155,7 +115,7 @@ raymarch_kernel(RaymarchArgs a) {
         t0 = fmax(t0, lo);
         t1 = fmin(t1, hi);
     }
-    hit = hit && t0 <= t1 && t0 > 0.0;
+    hit = inside && hit && t0 <= t1 && t0 > 0.0;
     double start[3], end[3];
     double zs = 0, ws = 1, dz = 0, dw = 0;
     if (hit) {
@@ -195,7 +155,7 @@ raymarch_kernel(RaymarchArgs a) {
             const double td = double(t);
             // depth test: db < z/w  <=>  db*w < z  (w > 0 in front of the eye)
             if (db * fma(td, dw, ws) < fma(td, dz, zs)) break;
-            float d = density_at(a, fma(td, dx[0], xs[0]), fma(td, dx[1], xs[1]), fma(td, dx[2], xs[2]));
+            float d = density_at(a.vol, fma(td, dx[0], xs[0]), fma(td, dx[1], xs[1]), fma(td, dx[2], xs[2]));
             acc += double(d);
             if (d != 0.0f) {
                 if (acc <= a.iso) { surf = i; t_surf = t; }
@@ -227,12 +187,7 @@ raymarch_kernel(RaymarchArgs a) {
                 double tg[3];
                 tangent_at(a, sx[0], sx[1], sx[2], tg);
                 normalize3(tg);
-                double ctl = tg[0] * l[0] + tg[1] * l[1] + tg[2] * l[2];
-                double cte = tg[0] * e[0] + tg[1] * e[1] + tg[2] * e[2];
-                double stl = sqrt(fmax(1.0 - ctl * ctl, 0.0)), ste = sqrt(fmax(1.0 - cte * cte, 0.0));
-                double spec = pow(fmax(ctl * cte + stl * ste, 0.0), a.shininess);
-#pragma unroll
-                for (int c = 0; c < 3; ++c) shading[c] = a.hair_color[c] * stl + a.intensity[c] * spec;
+                kajiya_kay(a.hair_color, a.intensity, a.shininess, tg, l, e, shading);
             }
             double occlusion = 1.0;
             if (a.adsm_on == 1 && a.shading_model != 3) {
@@ -242,23 +197,13 @@ raymarch_kernel(RaymarchArgs a) {
                 double strands = 0.0;
                 for (float t = 0.0f; t < 1.0f; t += a.step_size) {
                     const double td = double(t);
-                    strands += double(density_at(a, fma(td, xl[0], sx[0]), fma(td, xl[1], sx[1]),
+                    strands += double(density_at(a.vol, fma(td, xl[0], sx[0]), fma(td, xl[1], sx[1]),
                                                  fma(td, xl[2], sx[2])));
                 }
                 occlusion *= pow(1.0 - a.alpha, strands * 11.0);
             }
-            if (a.shading_model != 2) {
-                double dens = 0.0;
-#pragma unroll
-                for (int n = 0; n < 8; ++n) {
-                    double ox = ((n & 1) ? 0.5 : -0.5) * a.ao_scaling;
-                    double oy = (((n >> 1) & 1) ? 0.5 : -0.5) * a.ao_scaling;
-                    double oz = ((n >> 2) ? 0.5 : -0.5) * a.ao_scaling;
-                    double s = double(density_at(a, sx[0] + ox, sx[1] + oy, sx[2] + oz));
-                    dens += fmin(s, a.ao_clamp);
-                }
-                occlusion *= pow(1.0 - dens / 8.0, a.ao_exponent);
-            }
+            if (a.shading_model != 2)
+                occlusion *= local_ambient_occlusion(a.vol, sx, a.ao_scaling, a.ao_clamp, a.ao_exponent);
             double zz = a.vp_z[0] * sp[0] + a.vp_z[1] * sp[1] + a.vp_z[2] * sp[2] + a.vp_z[3];
             double ww = a.vp_w[0] * sp[0] + a.vp_w[1] * sp[1] + a.vp_w[2] * sp[2] + a.vp_w[3];
             out_depth = float(zz / ww);
@@ -268,7 +213,9 @@ raymarch_kernel(RaymarchArgs a) {
                                 float(fmin(fmax(coverage, 0.0), 1.0)));
         }
     }
-    a.rgba[pix] = color;
+    if (a.ppll.heads) ppll_insert(a.ppll, info.x == 1, uint32_t(pix), color, out_depth);
+    if (!inside) return;
+    if (a.rgba) a.rgba[pix] = color;
     if (a.depth) a.depth[pix] = out_depth;
     if (a.info) a.info[pix] = info;
 }
@@ -369,8 +316,9 @@ int make_raymarch_args(const vkhr_volume* vol, const vkhr_camera* cam, const vkh
     a->adsm_on = rp->adsm_on;
     a->width = int(width);
     a->height = int(height);
-    a->cells = reinterpret_cast<const unsigned long long*>(vol->cells);
-    a->density_f32 = vol->density_f32;
+    a->vol.cells = reinterpret_cast<const unsigned long long*>(vol->cells);
+    a->vol.f32 = vol->density_f32;
+    a->vol.nx = a->nx; a->vol.ny = a->ny; a->vol.nz = a->nz;
     a->tangents = reinterpret_cast<const char4*>(vol->tangents);
     return VKHR_OK;
 }
@@ -386,13 +334,33 @@ extern "C" int vkhr_raymarch(const vkhr_volume* volume, const vkhr_camera* camer
     clear_error();
     int rc = ensure_device();
     if (rc) return rc;
-    VKHR_REQUIRE(rgba, VKHR_ERR_INVALID, "rgba is null");
+    VKHR_REQUIRE(rgba || surface_info, VKHR_ERR_INVALID, "rgba is null");
     RaymarchArgs a;
     if ((rc = make_raymarch_args(volume, camera, light, strand, params, width, height, &a))) return rc;
     a.depth_buffer = depth_buffer;
     a.rgba = reinterpret_cast<float4*>(rgba);
     a.depth = depth;
     a.info = reinterpret_cast<int4*>(surface_info);
+    dim3 grid((width + 15) / 16, (height + 15) / 16);
+    raymarch_kernel<<<grid, 256, 0, as_stream(stream)>>>(a);
+    VKHR_CHECK_LAUNCH();
+    return VKHR_OK;
+}
+
+extern "C" int vkhr_raymarch_ppll(const vkhr_volume* volume, const vkhr_camera* camera, const vkhr_light* light,
+                                  const vkhr_strand_params* strand, const vkhr_render_params* params,
+                                  uint32_t width, uint32_t height, const float* depth_buffer, const vkhr_ppll* ppll,
+                                  vkhr_stream stream) {
+    clear_error();
+    int rc = ensure_device();
+    if (rc) return rc;
+    VKHR_REQUIRE(ppll && ppll->heads && ppll->nodes && ppll->counter, VKHR_ERR_INVALID, "ppll is incomplete");
+    VKHR_REQUIRE(ppll->width == width && ppll->height == height, VKHR_ERR_INVALID,
+                 "ppll is %ux%u, target %ux%u", ppll->width, ppll->height, width, height);
+    RaymarchArgs a;
+    if ((rc = make_raymarch_args(volume, camera, light, strand, params, width, height, &a))) return rc;
+    a.depth_buffer = depth_buffer;
+    a.ppll = PpllView{ppll->heads, ppll->nodes, ppll->counter, ppll->node_count};
     dim3 grid((width + 15) / 16, (height + 15) / 16);
     raymarch_kernel<<<grid, 256, 0, as_stream(stream)>>>(a);
     VKHR_CHECK_LAUNCH();

@@ -1,0 +1,133 @@
+// Device pieces shared by the raymarcher and the strand rasterizer:
+// exact trilinear density sampling, packUnorm4x8, and the per-pixel linked
+// list insert with wave-aggregated node allocation.
+#pragma once
+
+#include "common.hpp"
+
+namespace vkhr {
+
+constexpr uint32_t kPpllNull = 0xffffffffu;
+
+// The density volume as the kernels read it: corner-packed uint8 cells
+// (one 8-byte gather per trilinear tap) or a float volume.
+struct DensityVolume {
+    const unsigned long long* cells;
+    const float* f32;
+    int nx, ny, nz;
+};
+
+// texture(density, uvw).r at texel coordinate (x, y, z) = uvw * n - 0.5:
+// exact weights (the fraction of a float64 coordinate), zero border.
+__device__ __forceinline__ float density_at(const DensityVolume& v, double x, double y, double z) {
+    if (!(x >= -1.0 && y >= -1.0 && z >= -1.0 && x < double(v.nx) && y < double(v.ny) && z < double(v.nz)))
+        return 0.0f;
+    double fx = floor(x), fy = floor(y), fz = floor(z);
+    float wx = float(x - fx), wy = float(y - fy), wz = float(z - fz);
+    int ix = int(fx), iy = int(fy), iz = int(fz);
+    float b[8];
+    if (v.f32) {
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            int tx = ix + (n & 1), ty = iy + ((n >> 1) & 1), tz = iz + (n >> 2);
+            bool in = tx >= 0 && ty >= 0 && tz >= 0 && tx < v.nx && ty < v.ny && tz < v.nz;
+            b[n] = in ? v.f32[size_t(tx) + size_t(v.nx) * (size_t(ty) + size_t(v.ny) * size_t(tz))] : 0.0f;
+        }
+    } else {
+        size_t cx = size_t(v.nx) + 1, cy = size_t(v.ny) + 1;
+        unsigned long long c = v.cells[size_t(ix + 1) + cx * (size_t(iy + 1) + cy * size_t(iz + 1))];
+        uint32_t lo = uint32_t(c), hi = uint32_t(c >> 32);
+        b[0] = float(lo & 0xffu); b[1] = float((lo >> 8) & 0xffu);
+        b[2] = float((lo >> 16) & 0xffu); b[3] = float(lo >> 24);
+        b[4] = float(hi & 0xffu); b[5] = float((hi >> 8) & 0xffu);
+        b[6] = float((hi >> 16) & 0xffu); b[7] = float(hi >> 24);
+    }
+    float c00 = fmaf(wx, b[1] - b[0], b[0]);
+    float c10 = fmaf(wx, b[3] - b[2], b[2]);
+    float c01 = fmaf(wx, b[5] - b[4], b[4]);
+    float c11 = fmaf(wx, b[7] - b[6], b[6]);
+    float c0 = fmaf(wy, c10 - c00, c00);
+    float c1 = fmaf(wy, c11 - c01, c01);
+    float r = fmaf(wz, c1 - c0, c0);
+    return v.f32 ? r : r * (1.0f / 255.0f);
+}
+
+// local_ambient_occlusion with kernel size 2 (8 taps at +-scale/2 texels)
+// around texel coordinate s (local_ambient_occlusion.glsl:9-30).
+__device__ __forceinline__ double local_ambient_occlusion(const DensityVolume& v, const double s[3], double scale,
+                                                          double clamp_v, double exponent) {
+    double dens = 0.0;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        double ox = ((n & 1) ? 0.5 : -0.5) * scale;
+        double oy = (((n >> 1) & 1) ? 0.5 : -0.5) * scale;
+        double oz = ((n >> 2) ? 0.5 : -0.5) * scale;
+        dens += fmin(double(density_at(v, s[0] + ox, s[1] + oy, s[2] + oz)), clamp_v);
+    }
+    return pow(1.0 - dens / 8.0, exponent);
+}
+
+// Kajiya-Kay (kajiya-kay.glsl:7-24); sqrt of a negative rounding residue is 0.
+__device__ __forceinline__ void kajiya_kay(const double diffuse[3], const double specular[3], double p,
+                                           const double t[3], const double l[3], const double e[3], double out[3]) {
+    double ctl = t[0] * l[0] + t[1] * l[1] + t[2] * l[2];
+    double cte = t[0] * e[0] + t[1] * e[1] + t[2] * e[2];
+    double stl = sqrt(fmax(1.0 - ctl * ctl, 0.0)), ste = sqrt(fmax(1.0 - cte * cte, 0.0));
+    double spec = pow(fmax(ctl * cte + stl * ste, 0.0), p);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = diffuse[c] * stl + specular[c] * spec;
+}
+
+__device__ __forceinline__ void normalize3(double v[3]) {
+    double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    double s = l > 0.0 ? 1.0 / l : 0.0;
+    v[0] *= s; v[1] *= s; v[2] *= s;
+}
+
+// packUnorm4x8: round(clamp(c, 0, 1) * 255), ties to even.
+__device__ __forceinline__ uint32_t pack_unorm4x8(float4 c) {
+    auto q = [](float v) -> uint32_t {
+        v = fminf(fmaxf(v, 0.0f), 1.0f);
+        return uint32_t(rintf(v * 255.0f));
+    };
+    return q(c.x) | (q(c.y) << 8) | (q(c.z) << 16) | (q(c.w) << 24);
+}
+
+struct PpllView {
+    uint32_t* heads;
+    uint32_t* nodes;   // 3 words per node: colour, depth bits, previous
+    uint32_t* counter;
+    uint32_t node_count;
+};
+
+// Optional per-node capture (tests/debugging).
+struct CaptureView {
+    uint32_t* segment;
+    uint32_t* pixel;
+    float4* color;
+};
+
+// Inserts one node per lane with `want` set (ppll.glsl:23-48).  Must be
+// called by every lane of the wave (it ballots): one counter atomic per
+// wave, a lane's node is base + its rank among the wanting lanes.  Returns
+// the node index or kPpllNull when the pool is exhausted.
+__device__ __forceinline__ uint32_t ppll_insert(const PpllView& p, bool want, uint32_t pixel, float4 color,
+                                                float depth) {
+    unsigned long long mask = __ballot(want);
+    if (mask == 0ull) return kPpllNull;
+    const int lane = __lane_id();
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(p.counter, uint32_t(__popcll(mask)));
+    base = __shfl(base, leader, kWave);
+    if (!want) return kPpllNull;
+    uint32_t node = base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+    if (node >= p.node_count) return kPpllNull;  // pool exhausted: fragment dropped
+    uint32_t* n = p.nodes + 3ull * node;
+    n[0] = pack_unorm4x8(color);
+    n[1] = __float_as_uint(depth);
+    n[2] = atomicExch(&p.heads[pixel], node);
+    return node;
+}
+
+}  // namespace vkhr

@@ -110,3 +110,59 @@ def style_volume(style, res, device, segments=True, want_f32=False):
     counts, sums = voxelize(xyz, b, res, indices=idx, tangents=tan, want_sums=True)
     vol = finalize(counts, res, sums=sums, normalize=True, want_f32=want_f32, want_cells=True)
     return b, vol
+
+
+class PPLL:
+    """Device storage of a per-pixel linked list (heads, 12-byte nodes, counter)."""
+
+    def __init__(self, width: int, height: int, node_count: int, device="cuda"):
+        self.width, self.height, self.node_count = width, height, node_count
+        self.heads = torch.empty(width * height, dtype=torch.int32, device=device)
+        self.nodes = torch.empty(3 * node_count, dtype=torch.int32, device=device)
+        self.counter = torch.zeros(1, dtype=torch.int32, device=device)
+        self.c = capi.Ppll()
+        self.c.heads, self.c.nodes, self.c.counter = ptr(self.heads), ptr(self.nodes), ptr(self.counter)
+        self.c.node_count, self.c.width, self.c.height = node_count, width, height
+
+    def clear(self):
+        call("vkhr_ppll_clear", C.byref(self.c), stream_of())
+
+    def resolve(self, background=None, out=None):
+        if out is None:
+            out = torch.empty((self.height, self.width, 4), dtype=torch.float32, device=self.heads.device)
+        call("vkhr_ppll_resolve", C.byref(self.c), ptr(background), ptr(out), stream_of())
+        return out
+
+    def used(self) -> int:
+        return int(self.counter.item() & 0xFFFFFFFF)
+
+
+def shadow_depth(xyz: torch.Tensor, indices: torch.Tensor, light_vp_cm, line_width: float, size=(1024, 1024),
+                 out=None):
+    m = (C.c_float * 16)(*[float(v) for v in light_vp_cm])
+    if out is None:
+        out = torch.empty((size[1], size[0]), dtype=torch.float32, device=xyz.device)
+    call("vkhr_shadow_depth", ptr(xyz), xyz.shape[0], ptr(indices), indices.shape[0], m, float(line_width), size[0],
+         size[1], ptr(out), stream_of())
+    return out
+
+
+def raster_strands(xyz, tangents, thickness, indices, cam, light, strand, render, vol: dict | None, res,
+                   shadow_map, ppll: PPLL, depth_buffer=None, capture: dict | None = None):
+    v = volume_struct(vol, res) if vol is not None else None
+    cap = None
+    if capture is not None:
+        cap = capi.FragCapture()
+        cap.segment, cap.pixel, cap.color = ptr(capture.get("segment")), ptr(capture.get("pixel")), \
+            ptr(capture.get("color"))
+    sw, sh = (shadow_map.shape[1], shadow_map.shape[0]) if shadow_map is not None else (0, 0)
+    call("vkhr_raster_strands", ptr(xyz), ptr(tangents), ptr(thickness), xyz.shape[0], ptr(indices), indices.shape[0],
+         C.byref(cam), C.byref(light), C.byref(strand), C.byref(render), C.byref(v) if v is not None else None,
+         ptr(shadow_map), sw, sh, ptr(depth_buffer), C.byref(ppll.c), C.byref(cap) if cap is not None else None,
+         stream_of())
+
+
+def raymarch_ppll(vol: dict, res, cam, light, strand, render, ppll: PPLL, depth_buffer=None):
+    v = volume_struct(vol, res)
+    call("vkhr_raymarch_ppll", C.byref(v), C.byref(cam), C.byref(light), C.byref(strand), C.byref(render),
+         ppll.width, ppll.height, ptr(depth_buffer), C.byref(ppll.c), stream_of())
