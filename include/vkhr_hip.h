@@ -236,6 +236,45 @@ int vkhr_raymarch(const vkhr_volume* volume, const vkhr_camera* camera, const vk
                   uint32_t height, const float* depth_buffer, float* rgba, float* depth,
                   int32_t* surface_info, vkhr_stream stream);
 
+/* ------------------------------------------------------ differentiable form */
+
+/* Learnable parameters of the differentiable raymarch: they replace
+ * strand->hair_opacity (absorption alpha, coverage and ADSM base),
+ * the ADSM strand thickness 11 of volume.frag:78, and
+ * strand->hair_shininess (Kajiya-Kay exponent p). */
+typedef struct vkhr_diff_params {
+    float absorption;
+    float adsm_thickness;
+    float shininess;
+    float reserved_;
+} vkhr_diff_params;
+
+/* Bytes of the per-pixel record the differentiable forward leaves for the
+ * adjoint (64 per pixel: the surface point and the per-ray scalars; no
+ * per-sample state). */
+size_t vkhr_march_record_bytes(uint32_t width, uint32_t height);
+
+/* The raymarch (as vkhr_raymarch, on volume->density_f32 or the cells) with
+ * the parameters of diff (NULL: the reference's) that also writes record. */
+int vkhr_raymarch_record(const vkhr_volume* volume, const vkhr_camera* camera, const vkhr_light* light,
+                         const vkhr_strand_params* strand, const vkhr_render_params* params,
+                         const vkhr_diff_params* diff, uint32_t width, uint32_t height, const float* depth_buffer,
+                         float* rgba, float* depth, void* record, vkhr_stream stream);
+
+/* No counterpart in the reference: the adjoint of vkhr_raymarch_record.
+ * Given dL/d(rgba) (float4 W*H, the gradient of the clamped output), adds
+ * dL/d(density) into d_density (nx*ny*nz floats, volume->density_f32 is
+ * required) and dL/d(absorption, adsm_thickness, shininess) into d_params
+ * (3 floats); both are accumulated, not cleared.  The surface/isosurface
+ * choices are piecewise constant (no gradient); clamps pass gradient strictly
+ * inside [0,1].  Rays are re-walked in 32-step chunks from the ray set-up;
+ * nothing per sample is stored.  Float atomics: summation order is not
+ * deterministic. */
+int vkhr_raymarch_adjoint(const vkhr_volume* volume, const vkhr_camera* camera, const vkhr_light* light,
+                          const vkhr_strand_params* strand, const vkhr_render_params* params,
+                          const vkhr_diff_params* diff, uint32_t width, uint32_t height, const void* record,
+                          const float* d_rgba, float* d_density, float* d_params, vkhr_stream stream);
+
 /* ------------------------------------------------- strand rasterizer, PPLL */
 
 /* Per-pixel linked list (vulkan::LinkedList, src/vkhr/rasterizer/

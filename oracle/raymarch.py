@@ -79,3 +79,20 @@ def lao(density, res, origin, size, p, radius=2.5, exponent=10.0, clamp=0.16):
     pp = np.ascontiguousarray(p, np.float64)
     return lib.oracle_lao(_p(d), C.c_uint32(nx), C.c_uint32(ny), C.c_uint32(nz), _p(o), _p(s), _p(pp),
                           C.c_double(radius), C.c_double(exponent), C.c_double(clamp))
+
+
+def raymarch_grad(tangents, density_f32, res, cam, light, strand, render, diff, width, height, d_rgba=None):
+    """Float64 differentiable forward + analytic gradient.  diff = (alpha,
+    adsm_thickness, shininess).  Returns (rgba (H,W,4), d_density (N,), d_params (3,))."""
+    nx, ny, nz = res
+    tan = np.ascontiguousarray(tangents, np.int8)
+    f32 = np.ascontiguousarray(density_f32, np.float32)
+    dp = np.ascontiguousarray(diff, np.float32)
+    g = np.zeros((height * width, 4), np.float32) if d_rgba is None else np.ascontiguousarray(d_rgba, np.float32)
+    rgba = np.zeros((height * width, 4), np.float64)
+    dd = np.zeros(nx * ny * nz, np.float64)
+    dpar = np.zeros(3, np.float64)
+    lib.oracle_raymarch_grad(_p(tan), _p(f32), C.c_uint32(nx), C.c_uint32(ny), C.c_uint32(nz), C.byref(cam),
+                             C.byref(light), C.byref(strand), C.byref(render), _p(dp), C.c_uint32(width),
+                             C.c_uint32(height), _p(g), _p(rgba), _p(dd), _p(dpar))
+    return rgba.reshape(height, width, 4), dd, dpar

@@ -15,11 +15,13 @@ def test_every_declared_symbol_is_exported():
 
 def test_every_declared_symbol_has_a_binding():
     declared = set(capi.declared_symbols()) - {"vkhr_last_error"}
-    assert declared == set(capi.SIGNATURES), declared ^ set(capi.SIGNATURES)
+    bound = set(capi.SIGNATURES) | set(capi.SIZE_FUNCTIONS)
+    assert declared == bound, declared ^ bound
 
 
 def test_abi_version_and_error_string():
     assert capi.lib.vkhr_abi_version() == 1
+    assert capi.lib.vkhr_march_record_bytes(16, 4) == 64 * 64
     assert isinstance(capi.lib.vkhr_last_error(), bytes)
 
 
@@ -46,7 +48,7 @@ def test_struct_layouts_match_header():
     pairs = [("vkhr_volume", capi.Volume), ("vkhr_aabb", capi.AABB), ("vkhr_camera", capi.Camera),
              ("vkhr_light", capi.Light), ("vkhr_strand_params", capi.StrandParams),
              ("vkhr_render_params", capi.RenderParams), ("vkhr_ppll", capi.Ppll),
-             ("vkhr_frag_capture", capi.FragCapture)]
+             ("vkhr_frag_capture", capi.FragCapture), ("vkhr_diff_params", capi.DiffParams)]
     for cname, cls in pairs:
         hdr = _header_struct_fields(cname)
         names = [f[0] for f in cls._fields_]

@@ -73,6 +73,11 @@ class FragCapture(C.Structure):
     _fields_ = [("segment", C.c_void_p), ("pixel", C.c_void_p), ("color", C.c_void_p)]
 
 
+class DiffParams(C.Structure):
+    _fields_ = [("absorption", C.c_float), ("adsm_thickness", C.c_float), ("shininess", C.c_float),
+                ("reserved_", C.c_float)]
+
+
 P = C.c_void_p
 U32 = C.c_uint32
 I32 = C.c_int
@@ -97,7 +102,13 @@ SIGNATURES = {
                             C.POINTER(RenderParams), C.POINTER(Volume), P, U32, U32, P, C.POINTER(Ppll),
                             C.POINTER(FragCapture), P],
     "vkhr_ppll_resolve": [C.POINTER(Ppll), P, P, P],
+    "vkhr_raymarch_record": [C.POINTER(Volume), C.POINTER(Camera), C.POINTER(Light), C.POINTER(StrandParams),
+                             C.POINTER(RenderParams), C.POINTER(DiffParams), U32, U32, P, P, P, P, P],
+    "vkhr_raymarch_adjoint": [C.POINTER(Volume), C.POINTER(Camera), C.POINTER(Light), C.POINTER(StrandParams),
+                              C.POINTER(RenderParams), C.POINTER(DiffParams), U32, U32, P, P, P, P, P],
 }
+# entry points that do not return a status
+SIZE_FUNCTIONS = {"vkhr_march_record_bytes": [U32, U32]}
 
 
 def declared_symbols() -> list[str]:
@@ -122,6 +133,10 @@ def _load() -> C.CDLL:
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = C.c_int
+    for name, args in SIZE_FUNCTIONS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_size_t
     return lib
 
 

@@ -17,48 +17,13 @@
 // Launch: 16x16-pixel blocks of 256 threads, each wave covering an 8x8 tile
 // (neighbouring rays march neighbouring texels, so gathers hit the same
 // cache lines).
-#include "render.hpp"
+#include "march.hpp"
 
 #include <cmath>
 #include <cstring>
 
 namespace vkhr {
 namespace {
-
-struct RaymarchArgs {
-    double ivp[16];            // inverse(projection * view), column-major
-    double vp_z[4], vp_w[4];   // rows 2 and 3 of projection * view
-    double eye[3];
-    double bmin[3], bmax[3];
-    double origin[3], size[3];
-    double nres[3];            // volume resolution
-    double radius;
-    double light_origin[3];
-    double hair_color[3], intensity[3];
-    double alpha, shininess, iso, lod;
-    double ao_scaling;         // LAO tap offset in texels per kernel unit
-    double ao_clamp, ao_exponent;
-    float step_size;           // 1 / raycast_steps, in float as the shader
-    int shading_model, adsm_on;
-    int nx, ny, nz;
-    int width, height;
-    DensityVolume vol;
-    const char4* tangents;
-    const float* depth_buffer;
-    float4* rgba;
-    float* depth;
-    int4* info;
-    PpllView ppll;             // node insertion when ppll.heads is set
-};
-
-// Texel-space coordinate of a world point along one axis: uvw * n - 0.5.
-__device__ __forceinline__ double to_texel(double p, double o, double s, double n) {
-    return (p - o) / s * n - 0.5;
-}
-
-struct Texel3 {
-    double x, y, z;
-};
 
 // Trilinear SNORM tangent (xyz) at a texel coordinate.
 __device__ __forceinline__ void tangent_at(const RaymarchArgs& a, double x, double y, double z, double out[3]) {
@@ -94,56 +59,14 @@ raymarch_kernel(RaymarchArgs a) {
     float out_depth = 1.0f;
     int4 info = make_int4(0, -1, -1, 0);
 
-    // --- the pixel-centre ray and its AABB entry (the proxy cube fragment)
-    double nxd = 2.0 * (px + 0.5) / a.width - 1.0, nyd = 2.0 * (py + 0.5) / a.height - 1.0;
-    double far4[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-        far4[r] = a.ivp[0 * 4 + r] * nxd + a.ivp[1 * 4 + r] * nyd + a.ivp[2 * 4 + r] + a.ivp[3 * 4 + r];
-    double dir[3] = {far4[0] / far4[3] - a.eye[0], far4[1] / far4[3] - a.eye[1], far4[2] / far4[3] - a.eye[2]};
-    normalize3(dir);
-    double t0 = -1e300, t1 = 1e300;
-    bool hit = true;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        if (dir[c] == 0.0) {
-            if (a.eye[c] < a.bmin[c] || a.eye[c] > a.bmax[c]) hit = false;
-            continue;
-        }
-        double u = (a.bmin[c] - a.eye[c]) / dir[c], v = (a.bmax[c] - a.eye[c]) / dir[c];
-        double lo = fmin(u, v), hi = fmax(u, v);
-        t0 = fmax(t0, lo);
-        t1 = fmin(t1, hi);
-    }
-    hit = inside && hit && t0 <= t1 && t0 > 0.0;
-    double start[3], end[3];
-    double zs = 0, ws = 1, dz = 0, dw = 0;
-    if (hit) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) start[c] = a.eye[c] + dir[c] * t0;
-        zs = a.vp_z[0] * start[0] + a.vp_z[1] * start[1] + a.vp_z[2] * start[2] + a.vp_z[3];
-        ws = a.vp_w[0] * start[0] + a.vp_w[1] * start[1] + a.vp_w[2] * start[2] + a.vp_w[3];
-        double zn = zs / ws;
-        hit = zn >= 0.0 && zn <= 1.0;
-    }
-    if (hit) {
-        // end = start + normalize(start - eye) * radius (volume.frag:32-34)
-        double sd[3] = {start[0] - a.eye[0], start[1] - a.eye[1], start[2] - a.eye[2]};
-        normalize3(sd);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) end[c] = start[c] + sd[c] * a.radius;
-        double ze = a.vp_z[0] * end[0] + a.vp_z[1] * end[1] + a.vp_z[2] * end[2] + a.vp_z[3];
-        double we = a.vp_w[0] * end[0] + a.vp_w[1] * end[1] + a.vp_w[2] * end[2] + a.vp_w[3];
-        dz = ze - zs;
-        dw = we - ws;
+    const Ray ray = setup_ray(a, px, py, inside);
+    if (ray.hit) {
+        const double* xs = ray.xs;
+        const double* dx = ray.dx;
+        const double zs = ray.zs, ws = ray.ws, dz = ray.dz, dw = ray.dw;
+        const double* start = ray.start;
+        const double* end = ray.end;
         const double db = a.depth_buffer ? double(a.depth_buffer[pix]) : 1.0;
-
-        double xs[3], dx[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            xs[c] = to_texel(start[c], a.origin[c], a.size[c], a.nres[c]);
-            dx[c] = to_texel(end[c], a.origin[c], a.size[c], a.nres[c]) - xs[c];
-        }
 
         // --- volume_surface: isosurface march, no early exit
         double acc = 0.0;
@@ -164,6 +87,10 @@ raymarch_kernel(RaymarchArgs a) {
             }
         }
         info = make_int4(2, surf, entry, found ? 1 : 0);  // marched; 1 once a fragment is written
+        if (a.record) {
+            a.record[pix].acc = float(acc);
+            a.record[pix].n_primary = i;
+        }
         double sp[3] = {0.0, 0.0, 0.0};  // the shader's vec3(0) surface
         if (found ? (surf >= 0) : (entry >= 0)) {
             double ts = double(found ? t_surf : t_entry);
@@ -179,6 +106,7 @@ raymarch_kernel(RaymarchArgs a) {
             for (int c = 0; c < 3; ++c) sx[c] = to_texel(sp[c], a.origin[c], a.size[c], a.nres[c]);
 
             double shading[3] = {1.0, 1.0, 1.0};
+            double spec_base = 0.0, strands = 0.0, adsm = 1.0, lao = 1.0;
             double l[3] = {a.light_origin[0] - sp[0], a.light_origin[1] - sp[1], a.light_origin[2] - sp[2]};
             double e[3] = {sp[0] - a.eye[0], sp[1] - a.eye[1], sp[2] - a.eye[2]};
             normalize3(l);
@@ -187,23 +115,34 @@ raymarch_kernel(RaymarchArgs a) {
                 double tg[3];
                 tangent_at(a, sx[0], sx[1], sx[2], tg);
                 normalize3(tg);
-                kajiya_kay(a.hair_color, a.intensity, a.shininess, tg, l, e, shading);
+                kajiya_kay(a.hair_color, a.intensity, a.shininess, tg, l, e, shading, &spec_base);
             }
             double occlusion = 1.0;
             if (a.adsm_on == 1 && a.shading_model != 3) {
                 double xl[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) xl[c] = to_texel(a.light_origin[c], a.origin[c], a.size[c], a.nres[c]) - sx[c];
-                double strands = 0.0;
                 for (float t = 0.0f; t < 1.0f; t += a.step_size) {
                     const double td = double(t);
                     strands += double(density_at(a.vol, fma(td, xl[0], sx[0]), fma(td, xl[1], sx[1]),
                                                  fma(td, xl[2], sx[2])));
                 }
-                occlusion *= pow(1.0 - a.alpha, strands * 11.0);
+                adsm = pow(1.0 - a.alpha, strands * a.adsm_k);
+                occlusion *= adsm;
             }
-            if (a.shading_model != 2)
-                occlusion *= local_ambient_occlusion(a.vol, sx, a.ao_scaling, a.ao_clamp, a.ao_exponent);
+            if (a.shading_model != 2) {
+                lao = local_ambient_occlusion(a.vol, sx, a.ao_scaling, a.ao_clamp, a.ao_exponent);
+                occlusion *= lao;
+            }
+            if (a.record) {
+                MarchRecord& r = a.record[pix];
+                r.sx = sx[0]; r.sy = sx[1]; r.sz = sx[2];
+                r.shadow_sum = float(strands);
+                r.adsm = float(adsm);
+                r.lao = float(lao);
+                r.shade[0] = float(shading[0]); r.shade[1] = float(shading[1]); r.shade[2] = float(shading[2]);
+                r.spec_base = float(spec_base);
+            }
             double zz = a.vp_z[0] * sp[0] + a.vp_z[1] * sp[1] + a.vp_z[2] * sp[2] + a.vp_z[3];
             double ww = a.vp_w[0] * sp[0] + a.vp_w[1] * sp[1] + a.vp_w[2] * sp[2] + a.vp_w[3];
             out_depth = float(zz / ww);
@@ -215,6 +154,7 @@ raymarch_kernel(RaymarchArgs a) {
     }
     if (a.ppll.heads) ppll_insert(a.ppll, info.x == 1, uint32_t(pix), color, out_depth);
     if (!inside) return;
+    if (a.record) a.record[pix].state = info.x;
     if (a.rgba) a.rgba[pix] = color;
     if (a.depth) a.depth[pix] = out_depth;
     if (a.info) a.info[pix] = info;
@@ -264,6 +204,11 @@ double smoothstep(double e0, double e1, double x) {
 
 }  // namespace
 
+void launch_raymarch(const RaymarchArgs& a, hipStream_t stream) {
+    dim3 grid((a.width + 15) / 16, (a.height + 15) / 16);
+    raymarch_kernel<<<grid, 256, 0, stream>>>(a);
+}
+
 // Shared with the batched and adjoint paths: fills the per-launch arguments.
 int make_raymarch_args(const vkhr_volume* vol, const vkhr_camera* cam, const vkhr_light* light,
                        const vkhr_strand_params* sp, const vkhr_render_params* rp, uint32_t width,
@@ -308,6 +253,7 @@ int make_raymarch_args(const vkhr_volume* vol, const vkhr_camera* cam, const vkh
                                               cam->look_at_distance);
     // local_ambient_occlusion(..., kernel_size 2, radius, ...): taps at
     // +-0.5 * radius / ((2 - 1) / 2) voxels
+    a->adsm_k = 11.0;  // volume.frag:78
     a->ao_scaling = double(rp->occlusion_radius) / 0.5;
     a->ao_clamp = rp->ao_clamp;
     a->ao_exponent = rp->ao_exponent;
@@ -341,8 +287,7 @@ extern "C" int vkhr_raymarch(const vkhr_volume* volume, const vkhr_camera* camer
     a.rgba = reinterpret_cast<float4*>(rgba);
     a.depth = depth;
     a.info = reinterpret_cast<int4*>(surface_info);
-    dim3 grid((width + 15) / 16, (height + 15) / 16);
-    raymarch_kernel<<<grid, 256, 0, as_stream(stream)>>>(a);
+    launch_raymarch(a, as_stream(stream));
     VKHR_CHECK_LAUNCH();
     return VKHR_OK;
 }
@@ -361,8 +306,7 @@ extern "C" int vkhr_raymarch_ppll(const vkhr_volume* volume, const vkhr_camera* 
     if ((rc = make_raymarch_args(volume, camera, light, strand, params, width, height, &a))) return rc;
     a.depth_buffer = depth_buffer;
     a.ppll = PpllView{ppll->heads, ppll->nodes, ppll->counter, ppll->node_count};
-    dim3 grid((width + 15) / 16, (height + 15) / 16);
-    raymarch_kernel<<<grid, 256, 0, as_stream(stream)>>>(a);
+    launch_raymarch(a, as_stream(stream));
     VKHR_CHECK_LAUNCH();
     return VKHR_OK;
 }

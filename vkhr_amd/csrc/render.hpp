@@ -69,11 +69,14 @@ __device__ __forceinline__ double local_ambient_occlusion(const DensityVolume& v
 
 // Kajiya-Kay (kajiya-kay.glsl:7-24); sqrt of a negative rounding residue is 0.
 __device__ __forceinline__ void kajiya_kay(const double diffuse[3], const double specular[3], double p,
-                                           const double t[3], const double l[3], const double e[3], double out[3]) {
+                                           const double t[3], const double l[3], const double e[3], double out[3],
+                                           double* base_out = nullptr) {
     double ctl = t[0] * l[0] + t[1] * l[1] + t[2] * l[2];
     double cte = t[0] * e[0] + t[1] * e[1] + t[2] * e[2];
     double stl = sqrt(fmax(1.0 - ctl * ctl, 0.0)), ste = sqrt(fmax(1.0 - cte * cte, 0.0));
-    double spec = pow(fmax(ctl * cte + stl * ste, 0.0), p);
+    double base = fmax(ctl * cte + stl * ste, 0.0);
+    if (base_out) *base_out = base;
+    double spec = pow(base, p);
 #pragma unroll
     for (int c = 0; c < 3; ++c) out[c] = diffuse[c] * stl + specular[c] * spec;
 }

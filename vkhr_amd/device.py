@@ -166,3 +166,35 @@ def raymarch_ppll(vol: dict, res, cam, light, strand, render, ppll: PPLL, depth_
     v = volume_struct(vol, res)
     call("vkhr_raymarch_ppll", C.byref(v), C.byref(cam), C.byref(light), C.byref(strand), C.byref(render),
          ppll.width, ppll.height, ptr(depth_buffer), C.byref(ppll.c), stream_of())
+
+
+def diff_params(absorption: float, adsm_thickness: float = 11.0, shininess: float = 80.0) -> capi.DiffParams:
+    d = capi.DiffParams()
+    d.absorption, d.adsm_thickness, d.shininess = absorption, adsm_thickness, shininess
+    return d
+
+
+def raymarch_record(vol: dict, res, cam, light, strand, render, diff, width, height, record=None, out=None):
+    """Differentiable forward: returns (rgba (H,W,4), record bytes tensor)."""
+    dev = next(t for t in vol.values() if t is not None).device
+    if record is None:
+        record = torch.empty(int(capi.lib.vkhr_march_record_bytes(width, height)), dtype=torch.uint8, device=dev)
+    rgba = out if out is not None else torch.empty((height, width, 4), dtype=torch.float32, device=dev)
+    v = volume_struct(vol, res)
+    call("vkhr_raymarch_record", C.byref(v), C.byref(cam), C.byref(light), C.byref(strand), C.byref(render),
+         C.byref(diff) if diff is not None else None, width, height, None, ptr(rgba), None, ptr(record), stream_of())
+    return rgba, record
+
+
+def raymarch_adjoint(vol: dict, res, cam, light, strand, render, diff, width, height, record, d_rgba,
+                     d_density=None, d_params=None):
+    dev = d_rgba.device
+    if d_density is None:
+        d_density = torch.zeros(res[0] * res[1] * res[2], dtype=torch.float32, device=dev)
+    if d_params is None:
+        d_params = torch.zeros(3, dtype=torch.float32, device=dev)
+    v = volume_struct(vol, res)
+    call("vkhr_raymarch_adjoint", C.byref(v), C.byref(cam), C.byref(light), C.byref(strand), C.byref(render),
+         C.byref(diff) if diff is not None else None, width, height, ptr(record), ptr(d_rgba.contiguous()),
+         ptr(d_density), ptr(d_params), stream_of())
+    return d_density, d_params
