@@ -333,10 +333,12 @@ void march_pixel(const Ctx& C, int64_t pix, const int32_t* forced, PixelOut& o) 
     o.n_primary = i;
     o.acc = acc;
     o.diag[0] = acc;
+    int forced_state = -1;
     if (forced) {
-        surf = forced[0];
-        entry = forced[1];
-        found = forced[2] != 0;
+        forced_state = forced[0];
+        surf = forced[1];
+        entry = forced[2];
+        found = forced[3] != 0;
     }
     o.info[0] = 2;  // the ray marched; 1 once a fragment is written
     o.info[1] = surf; o.info[2] = entry; o.info[3] = found ? 1 : 0;
@@ -354,7 +356,8 @@ void march_pixel(const Ctx& C, int64_t pix, const int32_t* forced, PixelOut& o) 
     }
     o.surface = surface;
     double a_surf = acc / C.iso;
-    if (a_surf == 0.0) return;  // discard (volume.frag:45-46)
+    // discard (volume.frag:45-46); a forced state overrides it
+    if (forced_state >= 0 ? forced_state != 1 : a_surf == 0.0) return;
     o.info[0] = 1;
 
     double coverage = C.lod * a_surf * C.alpha;
@@ -435,9 +438,9 @@ extern "C" {
 // found (0/1)}, diag[4] = {accumulated density, ADSM strands, min |acc - iso|
 // over the nonzero samples, |cos(tangent, light)|}.
 // pixels: list of linear pixel indices to evaluate (n_pixels of them).
-// forced: nullable; per evaluated pixel, {surface step, entry step, found}
-// decisions to use instead of the oracle's own (the shading is then
-// evaluated in float64 at those samples).
+// forced: nullable; per evaluated pixel, {state, surface step, entry step,
+// found} decisions to use instead of the oracle's own (fragment or not, and
+// the samples the shading is evaluated at, in float64).
 void oracle_raymarch(const uint8_t* density, const int8_t* tangent, const float* density_f32, uint32_t nx,
                      uint32_t ny, uint32_t nz, const vkhr_camera* cam, const vkhr_light* light,
                      const vkhr_strand_params* sp, const vkhr_render_params* rp, uint32_t width, uint32_t height,
@@ -446,7 +449,7 @@ void oracle_raymarch(const uint8_t* density, const int8_t* tangent, const float*
     Ctx C = make_ctx(density, tangent, density_f32, nx, ny, nz, cam, light, sp, rp, width, height, depth_buffer);
     PixelOut o;
     for (uint64_t q = 0; q < n_pixels; ++q) {
-        march_pixel(C, pixels[q], forced ? forced + 3 * q : nullptr, o);
+        march_pixel(C, pixels[q], forced ? forced + 4 * q : nullptr, o);
         for (int c = 0; c < 4; ++c) {
             rgba[4 * q + c] = o.rgba[c];
             info[4 * q + c] = o.info[c];

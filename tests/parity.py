@@ -40,7 +40,7 @@ def compare_raymarch(gpu: dict, vol_np: dict, res, cam, light, strand, render, w
     want_depth = o["depth"].copy()
     if n_ties:
         sel = np.nonzero(decision_diff)[0]
-        forced = g_info[sel, 1:4].astype(np.int32)
+        forced = g_info[sel, 0:4].astype(np.int32)
         f = OR.raymarch(vol_np.get("density"), vol_np.get("tangents"), res, cam, light, strand, render, width,
                         height, pixels=pixels[sel], density_f32=density_f32, depth_buffer=depth_buffer,
                         forced=forced)
