@@ -1,0 +1,160 @@
+// Host C++17 mirror of the reference's hair-style call surface
+// (include/vkhr/scene_graph/hair_style.hh:16-210 and the GPU-side classes of
+// include/vkhr/rasterizer/{hair_style,volume,linked_list}.hh), implemented on
+// the MI355X C ABI (vkhr_hip.h).  Names and argument meanings follow the
+// reference; glm types are replaced by small PODs (glm is not a dependency).
+// Errors: the voxelizers and renderers throw vkhr::Error on a failed C-ABI
+// call (the reference's voxelizers have no error path; its Vulkan wrapper
+// throws vkpp::Exception).
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "vkhr_hip.h"
+
+namespace vkhr {
+
+struct vec3 {
+    float x, y, z;
+};
+
+// struct AABB (hair_style.hh:16-21)
+struct AABB {
+    vec3 origin;
+    float radius;
+    vec3 size;
+    float volume;
+};
+
+struct Error : std::runtime_error {
+    int code;
+    Error(const std::string& what, int c) : std::runtime_error(what), code(c) {}
+};
+
+// A device allocation owned by its holder (move-only), as the reference's
+// vk::DeviceImage / vk::StorageBuffer RAII objects.
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    explicit DeviceBuffer(size_t bytes);
+    DeviceBuffer(const void* host, size_t bytes);
+    ~DeviceBuffer();
+    DeviceBuffer(DeviceBuffer&& o) noexcept;
+    DeviceBuffer& operator=(DeviceBuffer&& o) noexcept;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+
+    void* get() const { return ptr_; }
+    template <class T> T* as() const { return static_cast<T*>(ptr_); }
+    size_t size() const { return bytes_; }
+    void download(void* host, size_t bytes) const;
+    void upload(const void* host, size_t bytes);
+
+private:
+    void* ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+class HairStyle final {
+public:
+    HairStyle() = default;
+
+    // HairStyle::Volume (hair_style.hh:89-101): the host vectors of the
+    // reference plus the device volumes the renderers read.
+    struct Volume {
+        vec3 resolution;
+        AABB bounds;
+        std::vector<unsigned char> densities;            // filled by download()
+        std::vector<std::array<int8_t, 4>> tangents;     // filled by download()
+
+        void normalize();                                // Volume::normalize (hair_style.cc:344-357)
+        void download();                                 // device -> the host vectors
+        vkhr_volume view() const;                        // what the kernels take
+
+        DeviceBuffer counts;        // uint32 unsaturated counts
+        DeviceBuffer tangent_sums;  // float xyz sums
+        DeviceBuffer density;       // uint8 (min(count, 255), then normalized)
+        DeviceBuffer tangent;       // int8x4
+        DeviceBuffer cells;         // corner-packed density (raymarch layout)
+        bool normalized = false;
+    };
+
+    unsigned get_strand_count() const;
+    unsigned get_segment_count() const;
+    unsigned get_vertex_count() const;
+
+    void generate_thickness(float radius);   // hair_style.cc:155-169
+    void generate_tangents();                // hair_style.cc:171-194 (on the device)
+    void generate_indices();                 // hair_style.cc:196-213 (on the device)
+    void generate_bounding_box();            // hair_style.cc:215-234 (on the device)
+    AABB get_bounding_box() const;           // hair_style.cc:236-255
+
+    Volume voxelize_vertices(std::size_t width, std::size_t height, std::size_t depth) const;
+    Volume voxelize_segments(std::size_t width, std::size_t height, std::size_t depth) const;
+
+    // Public data, as in the reference (hair_style.hh:126-139).
+    std::vector<unsigned short> segments;
+    std::vector<vec3> vertices;
+    std::vector<float> thickness;
+    std::vector<vec3> tangents;
+    std::vector<unsigned> indices;
+
+    // Device copies, uploaded on demand from the vectors above.
+    const DeviceBuffer& device_vertices() const;
+    const DeviceBuffer& device_tangents() const;
+    const DeviceBuffer& device_thickness() const;
+    const DeviceBuffer& device_indices() const;
+    void invalidate_device() const;          // after editing the public vectors
+
+    vkhr_stream stream = nullptr;
+
+private:
+    DeviceBuffer strand_offsets() const;
+    AABB box_{};
+    bool has_box_ = false;
+    mutable DeviceBuffer d_vertices_, d_tangents_, d_thickness_, d_indices_;
+};
+
+// The raymarcher pass (vulkan::Volume::draw + volume.frag) for one style.
+class Raymarcher {
+public:
+    // Writes W*H float4 (clamped fragment colour) and W*H depth into device
+    // buffers; depth may be null.
+    void draw(const HairStyle::Volume& volume, const vkhr_camera& camera, const vkhr_light& light,
+              const vkhr_strand_params& strand, const vkhr_render_params& params, uint32_t width, uint32_t height,
+              float* rgba, float* depth, vkhr_stream stream = nullptr) const;
+};
+
+// vulkan::LinkedList (linked_list.hh:16-45): PPLL storage, clear, resolve.
+class LinkedList {
+public:
+    LinkedList(uint32_t width, uint32_t height, std::size_t node_count);
+    void clear(vkhr_stream stream = nullptr);
+    void resolve(float* out_rgba, const float* background = nullptr, vkhr_stream stream = nullptr) const;
+    uint32_t node_count_used() const;     // synchronous read of the counter
+    const vkhr_ppll& view() const { return ppll_; }
+
+    static constexpr std::size_t AverageFragmentsPerPixel = 32;
+    static constexpr std::size_t NodeSize = 12;
+
+private:
+    DeviceBuffer heads_, nodes_, counter_;
+    vkhr_ppll ppll_{};
+};
+
+// The strand passes: Rasterizer::draw_depth and vulkan::HairStyle::draw.
+class StrandRasterizer {
+public:
+    void draw_depth(const HairStyle& style, const float light_vp[16], float line_width, uint32_t size,
+                    float* depth_map, vkhr_stream stream = nullptr) const;
+    void draw(const HairStyle& style, const HairStyle::Volume& volume, const vkhr_camera& camera,
+              const vkhr_light& light, const vkhr_strand_params& strand, const vkhr_render_params& params,
+              const float* shadow_map, uint32_t shadow_size, LinkedList& ppll, vkhr_stream stream = nullptr) const;
+};
+
+}  // namespace vkhr

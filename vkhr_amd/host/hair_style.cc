@@ -1,0 +1,276 @@
+// Host C++17 side of the hair-style call surface (include/vkhr/hair_style.hh)
+// over the C ABI.  The public vectors stay the reference's data model; the
+// generators, voxelizers and renderers run on the device.
+#include "vkhr/hair_style.hh"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstring>
+
+namespace vkhr {
+
+namespace {
+
+void check(int rc, const char* what) {
+    if (rc != VKHR_OK) throw Error(std::string(what) + ": " + vkhr_last_error(), rc);
+}
+
+void check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(std::string(what) + ": " + hipGetErrorString(e), VKHR_ERR_HIP);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- DeviceBuffer
+
+DeviceBuffer::DeviceBuffer(size_t bytes) : bytes_(bytes) {
+    if (bytes) check_hip(hipMalloc(&ptr_, bytes), "hipMalloc");
+}
+
+DeviceBuffer::DeviceBuffer(const void* host, size_t bytes) : DeviceBuffer(bytes) {
+    if (bytes) upload(host, bytes);
+}
+
+DeviceBuffer::~DeviceBuffer() {
+    if (ptr_) (void)hipFree(ptr_);
+}
+
+DeviceBuffer::DeviceBuffer(DeviceBuffer&& o) noexcept : ptr_(o.ptr_), bytes_(o.bytes_) {
+    o.ptr_ = nullptr;
+    o.bytes_ = 0;
+}
+
+DeviceBuffer& DeviceBuffer::operator=(DeviceBuffer&& o) noexcept {
+    if (this != &o) {
+        if (ptr_) (void)hipFree(ptr_);
+        ptr_ = o.ptr_;
+        bytes_ = o.bytes_;
+        o.ptr_ = nullptr;
+        o.bytes_ = 0;
+    }
+    return *this;
+}
+
+void DeviceBuffer::download(void* host, size_t bytes) const {
+    check_hip(hipMemcpy(host, ptr_, bytes, hipMemcpyDeviceToHost), "download");
+}
+
+void DeviceBuffer::upload(const void* host, size_t bytes) {
+    check_hip(hipMemcpy(ptr_, host, bytes, hipMemcpyHostToDevice), "upload");
+}
+
+// ------------------------------------------------------------------ HairStyle
+
+unsigned HairStyle::get_strand_count() const { return static_cast<unsigned>(segments.size()); }
+unsigned HairStyle::get_vertex_count() const { return static_cast<unsigned>(vertices.size()); }
+unsigned HairStyle::get_segment_count() const { return get_vertex_count() - get_strand_count(); }
+
+void HairStyle::generate_thickness(float radius) {
+    thickness.clear();
+    thickness.reserve(get_vertex_count());
+    for (unsigned short s : segments) {
+        for (unsigned i = 0; i < s; ++i) thickness.push_back(radius);
+        thickness.push_back(0.0f);  // the tip
+    }
+    d_thickness_ = DeviceBuffer();
+}
+
+DeviceBuffer HairStyle::strand_offsets() const {
+    std::vector<uint32_t> off(segments.size() + 1, 0);
+    for (size_t s = 0; s < segments.size(); ++s) off[s + 1] = off[s] + segments[s] + 1u;
+    return DeviceBuffer(off.data(), off.size() * sizeof(uint32_t));
+}
+
+void HairStyle::generate_tangents() {
+    DeviceBuffer off = strand_offsets();
+    DeviceBuffer out(vertices.size() * sizeof(vec3));
+    check(vkhr_generate_tangents(device_vertices().as<float>(), off.as<uint32_t>(), get_strand_count(),
+                                 out.as<float>(), stream),
+          "vkhr_generate_tangents");
+    tangents.resize(vertices.size());
+    out.download(tangents.data(), tangents.size() * sizeof(vec3));
+    d_tangents_ = std::move(out);
+}
+
+void HairStyle::generate_indices() {
+    DeviceBuffer off = strand_offsets();
+    size_t n = 2 * size_t(get_segment_count());
+    DeviceBuffer out(n ? n * sizeof(uint32_t) : sizeof(uint32_t));
+    check(vkhr_generate_indices(off.as<uint32_t>(), get_strand_count(), out.as<uint32_t>(), stream),
+          "vkhr_generate_indices");
+    indices.resize(n);
+    if (n) out.download(indices.data(), n * sizeof(uint32_t));
+    d_indices_ = std::move(out);
+}
+
+void HairStyle::generate_bounding_box() {
+    vkhr_aabb b;
+    check(vkhr_aabb_compute(device_vertices().as<float>(), get_vertex_count(), &b, stream), "vkhr_aabb_compute");
+    box_ = AABB{{b.origin[0], b.origin[1], b.origin[2]}, b.radius, {b.size[0], b.size[1], b.size[2]}, b.volume};
+    has_box_ = true;
+}
+
+AABB HairStyle::get_bounding_box() const { return box_; }
+
+const DeviceBuffer& HairStyle::device_vertices() const {
+    if (!d_vertices_.get() && !vertices.empty()) d_vertices_ = DeviceBuffer(vertices.data(), vertices.size() * sizeof(vec3));
+    return d_vertices_;
+}
+const DeviceBuffer& HairStyle::device_tangents() const {
+    if (!d_tangents_.get() && !tangents.empty()) d_tangents_ = DeviceBuffer(tangents.data(), tangents.size() * sizeof(vec3));
+    return d_tangents_;
+}
+const DeviceBuffer& HairStyle::device_thickness() const {
+    if (!d_thickness_.get() && !thickness.empty())
+        d_thickness_ = DeviceBuffer(thickness.data(), thickness.size() * sizeof(float));
+    return d_thickness_;
+}
+const DeviceBuffer& HairStyle::device_indices() const {
+    if (!d_indices_.get() && !indices.empty()) d_indices_ = DeviceBuffer(indices.data(), indices.size() * sizeof(unsigned));
+    return d_indices_;
+}
+void HairStyle::invalidate_device() const {
+    d_vertices_ = DeviceBuffer();
+    d_tangents_ = DeviceBuffer();
+    d_thickness_ = DeviceBuffer();
+    d_indices_ = DeviceBuffer();
+}
+
+namespace {
+
+HairStyle::Volume make_volume(const AABB& box, std::size_t w, std::size_t h, std::size_t d) {
+    HairStyle::Volume v;
+    v.resolution = vec3{float(w), float(h), float(d)};
+    v.bounds = box;
+    size_t n = w * h * d;
+    v.counts = DeviceBuffer(n * sizeof(uint32_t));
+    v.tangent_sums = DeviceBuffer(3 * n * sizeof(float));
+    v.density = DeviceBuffer(n);
+    v.tangent = DeviceBuffer(4 * n);
+    v.cells = DeviceBuffer((w + 1) * (h + 1) * (d + 1) * sizeof(uint64_t));
+    return v;
+}
+
+vkhr_aabb to_c(const AABB& b) {
+    vkhr_aabb a;
+    a.origin[0] = b.origin.x; a.origin[1] = b.origin.y; a.origin[2] = b.origin.z;
+    a.radius = b.radius;
+    a.size[0] = b.size.x; a.size[1] = b.size.y; a.size[2] = b.size.z;
+    a.volume = b.volume;
+    return a;
+}
+
+void finalize(HairStyle::Volume& v, int normalize, vkhr_stream stream) {
+    check(vkhr_volume_finalize(v.counts.as<uint32_t>(), v.tangent_sums.as<float>(), uint32_t(v.resolution.x),
+                               uint32_t(v.resolution.y), uint32_t(v.resolution.z), normalize, v.density.as<uint8_t>(),
+                               v.tangent.as<int8_t>(), nullptr, v.cells.as<uint64_t>(), stream),
+          "vkhr_volume_finalize");
+}
+
+}  // namespace
+
+HairStyle::Volume HairStyle::voxelize_vertices(std::size_t width, std::size_t height, std::size_t depth) const {
+    Volume v = make_volume(get_bounding_box(), width, height, depth);
+    vkhr_aabb b = to_c(v.bounds);
+    check(vkhr_voxelize_vertices(device_vertices().as<float>(), get_vertex_count(), device_tangents().as<float>(), &b,
+                                 uint32_t(width), uint32_t(height), uint32_t(depth), v.counts.as<uint32_t>(),
+                                 v.tangent_sums.as<float>(), stream),
+          "vkhr_voxelize_vertices");
+    finalize(v, 0, stream);
+    return v;
+}
+
+HairStyle::Volume HairStyle::voxelize_segments(std::size_t width, std::size_t height, std::size_t depth) const {
+    Volume v = make_volume(get_bounding_box(), width, height, depth);
+    vkhr_aabb b = to_c(v.bounds);
+    check(vkhr_voxelize_segments(device_vertices().as<float>(), get_vertex_count(), device_indices().as<uint32_t>(),
+                                 uint32_t(indices.size()), device_tangents().as<float>(), &b, uint32_t(width),
+                                 uint32_t(height), uint32_t(depth), v.counts.as<uint32_t>(), v.tangent_sums.as<float>(),
+                                 stream),
+          "vkhr_voxelize_segments");
+    finalize(v, 0, stream);
+    return v;
+}
+
+void HairStyle::Volume::normalize() {
+    finalize(*this, 1, nullptr);
+    normalized = true;
+}
+
+void HairStyle::Volume::download() {
+    size_t n = size_t(resolution.x) * size_t(resolution.y) * size_t(resolution.z);
+    densities.resize(n);
+    tangents.resize(n);
+    density.download(densities.data(), n);
+    tangent.download(tangents.data(), 4 * n);
+}
+
+vkhr_volume HairStyle::Volume::view() const {
+    vkhr_volume v;
+    std::memset(&v, 0, sizeof(v));
+    v.density = density.as<uint8_t>();
+    v.cells = cells.as<uint64_t>();
+    v.tangents = tangent.as<int8_t>();
+    v.nx = uint32_t(resolution.x);
+    v.ny = uint32_t(resolution.y);
+    v.nz = uint32_t(resolution.z);
+    return v;
+}
+
+// ------------------------------------------------------------------ renderers
+
+void Raymarcher::draw(const HairStyle::Volume& volume, const vkhr_camera& camera, const vkhr_light& light,
+                      const vkhr_strand_params& strand, const vkhr_render_params& params, uint32_t width,
+                      uint32_t height, float* rgba, float* depth, vkhr_stream stream) const {
+    vkhr_volume v = volume.view();
+    check(vkhr_raymarch(&v, &camera, &light, &strand, &params, width, height, nullptr, rgba, depth, nullptr, stream),
+          "vkhr_raymarch");
+}
+
+LinkedList::LinkedList(uint32_t width, uint32_t height, std::size_t node_count)
+    : heads_(size_t(width) * height * sizeof(uint32_t)),
+      nodes_(node_count * NodeSize),
+      counter_(sizeof(uint32_t)) {
+    ppll_.heads = heads_.as<uint32_t>();
+    ppll_.nodes = nodes_.as<uint32_t>();
+    ppll_.counter = counter_.as<uint32_t>();
+    ppll_.node_count = uint32_t(node_count);
+    ppll_.width = width;
+    ppll_.height = height;
+}
+
+void LinkedList::clear(vkhr_stream stream) { check(vkhr_ppll_clear(&ppll_, stream), "vkhr_ppll_clear"); }
+
+void LinkedList::resolve(float* out_rgba, const float* background, vkhr_stream stream) const {
+    check(vkhr_ppll_resolve(&ppll_, background, out_rgba, stream), "vkhr_ppll_resolve");
+}
+
+uint32_t LinkedList::node_count_used() const {
+    uint32_t n = 0;
+    check_hip(hipDeviceSynchronize(), "synchronize");
+    counter_.download(&n, sizeof(n));
+    return n;
+}
+
+void StrandRasterizer::draw_depth(const HairStyle& style, const float light_vp[16], float line_width, uint32_t size,
+                                  float* depth_map, vkhr_stream stream) const {
+    check(vkhr_shadow_depth(style.device_vertices().as<float>(), style.get_vertex_count(),
+                            style.device_indices().as<uint32_t>(), uint32_t(style.indices.size()), light_vp,
+                            line_width, size, size, depth_map, stream),
+          "vkhr_shadow_depth");
+}
+
+void StrandRasterizer::draw(const HairStyle& style, const HairStyle::Volume& volume, const vkhr_camera& camera,
+                            const vkhr_light& light, const vkhr_strand_params& strand,
+                            const vkhr_render_params& params, const float* shadow_map, uint32_t shadow_size,
+                            LinkedList& ppll, vkhr_stream stream) const {
+    vkhr_volume v = volume.view();
+    check(vkhr_raster_strands(style.device_vertices().as<float>(), style.device_tangents().as<float>(),
+                              style.device_thickness().as<float>(), style.get_vertex_count(),
+                              style.device_indices().as<uint32_t>(), uint32_t(style.indices.size()), &camera, &light,
+                              &strand, &params, &v, shadow_map, shadow_size, shadow_size, nullptr, &ppll.view(),
+                              nullptr, stream),
+          "vkhr_raster_strands");
+}
+
+}  // namespace vkhr

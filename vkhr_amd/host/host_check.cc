@@ -1,0 +1,116 @@
+// Drives the host C++ surface end to end on one style, the way the
+// reference's load and frame do (HairStyle generators -> voxelize_segments ->
+// Volume::normalize -> raymarch; shadow depth -> strands -> PPLL resolve),
+// and writes the results for comparison with the oracle.
+//
+//   host_check <input.bin> <output-dir>
+//
+// input.bin (little endian): u32 n_strands, n_vertices, volume_n, width,
+// height, shadow_size; u16 segments[n_strands]; f32 xyz[3*n_vertices];
+// then the raw vkhr_camera, vkhr_light, vkhr_strand_params and
+// vkhr_render_params blocks (the strand block's bounds are replaced by the
+// style's own AABB), then f32 light_vp[16] (unbiased light view-projection,
+// column-major) for the shadow-map pass.
+#include <hip/hip_runtime_api.h>
+
+#include <cstdio>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "vkhr/hair_style.hh"
+
+namespace {
+
+template <class T>
+void read(std::ifstream& f, T* p, size_t n) {
+    if (!f.read(reinterpret_cast<char*>(p), sizeof(T) * n)) throw std::runtime_error("short input file");
+}
+
+void write(const std::string& path, const void* p, size_t bytes) {
+    std::ofstream f(path, std::ios::binary);
+    if (!f.write(static_cast<const char*>(p), bytes)) throw std::runtime_error("cannot write " + path);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc != 3) {
+        std::fprintf(stderr, "usage: %s <input.bin> <output-dir>\n", argv[0]);
+        return 2;
+    }
+    try {
+        std::ifstream in(argv[1], std::ios::binary);
+        uint32_t hdr[6];
+        read(in, hdr, 6);
+        const uint32_t n_strands = hdr[0], n_vertices = hdr[1], n = hdr[2], W = hdr[3], H = hdr[4], S = hdr[5];
+        vkhr::HairStyle style;
+        style.segments.resize(n_strands);
+        style.vertices.resize(n_vertices);
+        read(in, style.segments.data(), n_strands);
+        read(in, style.vertices.data(), n_vertices);
+        vkhr_camera cam;
+        vkhr_light light;
+        vkhr_strand_params sp;
+        vkhr_render_params rp;
+        read(in, &cam, 1);
+        read(in, &light, 1);
+        read(in, &sp, 1);
+        read(in, &rp, 1);
+
+        // SceneGraph::add_style preprocessing order (src/vkhr/scene_graph.cc:235-242)
+        style.generate_tangents();
+        style.generate_thickness(0.042f);
+        style.generate_indices();
+        style.generate_bounding_box();
+
+        // vulkan::HairStyle::load (src/vkhr/rasterizer/hair_style.cc:75-77)
+        vkhr::HairStyle::Volume volume = style.voxelize_segments(n, n, n);
+        std::vector<uint32_t> counts(size_t(n) * n * n);
+        volume.counts.download(counts.data(), counts.size() * 4);
+        volume.normalize();
+        volume.download();
+
+        vkhr::AABB b = style.get_bounding_box();
+        sp.volume_bounds = vkhr_aabb{{b.origin.x, b.origin.y, b.origin.z}, b.radius, {b.size.x, b.size.y, b.size.z},
+                                     b.volume};
+        const std::string out = argv[2];
+        write(out + "/aabb.bin", &sp.volume_bounds, sizeof(vkhr_aabb));
+        write(out + "/counts.bin", counts.data(), counts.size() * 4);
+        write(out + "/density.bin", volume.densities.data(), volume.densities.size());
+        write(out + "/tangents.bin", volume.tangents.data(), volume.tangents.size() * 4);
+        write(out + "/indices.bin", style.indices.data(), style.indices.size() * 4);
+
+        // raymarcher pass
+        vkhr::DeviceBuffer rgba(size_t(W) * H * 16);
+        vkhr::Raymarcher().draw(volume, cam, light, sp, rp, W, H, rgba.as<float>(), nullptr);
+        std::vector<float> img(size_t(W) * H * 4);
+        rgba.download(img.data(), img.size() * 4);
+        write(out + "/raymarch.bin", img.data(), img.size() * 4);
+
+        // strand pass: shadow map, strands into the PPLL, resolve
+        vkhr::DeviceBuffer shadow(size_t(S) * S * 4);
+        vkhr::StrandRasterizer raster;
+        float light_vp[16];
+        in.read(reinterpret_cast<char*>(light_vp), sizeof(light_vp));
+        raster.draw_depth(style, light_vp, sp.strand_radius, S, shadow.as<float>());
+        vkhr::LinkedList ppll(W, H, vkhr::LinkedList::AverageFragmentsPerPixel * W * H);
+        ppll.clear();
+        vkhr_render_params raster_params = rp;
+        raster_params.renderer = 0;
+        raster.draw(style, volume, cam, light, sp, raster_params, shadow.as<float>(), S, ppll);
+        std::vector<float> smap(size_t(S) * S);
+        shadow.download(smap.data(), smap.size() * 4);
+        write(out + "/shadow.bin", smap.data(), smap.size() * 4);
+        ppll.resolve(rgba.as<float>());
+        rgba.download(img.data(), img.size() * 4);
+        write(out + "/resolve.bin", img.data(), img.size() * 4);
+        uint32_t used = ppll.node_count_used();
+        std::printf("host_check: %u strands, %u segments, %u nodes\n", style.get_strand_count(),
+                    style.get_segment_count(), used);
+        return 0;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "host_check: %s\n", e.what());
+        return 1;
+    }
+}
