@@ -109,7 +109,7 @@ def main():
     n_seg = idx.shape[0] // 2
     nvox = res[0] * res[1] * res[2]
     counts = torch.empty(nvox, dtype=torch.int32, device=dev)
-    sums = torch.empty(3 * nvox, dtype=torch.float32, device=dev)
+    sums = torch.empty(2 * nvox, dtype=torch.int64, device=dev)
     scene = synth.make_scene(bounds, W, H)
     cam, light = params.camera(scene), params.light(scene)
     sp, rp = params.strand(bounds, res), params.render(raycast_steps=float(S))

@@ -77,7 +77,7 @@ public:
         vkhr_volume view() const;                        // what the kernels take
 
         DeviceBuffer counts;        // uint32 unsaturated counts
-        DeviceBuffer tangent_sums;  // float xyz sums
+        DeviceBuffer tangent_sums;  // fixed-point tangent accumulator (2 int64 per voxel)
         DeviceBuffer density;       // uint8 (min(count, 255), then normalized)
         DeviceBuffer tangent;       // int8x4
         DeviceBuffer cells;         // corner-packed density (raymarch layout)

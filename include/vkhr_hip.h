@@ -151,14 +151,17 @@ int vkhr_generate_tangents(const float* xyz, const uint32_t* strand_offsets,
 /* HairStyle::voxelize_vertices (hair_style.cc:257-294): one count per vertex
  * in voxel min(floor((v-o)/(size/res)), res-1), linear index formed in float.
  * counts (nx*ny*nz uint32) is zeroed by the call and receives unsaturated
- * counts; the reference's uint8 view is min(count, 255).  tangent_sums
- * (nullable, 3 floats per voxel) receives the per-voxel sum of the tangents
- * of all contributions (tangents must then be non-null).  Requires
- * nx*ny*nz <= 2^24 (the float index is exact only up to there). */
+ * counts; the reference's uint8 view is min(count, 255).  tangent_acc
+ * (nullable, 2 int64 per voxel) receives the per-voxel sum of the tangents
+ * of all contributions in fixed point, 2^-16 per unit, packed as
+ * word0 = x*2^32 + y and word1 = z (sign-extended); integer sums, so the
+ * result does not depend on thread order (the reference sums floats
+ * serially).  tangents must then be non-null.  Requires nx*ny*nz <= 2^24
+ * (the float index is exact only up to there). */
 int vkhr_voxelize_vertices(const float* xyz, uint32_t n_vertices,
                            const float* tangents, const vkhr_aabb* bounds,
                            uint32_t nx, uint32_t ny, uint32_t nz,
-                           uint32_t* counts, float* tangent_sums,
+                           uint32_t* counts, int64_t* tangent_acc,
                            vkhr_stream stream);
 
 /* HairStyle::voxelize_segments (hair_style.cc:296-342): per index pair, the
@@ -175,7 +178,7 @@ int vkhr_voxelize_segments(const float* xyz, uint32_t n_vertices,
                            const uint32_t* indices, uint32_t n_indices,
                            const float* tangents, const vkhr_aabb* bounds,
                            uint32_t nx, uint32_t ny, uint32_t nz,
-                           uint32_t* counts, float* tangent_sums,
+                           uint32_t* counts, int64_t* tangent_acc,
                            vkhr_stream stream);
 
 /* Turns raw counts into the volumes the renderers sample, as the reference's
@@ -184,8 +187,10 @@ int vkhr_voxelize_segments(const float* xyz, uint32_t n_vertices,
  *   density_u8   = min(count, 255), then, if normalize != 0,
  *                  HairStyle::Volume::normalize (min/max stretch to 0..255,
  *                  float scale, truncation; a constant volume becomes 0);
- *   tangent_i8x4 = trunc(sum / min(count,255) * 127) per component, w = 0,
- *                  0 for empty voxels (nullable; needs tangent_sums);
+ *   tangent_i8x4 = trunc(sum / count * 127) per component, w = 0, from the
+ *                  fixed-point sums of tangent_acc; 0 for empty voxels and
+ *                  for voxels of 32768 or more contributions (nullable;
+ *                  needs tangent_acc);
  *   density_f32  = density_u8 / 255 (nullable; the R8_UNORM value);
  *   cells        = corner-packed density (nullable): for every cell
  *                  (i,j,k), i,j,k in [-1, n-1], the 8 texels of its
@@ -194,7 +199,7 @@ int vkhr_voxelize_segments(const float* xyz, uint32_t n_vertices,
  *                  (nx+1)*(ny+1)*(nz+1) entries, index (i+1) + (j+1)*(nx+1)
  *                  + (k+1)*(nx+1)*(ny+1).  The raymarcher reads this layout:
  *                  one 8-byte gather per trilinear tap. */
-int vkhr_volume_finalize(const uint32_t* counts, const float* tangent_sums,
+int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tangent_acc,
                          uint32_t nx, uint32_t ny, uint32_t nz, int normalize,
                          uint8_t* density_u8, int8_t* tangent_i8x4,
                          float* density_f32, uint64_t* cells,
@@ -351,6 +356,57 @@ int vkhr_ppll_resolve(const vkhr_ppll* ppll, const float* background, float* out
 int vkhr_raymarch_ppll(const vkhr_volume* volume, const vkhr_camera* camera, const vkhr_light* light,
                        const vkhr_strand_params* strand, const vkhr_render_params* params, uint32_t width,
                        uint32_t height, const float* depth_buffer, const vkhr_ppll* ppll, vkhr_stream stream);
+
+/* ------------------------------------------------------------ batched form */
+
+/* One hair style of a batch (no counterpart in the reference, which draws
+ * styles one by one: Rasterizer::voxelize / strand_dvr loop over them,
+ * src/vkhr/rasterizer.cc:216-230, 326-333).  Inputs and outputs are device
+ * buffers the caller owns, laid out as for the per-style entry points. */
+typedef struct vkhr_batch_style {
+    const float* xyz;
+    uint32_t n_vertices;
+    uint32_t n_indices;
+    const uint32_t* indices;
+    const float* tangents;        /* nullable with tangent_acc/tangent_i8x4 */
+    vkhr_aabb bounds;             /* the style's AABB (vkhr_aabb_compute) */
+    uint32_t volume;              /* n of the n^3 volume */
+    uint32_t reserved_;
+    uint32_t* counts;             /* n^3 */
+    int64_t* tangent_acc;         /* 2 n^3, nullable */
+    uint8_t* density;             /* n^3, normalized */
+    int8_t* tangent_i8x4;         /* 4 n^3, nullable */
+    uint64_t* cells;              /* (n+1)^3 */
+    float* rgba;                  /* width*height float4 */
+    vkhr_camera camera;
+    vkhr_light light;
+    vkhr_strand_params strand;    /* volume_bounds should equal bounds */
+    vkhr_render_params params;
+} vkhr_batch_style;
+
+/* Launch shapes, chosen by the autotuner (vkhr_amd/tune.py). */
+typedef struct vkhr_tuning {
+    uint32_t tile_w, tile_h;      /* pixels per wave tile, tile_w * tile_h == 64 */
+    uint32_t blocks_per_cu;       /* persistent raymarch blocks per compute unit */
+    uint32_t reserved_;
+} vkhr_tuning;
+
+typedef struct vkhr_batch vkhr_batch;
+
+/* Builds the device tables of a batch: voxelize_segments + finalize
+ * (normalize, tangents, cells) + raymarch at width x height for every
+ * style.  tuning may be NULL (8x8 wave tiles, 4 blocks per CU). */
+int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styles, uint32_t width, uint32_t height,
+                      const vkhr_tuning* tuning, vkhr_batch** out);
+
+/* Runs the whole batch as one hipGraph (captured on the first call on a
+ * created, non-NULL stream, replayed after): a clear, a voxelization over
+ * the concatenated segments of all styles, chunked finalize / normalize /
+ * cell kernels, and a persistent raymarch over (style, tile) work items.
+ * Per-style results are bit-identical to vkhr_voxelize_segments +
+ * vkhr_volume_finalize(normalize=1) + vkhr_raymarch. */
+int vkhr_batch_launch(vkhr_batch* batch, vkhr_stream stream);
+int vkhr_batch_destroy(vkhr_batch* batch);
 
 #ifdef __cplusplus
 }

@@ -78,6 +78,19 @@ class DiffParams(C.Structure):
                 ("reserved_", C.c_float)]
 
 
+class BatchStyle(C.Structure):
+    _fields_ = [("xyz", C.c_void_p), ("n_vertices", C.c_uint32), ("n_indices", C.c_uint32),
+                ("indices", C.c_void_p), ("tangents", C.c_void_p), ("bounds", AABB), ("volume", C.c_uint32),
+                ("reserved_", C.c_uint32), ("counts", C.c_void_p), ("tangent_acc", C.c_void_p),
+                ("density", C.c_void_p), ("tangent_i8x4", C.c_void_p), ("cells", C.c_void_p), ("rgba", C.c_void_p),
+                ("camera", Camera), ("light", Light), ("strand", StrandParams), ("params", RenderParams)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+                ("reserved_", C.c_uint32)]
+
+
 P = C.c_void_p
 U32 = C.c_uint32
 I32 = C.c_int
@@ -106,6 +119,9 @@ SIGNATURES = {
                              C.POINTER(RenderParams), C.POINTER(DiffParams), U32, U32, P, P, P, P, P],
     "vkhr_raymarch_adjoint": [C.POINTER(Volume), C.POINTER(Camera), C.POINTER(Light), C.POINTER(StrandParams),
                               C.POINTER(RenderParams), C.POINTER(DiffParams), U32, U32, P, P, P, P, P],
+    "vkhr_batch_create": [C.POINTER(BatchStyle), U32, U32, U32, C.POINTER(Tuning), C.POINTER(C.c_void_p)],
+    "vkhr_batch_launch": [P, P],
+    "vkhr_batch_destroy": [P],
 }
 # entry points that do not return a status
 SIZE_FUNCTIONS = {"vkhr_march_record_bytes": [U32, U32]}

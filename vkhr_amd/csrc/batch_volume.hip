@@ -1,0 +1,122 @@
+// Batched voxelization and finalization of many styles (compiled with
+// -ffp-contract=off like the per-style path: the same DDA and finalize
+// bodies, so counts, tangents, densities and cells are bit-identical to
+// per-style launches).
+#include "batch.hpp"
+#include "volume.hpp"
+#include "voxel.hpp"
+
+namespace vkhr {
+namespace {
+
+__device__ __forceinline__ VoxelGrid grid_of(const BatchVolStyle& s) {
+    VoxelGrid g;
+    g.ox = s.ox; g.oy = s.oy; g.oz = s.oz;
+    g.vx = s.vx; g.vy = s.vy; g.vz = s.vz;
+    g.mx = s.mx; g.my = s.my; g.mz = s.mz;
+    g.nx = s.nx; g.ny = s.ny; g.nz = s.nz;
+    return g;
+}
+
+// Zeroes every style's counts and tangent accumulator, resets the per-style
+// min/max and the raymarch work queue.
+__global__ void __launch_bounds__(256)
+batch_clear_kernel(BatchDevice d) {
+    if (blockIdx.x == 0) {
+        for (uint32_t s = threadIdx.x; s < d.n_styles; s += blockDim.x) {
+            d.minmax[2 * s] = 255u;
+            d.minmax[2 * s + 1] = 0u;
+        }
+        if (threadIdx.x == 0) *d.work_counter = 0u;
+    }
+    for (uint32_t c = blockIdx.x; c < d.n_voxel_chunks; c += gridDim.x) {
+        const BatchChunk ch = d.voxel_chunks[c];
+        const BatchVolStyle& st = d.styles[ch.style];
+        for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
+            st.counts[i] = 0u;
+            if (st.acc) {
+                st.acc[2 * i] = 0;
+                st.acc[2 * i + 1] = 0;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256)
+batch_voxelize_kernel(BatchDevice d) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < d.total_pairs; g += stride) {
+        const uint32_t s = style_of(d.pair_prefix, d.n_styles, g);
+        const BatchVolStyle& st = d.styles[s];
+        const uint32_t p = uint32_t(g - d.pair_prefix[s]);
+        voxelize_segment(grid_of(st), st.xyz, st.n_vertices, st.indices[2u * p], st.indices[2u * p + 1u],
+                         st.tangents, st.counts, st.acc);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+batch_finalize_kernel(BatchDevice d) {
+    __shared__ uint32_t red[2][256 / kWave];
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    for (uint32_t c = blockIdx.x; c < d.n_voxel_chunks; c += gridDim.x) {
+        const BatchChunk ch = d.voxel_chunks[c];
+        const BatchVolStyle& st = d.styles[ch.style];
+        uint32_t lo = 255u, hi = 0u;
+        for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
+            uint32_t v = finalize_voxel(st.counts, st.acc, i, st.density, st.tangent_q);
+            lo = v < lo ? v : lo;
+            hi = v > hi ? v : hi;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            uint32_t a = __shfl_xor(lo, o, kWave), b = __shfl_xor(hi, o, kWave);
+            lo = a < lo ? a : lo;
+            hi = b > hi ? b : hi;
+        }
+        if (lane == 0) { red[0][wid] = lo; red[1][wid] = hi; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < int(blockDim.x / kWave); ++w) {
+                lo = red[0][w] < lo ? red[0][w] : lo;
+                hi = red[1][w] > hi ? red[1][w] : hi;
+            }
+            atomicMin(&d.minmax[2 * ch.style], lo);
+            atomicMax(&d.minmax[2 * ch.style + 1], hi);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256)
+batch_normalize_kernel(BatchDevice d) {
+    for (uint32_t c = blockIdx.x; c < d.n_voxel_chunks; c += gridDim.x) {
+        const BatchChunk ch = d.voxel_chunks[c];
+        const BatchVolStyle& st = d.styles[ch.style];
+        const uint32_t lo = d.minmax[2 * ch.style], hi = d.minmax[2 * ch.style + 1];
+        const bool flat = hi == lo;
+        const float scaling = normalize_scaling(lo, hi);
+        for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x)
+            st.density[i] = uint8_t(normalize_value(st.density[i], lo, scaling, flat));
+    }
+}
+
+__global__ void __launch_bounds__(256)
+batch_cells_kernel(BatchDevice d) {
+    for (uint32_t c = blockIdx.x; c < d.n_cell_chunks; c += gridDim.x) {
+        const BatchChunk ch = d.cell_chunks[c];
+        const BatchVolStyle& st = d.styles[ch.style];
+        for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x)
+            st.cells[i] = pack_cell(st.density, st.nx, st.ny, st.nz, i);
+    }
+}
+
+}  // namespace
+
+void launch_batch_volume(const BatchDevice& d, hipStream_t stream) {
+    batch_clear_kernel<<<4096, 256, 0, stream>>>(d);
+    batch_voxelize_kernel<<<grid_for(d.total_pairs, 256, 65536), 256, 0, stream>>>(d);
+    batch_finalize_kernel<<<4096, 256, 0, stream>>>(d);
+    batch_normalize_kernel<<<4096, 256, 0, stream>>>(d);
+    batch_cells_kernel<<<4096, 256, 0, stream>>>(d);
+}
+
+}  // namespace vkhr

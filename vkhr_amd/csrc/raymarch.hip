@@ -25,139 +25,12 @@
 namespace vkhr {
 namespace {
 
-// Trilinear SNORM tangent (xyz) at a texel coordinate.
-__device__ __forceinline__ void tangent_at(const RaymarchArgs& a, double x, double y, double z, double out[3]) {
-    out[0] = out[1] = out[2] = 0.0;
-    if (!(x >= -1.0 && y >= -1.0 && z >= -1.0 && x < double(a.nx) && y < double(a.ny) && z < double(a.nz)))
-        return;
-    double fx = floor(x), fy = floor(y), fz = floor(z);
-    double w[3] = {x - fx, y - fy, z - fz};
-    int ix = int(fx), iy = int(fy), iz = int(fz);
-#pragma unroll
-    for (int n = 0; n < 8; ++n) {
-        int tx = ix + (n & 1), ty = iy + ((n >> 1) & 1), tz = iz + (n >> 2);
-        if (tx < 0 || ty < 0 || tz < 0 || tx >= a.nx || ty >= a.ny || tz >= a.nz) continue;
-        char4 t = a.tangents[size_t(tx) + size_t(a.nx) * (size_t(ty) + size_t(a.ny) * size_t(tz))];
-        double wt = ((n & 1) ? w[0] : 1.0 - w[0]) * (((n >> 1) & 1) ? w[1] : 1.0 - w[1]) *
-                    ((n >> 2) ? w[2] : 1.0 - w[2]);
-        out[0] += wt * fmax(double(t.x) / 127.0, -1.0);
-        out[1] += wt * fmax(double(t.y) / 127.0, -1.0);
-        out[2] += wt * fmax(double(t.z) / 127.0, -1.0);
-    }
-}
-
 __global__ void __launch_bounds__(256)
 raymarch_kernel(RaymarchArgs a) {
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
     const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
-    // Out-of-image lanes stay converged (the PPLL insert ballots the wave).
-    const bool inside = px < a.width && py < a.height;
-    const size_t pix = inside ? size_t(py) * a.width + px : 0;
-
-    float4 color = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float out_depth = 1.0f;
-    int4 info = make_int4(0, -1, -1, 0);
-
-    const Ray ray = setup_ray(a, px, py, inside);
-    if (ray.hit) {
-        const double* xs = ray.xs;
-        const double* dx = ray.dx;
-        const double zs = ray.zs, ws = ray.ws, dz = ray.dz, dw = ray.dw;
-        const double* start = ray.start;
-        const double* end = ray.end;
-        const double db = a.depth_buffer ? double(a.depth_buffer[pix]) : 1.0;
-
-        // --- volume_surface: isosurface march, no early exit
-        double acc = 0.0;
-        float t_surf = 0.0f, t_entry = 0.0f;
-        int surf = -1, entry = -1;
-        bool found = false;
-        int i = 0;
-        for (float t = 0.0f; t < 1.0f; t += a.step_size, ++i) {
-            const double td = double(t);
-            // depth test: db < z/w  <=>  db*w < z  (w > 0 in front of the eye)
-            if (db * fma(td, dw, ws) < fma(td, dz, zs)) break;
-            float d = density_at(a.vol, fma(td, dx[0], xs[0]), fma(td, dx[1], xs[1]), fma(td, dx[2], xs[2]));
-            acc += double(d);
-            if (d != 0.0f) {
-                if (acc <= a.iso) { surf = i; t_surf = t; }
-                if (acc >= a.iso) found = true;
-                if (entry < 0) { entry = i; t_entry = t; }
-            }
-        }
-        info = make_int4(2, surf, entry, found ? 1 : 0);  // marched; 1 once a fragment is written
-        if (a.record) {
-            a.record[pix].acc = float(acc);
-            a.record[pix].n_primary = i;
-        }
-        double sp[3] = {0.0, 0.0, 0.0};  // the shader's vec3(0) surface
-        if (found ? (surf >= 0) : (entry >= 0)) {
-            double ts = double(found ? t_surf : t_entry);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) sp[c] = start[c] * (1.0 - ts) + end[c] * ts;
-        }
-        const double a_surf = acc / a.iso;
-        if (a_surf != 0.0) {
-            info.x = 1;
-            const double coverage = a.lod * a_surf * a.alpha;
-            double sx[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) sx[c] = to_texel(sp[c], a.origin[c], a.size[c], a.nres[c]);
-
-            double shading[3] = {1.0, 1.0, 1.0};
-            double spec_base = 0.0, strands = 0.0, adsm = 1.0, lao = 1.0;
-            double l[3] = {a.light_origin[0] - sp[0], a.light_origin[1] - sp[1], a.light_origin[2] - sp[2]};
-            double e[3] = {sp[0] - a.eye[0], sp[1] - a.eye[1], sp[2] - a.eye[2]};
-            normalize3(l);
-            normalize3(e);
-            if (a.shading_model == 0) {
-                double tg[3];
-                tangent_at(a, sx[0], sx[1], sx[2], tg);
-                normalize3(tg);
-                kajiya_kay(a.hair_color, a.intensity, a.shininess, tg, l, e, shading, &spec_base);
-            }
-            double occlusion = 1.0;
-            if (a.adsm_on == 1 && a.shading_model != 3) {
-                double xl[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) xl[c] = to_texel(a.light_origin[c], a.origin[c], a.size[c], a.nres[c]) - sx[c];
-                for (float t = 0.0f; t < 1.0f; t += a.step_size) {
-                    const double td = double(t);
-                    strands += double(density_at(a.vol, fma(td, xl[0], sx[0]), fma(td, xl[1], sx[1]),
-                                                 fma(td, xl[2], sx[2])));
-                }
-                adsm = pow(1.0 - a.alpha, strands * a.adsm_k);
-                occlusion *= adsm;
-            }
-            if (a.shading_model != 2) {
-                lao = local_ambient_occlusion(a.vol, sx, a.ao_scaling, a.ao_clamp, a.ao_exponent);
-                occlusion *= lao;
-            }
-            if (a.record) {
-                MarchRecord& r = a.record[pix];
-                r.sx = sx[0]; r.sy = sx[1]; r.sz = sx[2];
-                r.shadow_sum = float(strands);
-                r.adsm = float(adsm);
-                r.lao = float(lao);
-                r.shade[0] = float(shading[0]); r.shade[1] = float(shading[1]); r.shade[2] = float(shading[2]);
-                r.spec_base = float(spec_base);
-            }
-            double zz = a.vp_z[0] * sp[0] + a.vp_z[1] * sp[1] + a.vp_z[2] * sp[2] + a.vp_z[3];
-            double ww = a.vp_w[0] * sp[0] + a.vp_w[1] * sp[1] + a.vp_w[2] * sp[2] + a.vp_w[3];
-            out_depth = float(zz / ww);
-            color = make_float4(float(fmin(fmax(shading[0] * occlusion, 0.0), 1.0)),
-                                float(fmin(fmax(shading[1] * occlusion, 0.0), 1.0)),
-                                float(fmin(fmax(shading[2] * occlusion, 0.0), 1.0)),
-                                float(fmin(fmax(coverage, 0.0), 1.0)));
-        }
-    }
-    if (a.ppll.heads) ppll_insert(a.ppll, info.x == 1, uint32_t(pix), color, out_depth);
-    if (!inside) return;
-    if (a.record) a.record[pix].state = info.x;
-    if (a.rgba) a.rgba[pix] = color;
-    if (a.depth) a.depth[pix] = out_depth;
-    if (a.info) a.info[pix] = info;
+    raymarch_pixel(a, px, py);
 }
 
 void matmul_cm(const float* x, const float* y, double* out) {  // x * y, column-major

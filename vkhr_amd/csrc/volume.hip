@@ -9,7 +9,7 @@
 // Pass 3 (per cell):  corner packing, 8 texels -> one uint64.
 // Compiled with -ffp-contract=off (quantization and normalize are float
 // expressions whose rounding must match the reference's).
-#include "common.hpp"
+#include "volume.hpp"
 
 namespace vkhr {
 namespace {
@@ -20,37 +20,15 @@ minmax_init_kernel(uint32_t* mm) {
 }
 
 __global__ void __launch_bounds__(256)
-finalize_counts_kernel(const uint32_t* __restrict__ counts, const float* __restrict__ sums, size_t n,
+finalize_counts_kernel(const uint32_t* __restrict__ counts, const long long* __restrict__ sums, size_t n,
                        uint8_t* __restrict__ density, int8_t* __restrict__ tangents,
                        uint32_t* __restrict__ minmax) {
     uint32_t lo = 255u, hi = 0u;
     size_t stride = size_t(gridDim.x) * blockDim.x;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint32_t c = counts[i];
-        uint32_t d = c < 255u ? c : 255u;
-        density[i] = uint8_t(d);
+        uint32_t d = finalize_voxel(counts, sums, i, density, tangents);
         lo = d < lo ? d : lo;
         hi = d > hi ? d : hi;
-        if (tangents) {
-            char4 q = make_char4(0, 0, 0, 0);
-            if (c) {
-                // sum / float(count) * 127, truncated toward zero.  Up to 255
-                // contributions this is the reference's expression; above, the
-                // mean over all contributions (the reference keeps only the
-                // first 255 in its serial order).
-                float denom = float(c);
-                float x = sums[3 * i + 0] / denom * 127.0f;
-                float y = sums[3 * i + 1] / denom * 127.0f;
-                float z = sums[3 * i + 2] / denom * 127.0f;
-                auto cvt = [](float v) -> signed char {
-                    if (!(v == v)) return 0;  // NaN tangent contributions
-                    v = fminf(fmaxf(v, -127.0f), 127.0f);
-                    return static_cast<signed char>(static_cast<int>(v));
-                };
-                q = make_char4(cvt(x), cvt(y), cvt(z), 0);
-            }
-            reinterpret_cast<char4*>(tangents)[i] = q;
-        }
     }
     // wave min/max, then block min/max in LDS, then one atomic pair per block
     for (int o = 32; o > 0; o >>= 1) {
@@ -76,17 +54,14 @@ __global__ void __launch_bounds__(256)
 normalize_kernel(uint8_t* __restrict__ density, size_t n, const uint32_t* __restrict__ minmax, int normalize,
                  float* __restrict__ density_f32) {
     uint32_t lo = minmax ? minmax[0] : 0u, hi = minmax ? minmax[1] : 255u;
-    // scaling = 255.0f / (max - min) (hair_style.cc:351); a constant volume
-    // (division by zero, NaN products) is defined here to become all zero.
+    // a constant volume (division by zero, NaN products) becomes all zero
     bool flat = (hi == lo);
-    float scaling = flat ? 0.0f : 255.0f / float(int(hi) - int(lo));
+    float scaling = normalize_scaling(lo, hi);
     size_t stride = size_t(gridDim.x) * blockDim.x;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
         uint32_t d = density[i];
         if (normalize) {
-            uint32_t shifted = (d - lo) & 0xffu;           // densities[i] -= data_min (uint8)
-            float v = float(shifted) * scaling;            // densities[i] * scaling
-            d = flat ? 0u : (uint32_t(v) & 0xffu);         // float -> uint8, truncation
+            d = normalize_value(d, lo, scaling, flat);
             density[i] = uint8_t(d);
         }
         if (density_f32) density_f32[i] = float(d) / 255.0f;  // R8_UNORM
@@ -103,19 +78,7 @@ cells_kernel(const uint8_t* __restrict__ density, uint32_t nx, uint32_t ny, uint
     size_t total = size_t(cx) * cy * cz;
     size_t stride = size_t(gridDim.x) * blockDim.x;
     for (size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x; c < total; c += stride) {
-        int i = int(c % cx) - 1;
-        int j = int((c / cx) % cy) - 1;
-        int k = int(c / (size_t(cx) * cy)) - 1;
-        unsigned long long packed = 0ull;
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            int x = i + (b & 1), y = j + ((b >> 1) & 1), z = k + (b >> 2);
-            if (x >= 0 && y >= 0 && z >= 0 && x < int(nx) && y < int(ny) && z < int(nz)) {
-                unsigned long long t = density[size_t(x) + size_t(nx) * (size_t(y) + size_t(ny) * size_t(z))];
-                packed |= t << (8 * b);
-            }
-        }
-        cells[c] = packed;
+        cells[c] = pack_cell(density, nx, ny, nz, c);
     }
 }
 
@@ -134,7 +97,7 @@ int build_cells(const uint8_t* density, uint32_t nx, uint32_t ny, uint32_t nz, u
 
 using namespace vkhr;
 
-extern "C" int vkhr_volume_finalize(const uint32_t* counts, const float* tangent_sums, uint32_t nx, uint32_t ny,
+extern "C" int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tangent_acc, uint32_t nx, uint32_t ny,
                                     uint32_t nz, int normalize, uint8_t* density_u8, int8_t* tangent_i8x4,
                                     float* density_f32, uint64_t* cells, vkhr_stream stream) {
     clear_error();
@@ -142,7 +105,7 @@ extern "C" int vkhr_volume_finalize(const uint32_t* counts, const float* tangent
     if (rc) return rc;
     VKHR_REQUIRE(counts && density_u8, VKHR_ERR_INVALID, "counts/density_u8 is null");
     VKHR_REQUIRE(nx && ny && nz, VKHR_ERR_INVALID, "zero volume resolution");
-    VKHR_REQUIRE(!tangent_i8x4 || tangent_sums, VKHR_ERR_INVALID, "tangent_i8x4 needs tangent_sums");
+    VKHR_REQUIRE(!tangent_i8x4 || tangent_acc, VKHR_ERR_INVALID, "tangent_i8x4 needs tangent_acc");
     size_t n = size_t(nx) * ny * nz;
     VKHR_REQUIRE(n <= (size_t(1) << 31), VKHR_ERR_OVERFLOW, "volume too large");
     hipStream_t s = as_stream(stream);
@@ -156,7 +119,8 @@ extern "C" int vkhr_volume_finalize(const uint32_t* counts, const float* tangent
     }
     // A capped grid keeps the min/max atomics at one pair per block.
     unsigned grid = grid_for(n, 256, 2048);
-    finalize_counts_kernel<<<grid, 256, 0, s>>>(counts, tangent_sums, n, density_u8, tangent_i8x4, mm);
+    finalize_counts_kernel<<<grid, 256, 0, s>>>(counts, reinterpret_cast<const long long*>(tangent_acc), n, density_u8,
+                                                tangent_i8x4, mm);
     VKHR_CHECK_LAUNCH();
     if (normalize || density_f32) {
         normalize_kernel<<<grid, 256, 0, s>>>(density_u8, n, mm, normalize, density_f32);

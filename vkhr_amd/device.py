@@ -51,7 +51,7 @@ def voxelize(xyz: torch.Tensor, bounds: dict, res, indices: torch.Tensor | None 
     if counts is None:
         counts = torch.empty(n, dtype=torch.int32, device=dev)
     if want_sums and sums is None:
-        sums = torch.empty(3 * n, dtype=torch.float32, device=dev)
+        sums = torch.empty(2 * n, dtype=torch.int64, device=dev)   # fixed-point tangent accumulator
     b = capi.make_aabb(bounds)
     if indices is None:
         call("vkhr_voxelize_vertices", ptr(xyz), xyz.shape[0], ptr(tangents), C.byref(b), nx, ny, nz,

@@ -144,7 +144,7 @@ HairStyle::Volume make_volume(const AABB& box, std::size_t w, std::size_t h, std
     v.bounds = box;
     size_t n = w * h * d;
     v.counts = DeviceBuffer(n * sizeof(uint32_t));
-    v.tangent_sums = DeviceBuffer(3 * n * sizeof(float));
+    v.tangent_sums = DeviceBuffer(2 * n * sizeof(int64_t));
     v.density = DeviceBuffer(n);
     v.tangent = DeviceBuffer(4 * n);
     v.cells = DeviceBuffer((w + 1) * (h + 1) * (d + 1) * sizeof(uint64_t));
@@ -161,7 +161,7 @@ vkhr_aabb to_c(const AABB& b) {
 }
 
 void finalize(HairStyle::Volume& v, int normalize, vkhr_stream stream) {
-    check(vkhr_volume_finalize(v.counts.as<uint32_t>(), v.tangent_sums.as<float>(), uint32_t(v.resolution.x),
+    check(vkhr_volume_finalize(v.counts.as<uint32_t>(), v.tangent_sums.as<int64_t>(), uint32_t(v.resolution.x),
                                uint32_t(v.resolution.y), uint32_t(v.resolution.z), normalize, v.density.as<uint8_t>(),
                                v.tangent.as<int8_t>(), nullptr, v.cells.as<uint64_t>(), stream),
           "vkhr_volume_finalize");
@@ -174,7 +174,7 @@ HairStyle::Volume HairStyle::voxelize_vertices(std::size_t width, std::size_t he
     vkhr_aabb b = to_c(v.bounds);
     check(vkhr_voxelize_vertices(device_vertices().as<float>(), get_vertex_count(), device_tangents().as<float>(), &b,
                                  uint32_t(width), uint32_t(height), uint32_t(depth), v.counts.as<uint32_t>(),
-                                 v.tangent_sums.as<float>(), stream),
+                                 v.tangent_sums.as<int64_t>(), stream),
           "vkhr_voxelize_vertices");
     finalize(v, 0, stream);
     return v;
@@ -185,7 +185,7 @@ HairStyle::Volume HairStyle::voxelize_segments(std::size_t width, std::size_t he
     vkhr_aabb b = to_c(v.bounds);
     check(vkhr_voxelize_segments(device_vertices().as<float>(), get_vertex_count(), device_indices().as<uint32_t>(),
                                  uint32_t(indices.size()), device_tangents().as<float>(), &b, uint32_t(width),
-                                 uint32_t(height), uint32_t(depth), v.counts.as<uint32_t>(), v.tangent_sums.as<float>(),
+                                 uint32_t(height), uint32_t(depth), v.counts.as<uint32_t>(), v.tangent_sums.as<int64_t>(),
                                  stream),
           "vkhr_voxelize_segments");
     finalize(v, 0, stream);
