@@ -39,7 +39,71 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=8.0, help="seconds per CPU-baseline leg")
+    ap.add_argument("--no-extras", action="store_true", help="skip the adjoint and batched legs")
     return ap.parse_args()
+
+
+def time_adjoint(dev, reps=10):
+    """Differentiable form: 128^3 float32 normalized density from the
+    10,000-strand style, 512x512, 256 steps; returns (forward ms, adjoint ms)."""
+    import torch
+    from vkhr_amd import params, synth
+    from vkhr_amd import device as D
+    res, w, h = (128, 128, 128), 512, 512
+    s = synth.config0()
+    b, vol = D.style_volume(s, res, dev, segments=False, want_f32=True)
+    fvol = dict(density_f32=vol["density_f32"], tangents=vol["tangents"])
+    sc = synth.make_scene(b, w, h)
+    cam, light, sp = params.camera(sc), params.light(sc), params.strand(b, res)
+    rp = params.render(raycast_steps=256.0)
+    diff = D.diff_params(0.3, 11.0, 80.0)
+    rgba, rec = D.raymarch_record(fvol, res, cam, light, sp, rp, diff, w, h)
+    g = torch.ones_like(rgba) * 1e-3
+    gd = torch.zeros(res[0] ** 3, dtype=torch.float32, device=dev)
+    gp = torch.zeros(3, dtype=torch.float32, device=dev)
+    for _ in range(3):
+        D.raymarch_record(fvol, res, cam, light, sp, rp, diff, w, h, record=rec, out=rgba)
+        D.raymarch_adjoint(fvol, res, cam, light, sp, rp, diff, w, h, rec, g, gd, gp)
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    e[0].record()
+    for _ in range(reps):
+        D.raymarch_record(fvol, res, cam, light, sp, rp, diff, w, h, record=rec, out=rgba)
+    e[1].record()
+    e[2].record()
+    for _ in range(reps):
+        D.raymarch_adjoint(fvol, res, cam, light, sp, rp, diff, w, h, rec, g, gd, gp)
+    e[3].record()
+    torch.cuda.synchronize()
+    return e[0].elapsed_time(e[1]) / reps, e[2].elapsed_time(e[3]) / reps
+
+
+def time_batch(dev, reps=3):
+    """Batched form: 32 styles of uneven size at 960x540; returns (per-style
+    launches ms, batched graph ms, covered rays per batch)."""
+    import torch
+    from vkhr_amd import synth
+    from vkhr_amd.batch import Batch, StyleBuffers, load_tuning
+    styles, _ = synth.batch_styles()
+    bufs = [StyleBuffers(st, st.meta["volume"], 960, 540, dev, azimuth=11.25 * i) for i, st in enumerate(styles)]
+    batch = Batch(bufs, 960, 540, tuning=load_tuning("batch"))
+    for _ in range(2):
+        batch.launch()
+        for b in bufs:
+            b.run_plain()
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    e[0].record()
+    for _ in range(reps):
+        for b in bufs:
+            b.run_plain()
+    e[1].record()
+    torch.cuda.synchronize()
+    e[2].record(batch.stream)
+    for _ in range(reps):
+        batch.launch(wait=False)
+    e[3].record(batch.stream)
+    torch.cuda.synchronize()
+    return e[0].elapsed_time(e[1]) / reps, e[2].elapsed_time(e[3]) / reps
 
 
 def embree_present() -> bool:
@@ -191,6 +255,14 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "raymarch_kernel", "algorithmic_bytes": ray_bytes},
     }
+    if not args.no_extras:
+        t_f, t_b = time_adjoint(dev)
+        result["adjoint_fwd_ratio"] = t_b / t_f
+        result["adjoint"] = {"workload": "128^3 float32 density, 512x512, 256 steps", "forward_ms": t_f,
+                             "adjoint_ms": t_b}
+        t_plain, t_batch = time_batch(dev)
+        result["batched"] = {"workload": "32 styles, 5k-140k strands, 64^3/128^3/256^3, 960x540, 512 steps",
+                             "per_style_ms": t_plain, "batched_graph_ms": t_batch, "speedup": t_plain / t_batch}
     if not args.no_cpu_baseline:
         vol_np = dict(density=vol["density"].cpu().numpy(), tangents=vol["tangents"].cpu().numpy())
         b = {k: np.asarray(v) for k, v in bounds.items()}
