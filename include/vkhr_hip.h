@@ -153,10 +153,11 @@ int vkhr_generate_tangents(const float* xyz, const uint32_t* strand_offsets,
  * counts (nx*ny*nz uint32) is zeroed by the call and receives unsaturated
  * counts; the reference's uint8 view is min(count, 255).  tangent_acc
  * (nullable, 2 int64 per voxel) receives the per-voxel sum of the tangents
- * of all contributions in fixed point, 2^-16 per unit, packed as
- * word0 = x*2^32 + y and word1 = z (sign-extended); integer sums, so the
- * result does not depend on thread order (the reference sums floats
- * serially).  tangents must then be non-null.  Requires nx*ny*nz <= 2^24
+ * of all contributions in fixed point, 2^-16 per unit, packed with the
+ * count as word0 = x*2^32 + y and word1 = z*2^32 + count (sign-extended
+ * fields; counts is then unpacked from word1 at the end of the call);
+ * integer sums, so the result does not depend on thread order (the
+ * reference sums floats serially).  tangents must then be non-null.  Requires nx*ny*nz <= 2^24
  * (the float index is exact only up to there). */
 int vkhr_voxelize_vertices(const float* xyz, uint32_t n_vertices,
                            const float* tangents, const vkhr_aabb* bounds,

@@ -9,15 +9,17 @@
 // isosurface and surface choices are piecewise constant and carry none; the
 // clamp to [0,1] passes gradient only strictly inside.
 //
-// One block owns a 16x16-pixel tile.  It re-walks its rays in chunks of 32
-// steps: the march state (t, sample index) is carried from chunk to chunk and
-// the positions are recomputed from the ray set-up (nothing per sample is
-// stored).  Per chunk the block reduces the bounding box of its samples'
-// trilinear footprints; if it fits in the LDS window, the scatter of
-// weight * trilinear weight goes to LDS atomics and the window is flushed with
-// one global atomic per touched texel, otherwise straight to global atomics.
-// LAO taps (8 per fragment) scatter directly.  Parameter gradients are
-// reduced per block, then one atomic each.
+// Rays are re-walked in chunks of 32 steps: the march state (the float
+// parameter t of the shader's loop) is carried from chunk to chunk and the
+// positions are recomputed from the ray set-up (nothing per sample is
+// stored).  The scatter of weight * trilinear weight goes to a corner-packed
+// accumulator of 8 floats per cell (the cell layout of the density: cell
+// (i,j,k) for i,j,k in [-1, n-1] holds texels i..i+1 x j..j+1 x k..k+1), and
+// the lanes of a wave are transposed so that 8 lanes add the 8 corners of one
+// sample: one 32-byte line per sample instead of 8 scattered words (device
+// atomics are bound by line requests).  A gather pass then folds the cells
+// into the texel gradient.  Parameter gradients are reduced per block, then
+// one atomic each.
 #include "march.hpp"
 
 #include <cmath>
@@ -27,142 +29,118 @@ namespace vkhr {
 namespace {
 
 constexpr int kChunk = 32;
-constexpr int kWindow = 8192;  // floats of LDS per block (32 KiB)
+constexpr uint32_t kNoCell = 0xffffffffu;
 
 struct AdjointArgs {
     RaymarchArgs f;
     const MarchRecord* record;
     const float4* d_rgba;
-    float* d_density;
+    float* cell_grad;           // 8 floats per cell, (nx+1)(ny+1)(nz+1) cells
     float* d_params;            // absorption, ADSM thickness, shininess
     int n_steps;                // iterations of the shader's float loop
     double ln_transmit;         // log(1 - alpha)
     double light_texel[3];
 };
 
-__device__ __forceinline__ size_t vidx(const RaymarchArgs& f, int x, int y, int z) {
-    return size_t(x) + size_t(f.nx) * (size_t(y) + size_t(f.ny) * size_t(z));
-}
-
-// Scatter c * (trilinear weights at x) to global memory.
-__device__ __forceinline__ void scatter_global(const AdjointArgs& a, const double x[3], float c) {
-    const RaymarchArgs& f = a.f;
-    double fx = floor(x[0]), fy = floor(x[1]), fz = floor(x[2]);
-    float w[3] = {float(x[0] - fx), float(x[1] - fy), float(x[2] - fz)};
-    int ix = int(fx), iy = int(fy), iz = int(fz);
-#pragma unroll
-    for (int n = 0; n < 8; ++n) {
-        int tx = ix + (n & 1), ty = iy + ((n >> 1) & 1), tz = iz + (n >> 2);
-        if (tx < 0 || ty < 0 || tz < 0 || tx >= f.nx || ty >= f.ny || tz >= f.nz) continue;
-        float wt = ((n & 1) ? w[0] : 1.0f - w[0]) * (((n >> 1) & 1) ? w[1] : 1.0f - w[1]) *
-                   ((n >> 2) ? w[2] : 1.0f - w[2]);
-        unsafeAtomicAdd(&a.d_density[vidx(f, tx, ty, tz)], c * wt);
-    }
-}
-
 __device__ __forceinline__ bool in_range(const RaymarchArgs& f, const double x[3]) {
     return x[0] >= -1.0 && x[1] >= -1.0 && x[2] >= -1.0 && x[0] < double(f.nx) && x[1] < double(f.ny) &&
            x[2] < double(f.nz);
 }
 
+// The cell of a texel coordinate inside [-1, n)^3 and its trilinear
+// fractions (the weights of density_at).
+__device__ __forceinline__ uint32_t cell_of(const RaymarchArgs& f, const double x[3], float w[3]) {
+    const double fx = floor(x[0]), fy = floor(x[1]), fz = floor(x[2]);
+    w[0] = float(x[0] - fx);
+    w[1] = float(x[1] - fy);
+    w[2] = float(x[2] - fz);
+    const uint32_t cx = uint32_t(f.nx) + 1u, cy = uint32_t(f.ny) + 1u;
+    return uint32_t(int(fx) + 1) + cx * (uint32_t(int(fy) + 1) + cy * uint32_t(int(fz) + 1));
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Adds c * (trilinear corner weights) of every lane's sample to the cell
+// accumulator; lanes without a sample pass cell == kNoCell.  In round r the
+// 8 lanes of each group of 8 add the 8 corners of the sample of lane r of
+// the group.  Every lane of the wave must call it (shuffles).
+__device__ __forceinline__ void scatter_wave(float* __restrict__ cg, uint32_t cell, float wx, float wy, float wz,
+                                             float c) {
+    const unsigned long long live = __ballot(cell != kNoCell);
+    if (!live) return;
+    const int lane = __lane_id();
+    const int k = lane & 7, group = lane & 56;
+    for (int r = 0; r < 8; ++r) {
+        if (!((live >> r) & 0x0101010101010101ull)) continue;
+        const int src = group | r;
+        const uint32_t sc = __shfl(cell, src, kWave);
+        const float sx = __shfl(wx, src, kWave), sy = __shfl(wy, src, kWave), sz = __shfl(wz, src, kWave);
+        const float s_c = __shfl(c, src, kWave);
+        if (sc != kNoCell) {
+            const float wt = ((k & 1) ? sx : 1.0f - sx) * ((k & 2) ? sy : 1.0f - sy) * ((k & 4) ? sz : 1.0f - sz);
+            unsafeAtomicAdd(&cg[size_t(sc) * 8u + size_t(k)], s_c * wt);
+        }
+    }
+}
+
 // One family of rays (primary or shadow): x(t) = x0 + t * dx for the
-// sample parameters t of the shader's float loop, samples [0, n) of each ray,
-// weight c per sample.  All threads of the block call this (barriers).
-__device__ void scatter_rays(const AdjointArgs& a, float* win, int* bb, bool active, const double x0[3],
-                             const double dx[3], int n, float c) {
+// sample parameters t of the shader's float loop, samples [0, n) of each
+// active lane's ray, weight c per sample.  The loop runs to the wave's
+// longest ray so that every lane takes part in the scatter.
+__device__ void scatter_rays(const AdjointArgs& a, bool active, const double x0[3], const double dx[3], int n,
+                             float c) {
     const RaymarchArgs& f = a.f;
     const float step = f.step_size;
+    const int n_lane = active ? n : 0;
+    const int n_wave = wave_max(n_lane);
     float t = 0.0f;  // the march state carried across chunks
-    const int n_chunks = (a.n_steps + kChunk - 1) / kChunk;
-    for (int m = 0; m < n_chunks; ++m) {
-        const int i0 = m * kChunk;
-        const int i1 = min(i0 + kChunk, n);
-        const bool live = active && i0 < i1;
-        // footprint of this chunk: the segment between its first and last samples
-        int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-1, -1, -1};
-        if (live) {
-            float tl = t;
-            for (int i = i0 + 1; i < i1; ++i) tl += step;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                double p = fma(double(t), dx[k], x0[k]), q = fma(double(tl), dx[k], x0[k]);
-                double mn = fmin(p, q), mx = fmax(p, q);
-                int l = int(floor(fmax(mn, -1.0))), h = int(floor(fmin(mx, 1e9))) + 1;
-                int nk = k == 0 ? f.nx : (k == 1 ? f.ny : f.nz);
-                lo[k] = max(l, 0);
-                hi[k] = min(h, nk - 1);
-            }
-            if (lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2]) lo[0] = 0x7fffffff;
+    for (int i0 = 0; i0 < n_wave; i0 += kChunk) {
+        const int i1 = min(i0 + kChunk, n_wave);
+        float tt = t;
+        for (int i = i0; i < i1; ++i, tt += step) {
+            const double x[3] = {fma(double(tt), dx[0], x0[0]), fma(double(tt), dx[1], x0[1]),
+                                 fma(double(tt), dx[2], x0[2])};
+            float w[3] = {0.0f, 0.0f, 0.0f};
+            uint32_t cell = kNoCell;
+            if (i < n_lane && in_range(f, x)) cell = cell_of(f, x, w);
+            scatter_wave(a.cell_grad, cell, w[0], w[1], w[2], c);
         }
-        if (threadIdx.x == 0) {
-            bb[0] = bb[1] = bb[2] = 0x7fffffff;
-            bb[3] = bb[4] = bb[5] = -1;
-        }
-        __syncthreads();
-        if (live && lo[0] != 0x7fffffff) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                atomicMin(&bb[k], lo[k]);
-                atomicMax(&bb[3 + k], hi[k]);
-            }
-        }
-        __syncthreads();
-        const int wx0 = bb[0], wy0 = bb[1], wz0 = bb[2];
-        const int dimx = bb[3] - wx0 + 1, dimy = bb[4] - wy0 + 1, dimz = bb[5] - wz0 + 1;
-        const long long vol = (bb[3] < 0) ? 0 : (long long)dimx * dimy * dimz;
-        const bool use_lds = vol > 0 && vol <= kWindow;
-        if (use_lds) {
-            for (int j = threadIdx.x; j < vol; j += blockDim.x) win[j] = 0.0f;
-            __syncthreads();
-        }
-        if (live) {
-            float tt = t;
-            for (int i = i0; i < i1; ++i, tt += step) {
-                double x[3] = {fma(double(tt), dx[0], x0[0]), fma(double(tt), dx[1], x0[1]),
-                               fma(double(tt), dx[2], x0[2])};
-                if (!in_range(f, x)) continue;
-                if (!use_lds) {
-                    scatter_global(a, x, c);
-                    continue;
-                }
-                double fx = floor(x[0]), fy = floor(x[1]), fz = floor(x[2]);
-                float w[3] = {float(x[0] - fx), float(x[1] - fy), float(x[2] - fz)};
-                int ix = int(fx), iy = int(fy), iz = int(fz);
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    int tx = ix + (q & 1), ty = iy + ((q >> 1) & 1), tz = iz + (q >> 2);
-                    if (tx < 0 || ty < 0 || tz < 0 || tx >= f.nx || ty >= f.ny || tz >= f.nz) continue;
-                    float wt = ((q & 1) ? w[0] : 1.0f - w[0]) * (((q >> 1) & 1) ? w[1] : 1.0f - w[1]) *
-                               ((q >> 2) ? w[2] : 1.0f - w[2]);
-                    atomicAdd(&win[(tx - wx0) + dimx * ((ty - wy0) + dimy * (tz - wz0))], c * wt);
-                }
-            }
-        }
-        if (use_lds) {
-            __syncthreads();
-            for (int j = threadIdx.x; j < vol; j += blockDim.x) {
-                float v = win[j];
-                if (v != 0.0f) {
-                    int lx = j % dimx, ly = (j / dimx) % dimy, lz = j / (dimx * dimy);
-                    unsafeAtomicAdd(&a.d_density[vidx(f, wx0 + lx, wy0 + ly, wz0 + lz)], v);
-                }
-            }
-        }
-        __syncthreads();
         // advance the carried state past this chunk
         for (int i = 0; i < kChunk; ++i) t += step;
     }
 }
 
+// Folds the corner-packed accumulator into the texel gradient (added).
 __global__ void __launch_bounds__(256)
-adjoint_kernel(AdjointArgs a) {
-    __shared__ float win[kWindow];
-    __shared__ int bb[6];
-    __shared__ float red[3][256 / kWave];
+gather_cells_kernel(const float* __restrict__ cg, float* __restrict__ grad, int nx, int ny, int nz) {
+    const size_t n = size_t(nx) * ny * nz;
+    const size_t cx = size_t(nx) + 1, cy = size_t(ny) + 1;
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const size_t x = i % size_t(nx), yz = i / size_t(nx);
+        const size_t y = yz % size_t(ny), z = yz / size_t(ny);
+        float s = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            // texel (x,y,z) is corner q of the cell whose base is (x,y,z) - corner
+            const size_t c = (x + 1 - (q & 1)) + cx * ((y + 1 - ((q >> 1) & 1)) + cy * (z + 1 - (q >> 2)));
+            s += cg[c * 8u + size_t(q)];
+        }
+        grad[i] += s;
+    }
+}
+
+// The adjoint of one pixel (volume.frag:31-102 differentiated).  Called by
+// all 256 threads of a block: 2x2 waves of 8x8 pixels from (x0, y0); red is
+// the block's reduction buffer for the parameter gradients.
+__device__ __forceinline__ void adjoint_tile(const AdjointArgs& a, int x0, int y0, float (*red)[256 / kWave]) {
     const RaymarchArgs& f = a.f;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
-    const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
+    const int px = x0 + (wave & 1) * 8 + (lane & 7);
+    const int py = y0 + (wave >> 1) * 8 + (lane >> 3);
     const bool inside = px < f.width && py < f.height;
     const size_t pix = inside ? size_t(py) * f.width + px : 0;
 
@@ -215,33 +193,52 @@ adjoint_kernel(AdjointArgs a) {
         for (int w = 0; w < int(blockDim.x / kWave); ++w) v += red[threadIdx.x][w];
         if (v != 0.0f) atomicAdd(&a.d_params[threadIdx.x], v);
     }
+    __syncthreads();  // red is reused by the next tile of a persistent block
 
     // primary rays
     const Ray ray = setup_ray(f, px, py, inside && frag && c_primary != 0.0);
-    scatter_rays(a, win, bb, ray.hit && c_primary != 0.0, ray.xs, ray.dx, frag ? r.n_primary : 0,
-                 float(c_primary));
+    scatter_rays(a, ray.hit && c_primary != 0.0, ray.xs, ray.dx, frag ? r.n_primary : 0, float(c_primary));
     // shadow rays: from the surface point to the light origin, every sample
     double s[3] = {0, 0, 0}, sl[3] = {0, 0, 0};
     if (frag) {
         s[0] = r.sx; s[1] = r.sy; s[2] = r.sz;
         for (int k = 0; k < 3; ++k) sl[k] = a.light_texel[k] - s[k];
     }
-    scatter_rays(a, win, bb, frag && adsm_on && c_shadow != 0.0, s, sl, a.n_steps, float(c_shadow));
-    // LAO taps: d ln(lao)/dD = -e / (8 - D), through min(v, clamp)
-    if (frag && lao_on && G != 0.0) {
-        double v[8], D = 0.0;
-        double tap[8][3];
-        for (int n = 0; n < 8; ++n) {
-            tap[n][0] = s[0] + ((n & 1) ? 0.5 : -0.5) * f.ao_scaling;
-            tap[n][1] = s[1] + (((n >> 1) & 1) ? 0.5 : -0.5) * f.ao_scaling;
-            tap[n][2] = s[2] + ((n >> 2) ? 0.5 : -0.5) * f.ao_scaling;
-            v[n] = double(density_at(f.vol, tap[n][0], tap[n][1], tap[n][2]));
-            D += fmin(v[n], f.ao_clamp);
-        }
-        double cl = G * (-f.ao_exponent) / (8.0 - D);
-        for (int n = 0; n < 8; ++n)
-            if (v[n] < f.ao_clamp && in_range(f, tap[n])) scatter_global(a, tap[n], float(cl));
+    // samples past the volume's support carry no gradient: the walk stops
+    // there (an upper bound on their count is enough, in_range does the rest)
+    int n_shadow = 0;
+    if (frag) {
+        double lo, hi;
+        support_interval(f, s, sl, &lo, &hi);
+        if (hi >= 0.0 && lo <= hi) n_shadow = min(a.n_steps, int(fmin(hi, 2.0) / double(f.step_size) * 1.0001) + 2);
     }
+    scatter_rays(a, frag && adsm_on && c_shadow != 0.0, s, sl, n_shadow, float(c_shadow));
+    // LAO taps: d ln(lao)/dD = -e / (8 - D), through min(v, clamp)
+    const bool lao = frag && lao_on && G != 0.0;
+    double v[8], D = 0.0;
+    double tap[8][3];
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        tap[n][0] = s[0] + ((n & 1) ? 0.5 : -0.5) * f.ao_scaling;
+        tap[n][1] = s[1] + (((n >> 1) & 1) ? 0.5 : -0.5) * f.ao_scaling;
+        tap[n][2] = s[2] + ((n >> 2) ? 0.5 : -0.5) * f.ao_scaling;
+        v[n] = lao ? double(density_at(f.vol, tap[n][0], tap[n][1], tap[n][2])) : 0.0;
+        D += fmin(v[n], f.ao_clamp);
+    }
+    const float cl = lao ? float(G * (-f.ao_exponent) / (8.0 - D)) : 0.0f;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        float w[3] = {0.0f, 0.0f, 0.0f};
+        uint32_t cell = kNoCell;
+        if (lao && v[n] < f.ao_clamp && in_range(f, tap[n])) cell = cell_of(f, tap[n], w);
+        scatter_wave(a.cell_grad, cell, w[0], w[1], w[2], cl);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+adjoint_kernel(AdjointArgs a) {
+    __shared__ float red[3][256 / kWave];
+    adjoint_tile(a, blockIdx.x * 16, blockIdx.y * 16, red);
 }
 
 }  // namespace
@@ -306,7 +303,6 @@ extern "C" int vkhr_raymarch_adjoint(const vkhr_volume* volume, const vkhr_camer
     }
     a->record = static_cast<const MarchRecord*>(record);
     a->d_rgba = reinterpret_cast<const float4*>(d_rgba);
-    a->d_density = d_density;
     a->d_params = d_params;
     int n = 0;
     for (float t = 0.0f; t < 1.0f; t += a->f.step_size) ++n;
@@ -314,8 +310,28 @@ extern "C" int vkhr_raymarch_adjoint(const vkhr_volume* volume, const vkhr_camer
     a->ln_transmit = std::log(1.0 - a->f.alpha);
     for (int c = 0; c < 3; ++c)
         a->light_texel[c] = (a->f.light_origin[c] - a->f.origin[c]) / a->f.size[c] * a->f.nres[c] - 0.5;
+    const size_t n_cells = size_t(volume->nx + 1) * (volume->ny + 1) * (volume->nz + 1);
+    if (n_cells >= (size_t(1) << 32)) {
+        delete a;
+        set_error("volume %ux%ux%u too large for the adjoint", volume->nx, volume->ny, volume->nz);
+        return VKHR_ERR_INVALID;
+    }
+    void* cg = nullptr;
+    if ((rc = scratch(n_cells * 8 * sizeof(float), &cg))) {
+        delete a;
+        return rc;
+    }
+    a->cell_grad = static_cast<float*>(cg);
+    hipStream_t st = as_stream(stream);
+    if ((rc = fill_u32(static_cast<uint32_t*>(cg), n_cells * 8, 0u, st))) {
+        delete a;
+        return rc;
+    }
     dim3 grid((width + 15) / 16, (height + 15) / 16);
-    adjoint_kernel<<<grid, 256, 0, as_stream(stream)>>>(*a);
+    adjoint_kernel<<<grid, 256, 0, st>>>(*a);
+    const size_t n_vox = size_t(volume->nx) * volume->ny * volume->nz;
+    gather_cells_kernel<<<grid_for(n_vox, 256, 8192), 256, 0, st>>>(a->cell_grad, d_density, int(volume->nx),
+                                                                      int(volume->ny), int(volume->nz));
     delete a;
     VKHR_CHECK_LAUNCH();
     return VKHR_OK;

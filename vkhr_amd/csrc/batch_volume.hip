@@ -18,7 +18,7 @@ __device__ __forceinline__ VoxelGrid grid_of(const BatchVolStyle& s) {
     return g;
 }
 
-// Zeroes every style's counts and tangent accumulator, resets the per-style
+// Zeroes every style's counts or tangent accumulator, resets the per-style
 // min/max and the raymarch work queue.
 __global__ void __launch_bounds__(256)
 batch_clear_kernel(BatchDevice d) {
@@ -32,11 +32,13 @@ batch_clear_kernel(BatchDevice d) {
     for (uint32_t c = blockIdx.x; c < d.n_voxel_chunks; c += gridDim.x) {
         const BatchChunk ch = d.voxel_chunks[c];
         const BatchVolStyle& st = d.styles[ch.style];
+        // with the packed accumulator the counts are unpacked by finalize
         for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
-            st.counts[i] = 0u;
             if (st.acc) {
                 st.acc[2 * i] = 0;
                 st.acc[2 * i + 1] = 0;
+            } else {
+                st.counts[i] = 0u;
             }
         }
     }
@@ -63,7 +65,7 @@ batch_finalize_kernel(BatchDevice d) {
         const BatchVolStyle& st = d.styles[ch.style];
         uint32_t lo = 255u, hi = 0u;
         for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
-            uint32_t v = finalize_voxel(st.counts, st.acc, i, st.density, st.tangent_q);
+            uint32_t v = finalize_voxel(st.counts, st.acc, i, st.density, st.tangent_q, st.acc != nullptr);
             lo = v < lo ? v : lo;
             hi = v > hi ? v : hi;
         }

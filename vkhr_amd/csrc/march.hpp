@@ -17,7 +17,8 @@ struct MarchRecord {
     float adsm, lao;       // the two occlusion factors
     float shade[3];        // shading (Kajiya-Kay or 1)
     float spec_base;       // Kajiya-Kay specular base max(cTL cTE + sTL sTE, 0)
-    int n_primary;         // samples taken by the primary march (depth-buffer break)
+    int n_primary;         // samples taken by the primary march (up to the depth-buffer break or
+                           // the end of the volume's support)
     int state;             // 0 miss, 1 fragment, 2 marched and discarded
 };
 static_assert(sizeof(MarchRecord) == 64, "record layout");
@@ -134,6 +135,30 @@ __device__ __forceinline__ void tangent_at(const RaymarchArgs& a, double x, doub
     }
 }
 
+// The parameter interval [lo, hi] outside of which x0 + t*dx (texel
+// coordinates) lies outside [-1, n)^3, where every trilinear tap of the
+// zero-bordered volume reads 0.  Widened by far more than the rounding of
+// the fma that forms the sample position, so that a sample skipped for
+// lying outside it is exactly 0; lo > hi when the line misses the support.
+__device__ __forceinline__ void support_interval(const RaymarchArgs& a, const double x0[3], const double dx[3],
+                                                 double* lo, double* hi) {
+    double l = -1e300, h = 1e300;
+    const double n[3] = {double(a.nx), double(a.ny), double(a.nz)};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (dx[c] == 0.0) {
+            if (!(x0[c] >= -1.0 && x0[c] < n[c])) { l = 1.0; h = 0.0; }
+            continue;
+        }
+        const double margin = 1e-9 + 1e-14 * (fabs(x0[c]) + fabs(dx[c]) + n[c]) / fabs(dx[c]);
+        const double u = (-1.0 - x0[c]) / dx[c], v = (n[c] - x0[c]) / dx[c];
+        l = fmax(l, fmin(u, v) - margin);
+        h = fmin(h, fmax(u, v) + margin);
+    }
+    *lo = l;
+    *hi = h;
+}
+
 // One pixel of the raymarch pass (volume.frag:31-102).  Every lane of a
 // wave must call it (the optional PPLL insert ballots the wave); lanes
 // outside the image take no fragment.
@@ -160,11 +185,17 @@ __device__ __forceinline__ void raymarch_pixel(const RaymarchArgs& a, int px, in
         float t_surf = 0.0f, t_entry = 0.0f;
         int surf = -1, entry = -1;
         bool found = false;
+        // Past the end of the volume's support every sample is 0 and can
+        // change neither the sum nor the surface choices: the march stops
+        // there (the depth test before it changes nothing either).
+        double t_lo, t_hi;
+        support_interval(a, xs, dx, &t_lo, &t_hi);
         int i = 0;
         for (float t = 0.0f; t < 1.0f; t += a.step_size, ++i) {
             const double td = double(t);
             // depth test: db < z/w  <=>  db*w < z  (w > 0 in front of the eye)
             if (db * fma(td, dw, ws) < fma(td, dz, zs)) break;
+            if (td > t_hi) break;
             float d = density_at(a.vol, fma(td, dx[0], xs[0]), fma(td, dx[1], xs[1]), fma(td, dx[2], xs[2]));
             acc += double(d);
             if (d != 0.0f) {
@@ -209,8 +240,13 @@ __device__ __forceinline__ void raymarch_pixel(const RaymarchArgs& a, int px, in
                 double xl[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) xl[c] = to_texel(a.light_origin[c], a.origin[c], a.size[c], a.nres[c]) - sx[c];
+                // only the samples inside the volume's support add anything
+                double t_lo, t_hi;
+                support_interval(a, sx, xl, &t_lo, &t_hi);
                 for (float t = 0.0f; t < 1.0f; t += a.step_size) {
                     const double td = double(t);
+                    if (td > t_hi) break;
+                    if (td < t_lo) continue;
                     strands += double(density_at(a.vol, fma(td, xl[0], sx[0]), fma(td, xl[1], sx[1]),
                                                  fma(td, xl[2], sx[2])));
                 }

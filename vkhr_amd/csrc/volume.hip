@@ -26,7 +26,7 @@ finalize_counts_kernel(const uint32_t* __restrict__ counts, const long long* __r
     uint32_t lo = 255u, hi = 0u;
     size_t stride = size_t(gridDim.x) * blockDim.x;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint32_t d = finalize_voxel(counts, sums, i, density, tangents);
+        uint32_t d = finalize_voxel(const_cast<uint32_t*>(counts), sums, i, density, tangents);
         lo = d < lo ? d : lo;
         hi = d > hi ? d : hi;
     }

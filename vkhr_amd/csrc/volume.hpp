@@ -13,10 +13,18 @@ namespace vkhr {
 // keeps only the first 255 in its serial order, so saturated voxels may
 // differ).  The sums come from the fixed-point accumulator (voxel.hpp);
 // beyond 32767 contributions they are not decodable and the tangent is 0.
-__device__ __forceinline__ uint32_t finalize_voxel(const uint32_t* __restrict__ counts,
-                                                   const long long* __restrict__ acc, size_t i,
-                                                   uint8_t* __restrict__ density, int8_t* __restrict__ tangents) {
-    uint32_t c = counts[i];
+// With counts_from_acc the count is taken from the accumulator and stored
+// to counts[i] (the batched path unpacks the counts here).
+__device__ __forceinline__ uint32_t finalize_voxel(uint32_t* __restrict__ counts, const long long* __restrict__ acc,
+                                                   size_t i, uint8_t* __restrict__ density,
+                                                   int8_t* __restrict__ tangents, bool counts_from_acc = false) {
+    uint32_t c;
+    if (counts_from_acc) {
+        c = uint32_t((unsigned long long)(acc[2 * i + 1]) & 0xffffffffull);
+        counts[i] = c;
+    } else {
+        c = counts[i];
+    }
     uint32_t d = c < 255u ? c : 255u;
     density[i] = uint8_t(d);
     if (tangents) {
@@ -29,7 +37,8 @@ __device__ __forceinline__ uint32_t finalize_voxel(const uint32_t* __restrict__ 
             float denom = float(c);
             float x = float(sx) * inv / denom * 127.0f;
             float y = float(sy) * inv / denom * 127.0f;
-            float z = float(w1) * inv / denom * 127.0f;
+            long long sz = w1 >> 32;  // w1 = z * 2^32 + count, 0 <= count < 2^32
+            float z = float(sz) * inv / denom * 127.0f;
             auto cvt = [](float v) -> signed char {
                 v = fminf(fmaxf(v, -127.0f), 127.0f);
                 return static_cast<signed char>(static_cast<int>(v));

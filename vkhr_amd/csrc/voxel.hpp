@@ -34,11 +34,14 @@ __device__ __forceinline__ uint32_t voxel_of(const VoxelGrid& g, float px, float
     return uint32_t(ix) + g.nx * (uint32_t(iy) + g.ny * uint32_t(iz));
 }
 
-// Tangent accumulation in fixed point, 2^-16 per unit, packed in two
-// 64-bit integer sums per voxel: word 0 = x * 2^32 + y, word 1 = z (both
-// sign-extended).  Integer atomics make the sums independent of the order
-// in which threads add them; word 0 decodes exactly while |sum y| < 2^31,
-// i.e. for fewer than 32768 contributions to one voxel.
+// Tangent accumulation in fixed point, 2^-16 per unit, packed with the count
+// in two 64-bit integer sums per voxel: word 0 = x * 2^32 + y (sign-extended
+// fields), word 1 = z * 2^32 + count.  Integer atomics make the sums
+// independent of the order in which threads add them; word 0 decodes exactly
+// while |sum y| < 2^31, i.e. for fewer than 32768 contributions to one voxel,
+// and the low half of word 1 is the exact count whatever z does.  Two
+// atomics per sample instead of three (count, x/y, z): device atomics are
+// bound by the number of line requests.
 constexpr float kTangentScale = 65536.0f;
 
 __device__ __forceinline__ long long tangent_fixed(float v) {
@@ -46,11 +49,23 @@ __device__ __forceinline__ long long tangent_fixed(float v) {
     return (long long)rintf(v * kTangentScale);  // NaN components become 0 via the clamp
 }
 
-__device__ __forceinline__ void add_tangent(long long* __restrict__ acc, uint32_t v, float3 t) {
+// Adds one sample to voxel v: the count alone, or count and tangent packed.
+__device__ __forceinline__ void add_sample(uint32_t* __restrict__ counts, long long* __restrict__ acc, uint32_t v,
+                                           float3 t) {
+    if (!acc) {
+        atomicAdd(&counts[v], 1u);
+        return;
+    }
     long long x = tangent_fixed(t.x), y = tangent_fixed(t.y), z = tangent_fixed(t.z);
     unsigned long long w0 = (unsigned long long)(x) * (1ull << 32) + (unsigned long long)(y);
+    unsigned long long w1 = (unsigned long long)(z) * (1ull << 32) + 1ull;
     atomicAdd(reinterpret_cast<unsigned long long*>(&acc[2ull * v + 0ull]), w0);
-    atomicAdd(reinterpret_cast<unsigned long long*>(&acc[2ull * v + 1ull]), (unsigned long long)(z));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&acc[2ull * v + 1ull]), w1);
+}
+
+// The count of voxel v from word 1 of the packed accumulator.
+__device__ __forceinline__ uint32_t packed_count(const long long* __restrict__ acc, size_t v) {
+    return uint32_t((unsigned long long)(acc[2 * v + 1]) & 0xffffffffull);
 }
 
 // One vertex (hair_style.cc:272-281): count + optional tangent sum.
@@ -63,8 +78,7 @@ __device__ __forceinline__ void voxelize_vertex(const VoxelGrid& g, const float*
     float py = (p.y - g.oy) / g.vy;
     float pz = (p.z - g.oz) / g.vz;
     uint32_t v = voxel_of(g, px, py, pz);
-    atomicAdd(&counts[v], 1u);
-    if (sums) add_tangent(sums, v, load_xyz(tangents, i));
+    add_sample(counts, sums, v, sums ? load_xyz(tangents, i) : make_float3(0.0f, 0.0f, 0.0f));
 }
 
 // One index pair (hair_style.cc:311-329): the sequential float DDA.
@@ -94,8 +108,7 @@ __device__ __forceinline__ void voxelize_segment(const VoxelGrid& g, const float
     while (steps > 0.0f) {
         steps -= 1.0f;
         uint32_t v = voxel_of(g, rx, ry, rz);
-        atomicAdd(&counts[v], 1u);
-        if (sums) add_tangent(sums, v, t);
+        add_sample(counts, sums, v, t);
         rx += dx;
         ry += dy;
         rz += dz;

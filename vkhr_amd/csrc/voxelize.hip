@@ -36,6 +36,26 @@ voxelize_segments_kernel(const float* __restrict__ xyz, uint32_t n_vertices,
     }
 }
 
+// counts[v] = the count held in word 1 of the packed accumulator.
+__global__ void __launch_bounds__(256)
+unpack_counts_kernel(const long long* __restrict__ acc, uint32_t* __restrict__ counts, size_t n) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t v = size_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride) counts[v] = packed_count(acc, v);
+}
+
+// Zeroes what the voxelizer accumulates into: the packed accumulator when
+// there is one (the counts are unpacked from it afterwards), else the counts.
+int clear_targets(uint32_t* counts, int64_t* acc, size_t n, hipStream_t s) {
+    return acc ? fill_u32(reinterpret_cast<uint32_t*>(acc), 4 * n, 0u, s) : fill_u32(counts, n, 0u, s);
+}
+
+int unpack_counts(uint32_t* counts, const int64_t* acc, size_t n, hipStream_t s) {
+    if (!acc) return VKHR_OK;
+    unpack_counts_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(reinterpret_cast<const long long*>(acc), counts, n);
+    VKHR_CHECK_LAUNCH();
+    return VKHR_OK;
+}
+
 }  // namespace
 }  // namespace vkhr
 
@@ -54,13 +74,13 @@ extern "C" int vkhr_voxelize_vertices(const float* xyz, uint32_t n_vertices, con
     VKHR_REQUIRE(!tangent_acc || tangents, VKHR_ERR_INVALID, "tangent_acc needs tangents");
     hipStream_t s = as_stream(stream);
     size_t n = size_t(nx) * ny * nz;
-    if ((rc = fill_u32(counts, n, 0u, s))) return rc;
-    if (tangent_acc && (rc = fill_u32(reinterpret_cast<uint32_t*>(tangent_acc), 4 * n, 0u, s))) return rc;
-    if (n_vertices == 0) return VKHR_OK;
-    voxelize_vertices_kernel<<<grid_for(n_vertices, 256, 65536), 256, 0, s>>>(xyz, n_vertices, tangents, g,
-                                                                             counts, reinterpret_cast<long long*>(tangent_acc));
-    VKHR_CHECK_LAUNCH();
-    return VKHR_OK;
+    if ((rc = clear_targets(counts, tangent_acc, n, s))) return rc;
+    if (n_vertices) {
+        voxelize_vertices_kernel<<<grid_for(n_vertices, 256, 65536), 256, 0, s>>>(
+            xyz, n_vertices, tangents, g, counts, reinterpret_cast<long long*>(tangent_acc));
+        VKHR_CHECK_LAUNCH();
+    }
+    return unpack_counts(counts, tangent_acc, n, s);
 }
 
 extern "C" int vkhr_voxelize_segments(const float* xyz, uint32_t n_vertices, const uint32_t* indices,
@@ -78,11 +98,11 @@ extern "C" int vkhr_voxelize_segments(const float* xyz, uint32_t n_vertices, con
     VKHR_REQUIRE(!tangent_acc || tangents, VKHR_ERR_INVALID, "tangent_acc needs tangents");
     hipStream_t s = as_stream(stream);
     size_t n = size_t(nx) * ny * nz;
-    if ((rc = fill_u32(counts, n, 0u, s))) return rc;
-    if (tangent_acc && (rc = fill_u32(reinterpret_cast<uint32_t*>(tangent_acc), 4 * n, 0u, s))) return rc;
-    if (n_pairs == 0) return VKHR_OK;
-    voxelize_segments_kernel<<<grid_for(n_pairs, 256, 65536), 256, 0, s>>>(xyz, n_vertices, indices, n_pairs,
-                                                                           tangents, g, counts, reinterpret_cast<long long*>(tangent_acc));
-    VKHR_CHECK_LAUNCH();
-    return VKHR_OK;
+    if ((rc = clear_targets(counts, tangent_acc, n, s))) return rc;
+    if (n_pairs) {
+        voxelize_segments_kernel<<<grid_for(n_pairs, 256, 65536), 256, 0, s>>>(
+            xyz, n_vertices, indices, n_pairs, tangents, g, counts, reinterpret_cast<long long*>(tangent_acc));
+        VKHR_CHECK_LAUNCH();
+    }
+    return unpack_counts(counts, tangent_acc, n, s);
 }
