@@ -81,6 +81,7 @@ public:
         DeviceBuffer density;       // uint8 (min(count, 255), then normalized)
         DeviceBuffer tangent;       // int8x4
         DeviceBuffer cells;         // corner-packed density (raymarch layout)
+        DeviceBuffer occupancy;     // brick occupancy of the cells (empty bricks are skipped)
         bool normalized = false;
     };
 

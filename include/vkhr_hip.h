@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define VKHR_ABI_VERSION 1
+#define VKHR_ABI_VERSION 2
 
 enum vkhr_status {
     VKHR_OK = 0,
@@ -155,7 +155,9 @@ int vkhr_generate_tangents(const float* xyz, const uint32_t* strand_offsets,
  * (nullable, 2 int64 per voxel) receives the per-voxel sum of the tangents
  * of all contributions in fixed point, 2^-16 per unit, packed with the
  * count as word0 = x*2^32 + y and word1 = z*2^32 + count (sign-extended
- * fields; counts is then unpacked from word1 at the end of the call);
+ * fields).  With tangent_acc, counts may be NULL: the counts then stay
+ * packed (vkhr_volume_finalize unpacks them); otherwise they are unpacked
+ * into counts at the end of the call (one more pass over tangent_acc);
  * integer sums, so the result does not depend on thread order (the
  * reference sums floats serially).  tangents must then be non-null.  Requires nx*ny*nz <= 2^24
  * (the float index is exact only up to there). */
@@ -185,6 +187,8 @@ int vkhr_voxelize_segments(const float* xyz, uint32_t n_vertices,
 /* Turns raw counts into the volumes the renderers sample, as the reference's
  * load path does (rasterizer/hair_style.cc:75-77, hair_style.cc:283-291,
  * 331-357):
+ * The counts are read from counts, or, when it is NULL, from the packed
+ * tangent_acc of the voxelizer.
  *   density_u8   = min(count, 255), then, if normalize != 0,
  *                  HairStyle::Volume::normalize (min/max stretch to 0..255,
  *                  float scale, truncation; a constant volume becomes 0);
@@ -193,18 +197,31 @@ int vkhr_voxelize_segments(const float* xyz, uint32_t n_vertices,
  *                  for voxels of 32768 or more contributions (nullable;
  *                  needs tangent_acc);
  *   density_f32  = density_u8 / 255 (nullable; the R8_UNORM value);
+ *                  density_u8 must be 4-byte and counts, density_f32
+ *                  and tangent_i8x4 16-byte aligned (device allocations
+ *                  are);
  *   cells        = corner-packed density (nullable): for every cell
  *                  (i,j,k), i,j,k in [-1, n-1], the 8 texels of its
  *                  trilinear footprint as one uint64 (byte b = texel
  *                  (i+(b&1), j+(b>>1&1), k+(b>>2)), 0 outside the volume);
  *                  (nx+1)*(ny+1)*(nz+1) entries, index (i+1) + (j+1)*(nx+1)
  *                  + (k+1)*(nx+1)*(ny+1).  The raymarcher reads this layout:
- *                  one 8-byte gather per trilinear tap. */
+ *                  one 8-byte gather per trilinear tap;
+ *   occupancy    = one bit per brick of 8x8x8 cells, set when a cell of
+ *                  the brick holds a nonzero texel (nullable; needs cells;
+ *                  vkhr_occupancy_words(nx, ny, nz) uint32 words, brick
+ *                  (bx,by,bz) = cell (i+1,j+1,k+1) >> 3 at bit bx + BX*(by +
+ *                  BY*bz) with BX = (nx+8)/8, BY = (ny+8)/8).  The raymarcher
+ *                  skips the gathers of samples in empty bricks (their taps
+ *                  read 0: the result is unchanged). */
 int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tangent_acc,
                          uint32_t nx, uint32_t ny, uint32_t nz, int normalize,
                          uint8_t* density_u8, int8_t* tangent_i8x4,
-                         float* density_f32, uint64_t* cells,
+                         float* density_f32, uint64_t* cells, uint32_t* occupancy,
                          vkhr_stream stream);
+
+/* Words of the brick occupancy bitmask of an nx x ny x nz volume. */
+size_t vkhr_occupancy_words(uint32_t nx, uint32_t ny, uint32_t nz);
 
 /* --------------------------------------------------------------- raymarch */
 
@@ -218,6 +235,7 @@ typedef struct vkhr_volume {
     const float* density_f32;    /* nullable: float densities, used instead of cells when set */
     uint32_t nx, ny, nz;
     uint32_t reserved_;
+    const uint32_t* occupancy;   /* nullable: brick occupancy of the cells (vkhr_volume_finalize) */
 } vkhr_volume;
 
 /* Per-pixel volume raymarch of one hair style: the raymarcher pass
@@ -373,11 +391,12 @@ typedef struct vkhr_batch_style {
     vkhr_aabb bounds;             /* the style's AABB (vkhr_aabb_compute) */
     uint32_t volume;              /* n of the n^3 volume */
     uint32_t reserved_;
-    uint32_t* counts;             /* n^3 */
+    uint32_t* counts;             /* n^3; nullable with tangent_acc */
     int64_t* tangent_acc;         /* 2 n^3, nullable */
     uint8_t* density;             /* n^3, normalized */
     int8_t* tangent_i8x4;         /* 4 n^3, nullable */
     uint64_t* cells;              /* (n+1)^3 */
+    uint32_t* occupancy;          /* vkhr_occupancy_words(n, n, n), nullable */
     float* rgba;                  /* width*height float4 */
     vkhr_camera camera;
     vkhr_light light;
@@ -388,7 +407,8 @@ typedef struct vkhr_batch_style {
 /* Launch shapes, chosen by the autotuner (vkhr_amd/tune.py). */
 typedef struct vkhr_tuning {
     uint32_t tile_w, tile_h;      /* pixels per wave tile, tile_w * tile_h == 64 */
-    uint32_t blocks_per_cu;       /* persistent raymarch blocks per compute unit */
+    uint32_t blocks_per_cu;       /* persistent raymarch blocks per compute unit; 0: one block
+                                     per (style, tile) work item */
     uint32_t reserved_;
 } vkhr_tuning;
 
@@ -396,14 +416,15 @@ typedef struct vkhr_batch vkhr_batch;
 
 /* Builds the device tables of a batch: voxelize_segments + finalize
  * (normalize, tangents, cells) + raymarch at width x height for every
- * style.  tuning may be NULL (8x8 wave tiles, 4 blocks per CU). */
+ * style.  tuning may be NULL (8x8 wave tiles, one block per work item). */
 int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styles, uint32_t width, uint32_t height,
                       const vkhr_tuning* tuning, vkhr_batch** out);
 
 /* Runs the whole batch as one hipGraph (captured on the first call on a
  * created, non-NULL stream, replayed after): a clear, a voxelization over
- * the concatenated segments of all styles, chunked finalize / normalize /
- * cell kernels, and a persistent raymarch over (style, tile) work items.
+ * the concatenated segments of all styles, chunked finalize and cell
+ * kernels (the cells carry the normalized density), and a persistent
+ * raymarch over (style, tile) work items.
  * Per-style results are bit-identical to vkhr_voxelize_segments +
  * vkhr_volume_finalize(normalize=1) + vkhr_raymarch. */
 int vkhr_batch_launch(vkhr_batch* batch, vkhr_stream stream);

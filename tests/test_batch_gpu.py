@@ -30,6 +30,7 @@ def _plain_outputs(b):
                density=torch.empty(nv, dtype=torch.uint8, device=dev),
                tangents=torch.empty(4 * nv, dtype=torch.int8, device=dev),
                cells=torch.empty((b.n + 1) ** 3, dtype=torch.int64, device=dev),
+               occupancy=torch.empty_like(b.occupancy),
                rgba=torch.empty((H, W, 4), dtype=torch.float32, device=dev))
     b.run_plain(out)
     return out
@@ -52,7 +53,7 @@ def test_batch_matches_per_style_bit_for_bit(batch32, tuning):
     for b in batch32:
         ref = _plain_outputs(b)
         for k, mine in [("counts", b.counts), ("acc", b.acc), ("density", b.density), ("tangents", b.tangents),
-                        ("cells", b.cells), ("rgba", b.rgba)]:
+                        ("cells", b.cells), ("occupancy", b.occupancy), ("rgba", b.rgba)]:
             assert torch.equal(mine, ref[k]), (k, b.n)
         del ref
     # replays of the captured graph give the same frame

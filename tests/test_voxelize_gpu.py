@@ -152,3 +152,58 @@ def test_finalize_without_normalize(device):
     ref, counts, _ = _check_counts(s, res, device, True)
     vol = D.finalize(counts, res, normalize=False, want_cells=False)
     assert np.array_equal(vol["density"].cpu().numpy(), ref["dens_u8"])
+
+
+def test_packed_counts_path_matches(device):
+    """With the tangent accumulator and no count volume the counts stay
+    packed in the accumulator's second word; finalize unpacks them and gives
+    the same volumes as from the unpacked counts (cells also without the
+    normalize)."""
+    s = synth.config0()
+    res = (128, 128, 128)
+    ref, counts, sums = _check_counts(s, res, device, True)
+    xyz = D.to_dev(s.vertices, device)
+    off = D.to_dev(s.strand_offsets, device)
+    idx = D.generate_indices(off)
+    tan = D.generate_tangents(xyz, off)
+    none, packed = D.voxelize(xyz, D.aabb(xyz), res, indices=idx, tangents=tan, want_sums=True, want_counts=False)
+    assert none is None
+    unpacked = (packed.view(-1, 2)[:, 1] & 0xFFFFFFFF).cpu().numpy().astype(np.uint32)
+    assert np.array_equal(unpacked, ref["counts"])
+    for normalize in (True, False):
+        a = D.finalize(counts, res, sums=sums, normalize=normalize, want_f32=True, want_cells=True)
+        b = D.finalize(None, res, sums=packed, normalize=normalize, want_f32=True, want_cells=True)
+        for k in ("density", "tangents", "density_f32", "cells"):
+            assert torch.equal(a[k], b[k]), (normalize, k)
+    # the cells of the unnormalized view hold the saturating uint8 counts
+    c = D.finalize(None, res, sums=packed, normalize=False, want_cells=True)["cells"].cpu().numpy().view(np.uint64)
+    u8 = ref["dens_u8"].reshape(128, 128, 128)
+    assert int(c.reshape(129, 129, 129)[5, 7, 9] & np.uint64(0xFF)) == int(u8[4, 6, 8])
+
+
+def _occupancy_from_cells(cells: np.ndarray, res) -> np.ndarray:
+    """One bit per brick of 8x8x8 cells, set when a cell of the brick is
+    nonzero (the layout of vkhr_volume_finalize's occupancy)."""
+    nx, ny, nz = res
+    c = cells.view(np.uint64).reshape(nz + 1, ny + 1, nx + 1) != 0
+    bx, by, bz = (nx + 8) // 8, (ny + 8) // 8, (nz + 8) // 8
+    pad = np.zeros((bz * 8, by * 8, bx * 8), bool)
+    pad[:nz + 1, :ny + 1, :nx + 1] = c
+    bricks = pad.reshape(bz, 8, by, 8, bx, 8).any(axis=(1, 3, 5)).reshape(-1)
+    words = np.zeros((bricks.size + 31) // 32, np.uint32)
+    idx = np.nonzero(bricks)[0]
+    np.bitwise_or.at(words, idx >> 5, (np.uint32(1) << (idx & 31).astype(np.uint32)))
+    return words
+
+
+@pytest.mark.parametrize("res", [(128, 128, 128), (40, 72, 9)])
+def test_occupancy_bits(device, res):
+    s = synth.config0()
+    _, counts, sums = _check_counts(s, res, device, True)
+    for normalize in (True, False):
+        vol = D.finalize(counts, res, sums=sums, normalize=normalize, want_cells=True)
+        got = vol["occupancy"].cpu().numpy().view(np.uint32)
+        assert got.size == D.occupancy_words(res)
+        want = _occupancy_from_cells(vol["cells"].cpu().numpy(), res)
+        assert np.array_equal(got, want)
+        assert 0 < int(np.unpackbits(want.view(np.uint8)).sum()) < 32 * want.size

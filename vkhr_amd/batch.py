@@ -31,7 +31,9 @@ class StyleBuffers:
     """Device inputs and outputs of one style of a batch."""
 
     def __init__(self, style: synth.Style, n: int, width: int, height: int, device, azimuth: float = 0.0,
-                 steps: float = 512.0):
+                 steps: float = 512.0, with_counts: bool = True):
+        """with_counts=False leaves the counts packed in the tangent
+        accumulator (no separate uint32 count volume is written)."""
         self.n = n
         self.xyz = D.to_dev(style.vertices, device)
         off = D.to_dev(style.strand_offsets, device)
@@ -39,11 +41,12 @@ class StyleBuffers:
         self.tan = D.generate_tangents(self.xyz, off)
         self.bounds = D.aabb(self.xyz)
         nv = n ** 3
-        self.counts = torch.empty(nv, dtype=torch.int32, device=device)
+        self.counts = torch.empty(nv, dtype=torch.int32, device=device) if with_counts else None
         self.acc = torch.empty(2 * nv, dtype=torch.int64, device=device)
         self.density = torch.empty(nv, dtype=torch.uint8, device=device)
         self.tangents = torch.empty(4 * nv, dtype=torch.int8, device=device)
         self.cells = torch.empty((n + 1) ** 3, dtype=torch.int64, device=device)
+        self.occupancy = torch.empty(D.occupancy_words((n, n, n)), dtype=torch.int32, device=device)
         self.rgba = torch.empty((height, width, 4), dtype=torch.float32, device=device)
         sc = synth.make_scene(self.bounds, width, height, azimuth_deg=azimuth)
         self.cam, self.light = params.camera(sc), params.light(sc)
@@ -59,6 +62,7 @@ class StyleBuffers:
         b.volume = self.n
         b.counts, b.tangent_acc, b.density = ptr(self.counts), ptr(self.acc), ptr(self.density)
         b.tangent_i8x4, b.cells, b.rgba = ptr(self.tangents), ptr(self.cells), ptr(self.rgba)
+        b.occupancy = ptr(self.occupancy)
         b.camera, b.light, b.strand, b.params = self.cam, self.light, self.sp, self.rp
         return b
 
@@ -66,13 +70,14 @@ class StyleBuffers:
         """The per-style sequence: voxelize_segments -> finalize -> raymarch,
         into separate output buffers (out) or this style's own."""
         o = out if out is not None else dict(counts=self.counts, acc=self.acc, density=self.density,
-                                             tangents=self.tangents, cells=self.cells, rgba=self.rgba)
+                                             tangents=self.tangents, cells=self.cells, occupancy=self.occupancy,
+                                             rgba=self.rgba)
         res = (self.n,) * 3
         D.voxelize(self.xyz, self.bounds, res, indices=self.idx, tangents=self.tan, want_sums=True,
-                   counts=o["counts"], sums=o["acc"])
+                   counts=o["counts"], sums=o["acc"], want_counts=o["counts"] is not None)
         call("vkhr_volume_finalize", ptr(o["counts"]), ptr(o["acc"]), self.n, self.n, self.n, 1, ptr(o["density"]),
-             ptr(o["tangents"]), None, ptr(o["cells"]), capi.stream_of())
-        vol = dict(density=o["density"], cells=o["cells"], tangents=o["tangents"])
+             ptr(o["tangents"]), None, ptr(o["cells"]), ptr(o.get("occupancy")), capi.stream_of())
+        vol = dict(density=o["density"], cells=o["cells"], tangents=o["tangents"], occupancy=o.get("occupancy"))
         D.raymarch(vol, res, self.cam, self.light, self.sp, self.rp, self.width, self.height, want_depth=False,
                    out=o["rgba"])
 

@@ -61,7 +61,7 @@ class RenderParams(C.Structure):
 class Volume(C.Structure):
     _fields_ = [("density", C.c_void_p), ("cells", C.c_void_p), ("tangents", C.c_void_p),
                 ("density_f32", C.c_void_p), ("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32),
-                ("reserved_", C.c_uint32)]
+                ("reserved_", C.c_uint32), ("occupancy", C.c_void_p)]
 
 
 class Ppll(C.Structure):
@@ -82,7 +82,8 @@ class BatchStyle(C.Structure):
     _fields_ = [("xyz", C.c_void_p), ("n_vertices", C.c_uint32), ("n_indices", C.c_uint32),
                 ("indices", C.c_void_p), ("tangents", C.c_void_p), ("bounds", AABB), ("volume", C.c_uint32),
                 ("reserved_", C.c_uint32), ("counts", C.c_void_p), ("tangent_acc", C.c_void_p),
-                ("density", C.c_void_p), ("tangent_i8x4", C.c_void_p), ("cells", C.c_void_p), ("rgba", C.c_void_p),
+                ("density", C.c_void_p), ("tangent_i8x4", C.c_void_p), ("cells", C.c_void_p), ("occupancy", C.c_void_p),
+                ("rgba", C.c_void_p),
                 ("camera", Camera), ("light", Light), ("strand", StrandParams), ("params", RenderParams)]
 
 
@@ -104,7 +105,7 @@ SIGNATURES = {
     "vkhr_generate_tangents": [P, P, U32, P, P],
     "vkhr_voxelize_vertices": [P, U32, P, C.POINTER(AABB), U32, U32, U32, P, P, P],
     "vkhr_voxelize_segments": [P, U32, P, U32, P, C.POINTER(AABB), U32, U32, U32, P, P, P],
-    "vkhr_volume_finalize": [P, P, U32, U32, U32, I32, P, P, P, P, P],
+    "vkhr_volume_finalize": [P, P, U32, U32, U32, I32, P, P, P, P, P, P],
     "vkhr_raymarch": [C.POINTER(Volume), C.POINTER(Camera), C.POINTER(Light), C.POINTER(StrandParams),
                       C.POINTER(RenderParams), U32, U32, P, P, P, P, P],
     "vkhr_raymarch_ppll": [C.POINTER(Volume), C.POINTER(Camera), C.POINTER(Light), C.POINTER(StrandParams),
@@ -124,7 +125,7 @@ SIGNATURES = {
     "vkhr_batch_destroy": [P],
 }
 # entry points that do not return a status
-SIZE_FUNCTIONS = {"vkhr_march_record_bytes": [U32, U32]}
+SIZE_FUNCTIONS = {"vkhr_march_record_bytes": [U32, U32], "vkhr_occupancy_words": [U32, U32, U32]}
 
 
 def declared_symbols() -> list[str]:

@@ -21,9 +21,10 @@ struct BatchVolStyle {
     uint8_t* density;
     int8_t* tangent_q;
     unsigned long long* cells;
+    uint32_t* occupancy;                        // nullable
 };
 
-// A contiguous range of one style's voxels (or cells).
+// A contiguous range of one style's voxels (or one block of its cells).
 struct BatchChunk {
     uint32_t style;
     uint32_t reserved;

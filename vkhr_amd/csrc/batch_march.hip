@@ -1,8 +1,9 @@
 // Batched raymarch (persistent blocks pulling (style, tile) work items from
 // an atomic queue) and the host side of the batched launch: device tables,
-// one hipGraph capturing clear -> voxelize -> finalize -> normalize -> cells
+// one hipGraph capturing clear -> voxelize -> finalize -> cells (normalized)
 // -> raymarch, replayed by vkhr_batch_launch.
 #include "batch.hpp"
+#include "volume.hpp"
 #include "voxel.hpp"
 
 #include <algorithm>
@@ -12,32 +13,64 @@
 namespace vkhr {
 namespace {
 
-__global__ void __launch_bounds__(256)
-batch_raymarch_kernel(BatchDevice d) {
-    __shared__ uint32_t item;
+// Work item it = one block tile (2x2 wave tiles of tile_w x tile_h pixels)
+// of one style; all threads of the block call it with the same it.
+__device__ __forceinline__ void batch_item(const BatchDevice& d, const RaymarchArgs* __restrict__ args,
+                                           uint32_t it, uint32_t* occ_lds, uint32_t* staged, bool* use_occ) {
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int tw = int(d.tile_w), th = int(d.tile_h);
+    const uint32_t s = uint32_t(__builtin_amdgcn_readfirstlane(int(style_of(d.tile_prefix, d.n_styles, it))));
+    // a read-only, non-aliased argument: the uniform loads of the style's
+    // arguments become scalar loads instead of filling vector registers
+    const RaymarchArgs& a = args[s];
+    if (s != *staged) {  // block-uniform
+        *use_occ = stage_occupancy(a.vol, occ_lds);
+        *staged = s;
+    }
+    const uint32_t local = it - d.tile_prefix[s];
+    const int bx = int(local % d.tiles_x), by = int(local / d.tiles_x);
+    const int px = bx * 2 * tw + (wave & 1) * tw + lane % tw;
+    const int py = by * 2 * th + (wave >> 1) * th + lane / tw;
+    raymarch_pixel<false>(a, px, py, OccupancyView{occ_lds, *use_occ});
+}
+
+// One block per work item: the hardware dispatcher balances the tiles of all
+// styles.
+__global__ void __launch_bounds__(256)
+batch_raymarch_grid_kernel(BatchDevice d, const RaymarchArgs* __restrict__ args) {
+    __shared__ uint32_t occ_lds[kOccupancyLdsWords];
+    uint32_t staged = 0xffffffffu;
+    bool use_occ = false;
+    batch_item(d, args, uint32_t(__builtin_amdgcn_readfirstlane(int(blockIdx.x))), occ_lds, &staged, &use_occ);
+}
+
+// Persistent blocks pulling work items from an atomic queue.  With the pixel
+// body inlined into the loop the compiler allocates 141 registers (3 waves
+// per SIMD against the grid form's 5); the autotuner measures both.
+__global__ void __launch_bounds__(256)
+batch_raymarch_kernel(BatchDevice d, const RaymarchArgs* __restrict__ args) {
+    __shared__ uint32_t item;
+    __shared__ uint32_t occ_lds[kOccupancyLdsWords];
+    uint32_t staged = 0xffffffffu;   // the style whose occupancy is in LDS
+    bool use_occ = false;
     for (;;) {
         if (threadIdx.x == 0) item = atomicAdd(d.work_counter, 1u);
         __syncthreads();
-        const uint32_t it = item;
+        const uint32_t it = uint32_t(__builtin_amdgcn_readfirstlane(int(item)));
         __syncthreads();
         if (it >= d.total_tiles) break;
-        const uint32_t s = style_of(d.tile_prefix, d.n_styles, it);
-        const RaymarchArgs& a = d.args[s];
-        const uint32_t local = it - d.tile_prefix[s];
-        const int bx = int(local % d.tiles_x), by = int(local / d.tiles_x);
-        // a block tile is 2x2 wave tiles of tw x th pixels
-        const int px = bx * 2 * tw + (wave & 1) * tw + lane % tw;
-        const int py = by * 2 * th + (wave >> 1) * th + lane / tw;
-        raymarch_pixel(a, px, py);
+        batch_item(d, args, it, occ_lds, &staged, &use_occ);
     }
 }
 
 }  // namespace
 
+// grid == 0: one block per work item; else that many persistent blocks.
 void launch_batch_march(const BatchDevice& d, unsigned grid, hipStream_t stream) {
-    batch_raymarch_kernel<<<grid, 256, 0, stream>>>(d);
+    if (grid == 0)
+        batch_raymarch_grid_kernel<<<d.total_tiles, 256, 0, stream>>>(d, d.args);
+    else
+        batch_raymarch_kernel<<<grid, 256, 0, stream>>>(d, d.args);
 }
 
 }  // namespace vkhr
@@ -81,11 +114,11 @@ extern "C" int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styl
     if (rc) return rc;
     VKHR_REQUIRE(styles && n_styles && out, VKHR_ERR_INVALID, "no styles");
     VKHR_REQUIRE(width && height, VKHR_ERR_INVALID, "empty image");
-    vkhr_tuning t = {8, 8, 4, 0};
+    vkhr_tuning t = {8, 8, 0, 0};
     if (tuning) t = *tuning;
     VKHR_REQUIRE(t.tile_w * t.tile_h == 64 && t.tile_w >= 1 && t.tile_h >= 1, VKHR_ERR_INVALID,
                  "wave tile %ux%u is not 64 pixels", t.tile_w, t.tile_h);
-    VKHR_REQUIRE(t.blocks_per_cu >= 1 && t.blocks_per_cu <= 32, VKHR_ERR_INVALID, "blocks_per_cu %u", t.blocks_per_cu);
+    VKHR_REQUIRE(t.blocks_per_cu <= 32, VKHR_ERR_INVALID, "blocks_per_cu %u", t.blocks_per_cu);
     vkhr_batch* b = new vkhr_batch();
     std::memset(&b->d, 0, sizeof(b->d));
     std::vector<BatchVolStyle> vs(n_styles);
@@ -93,7 +126,7 @@ extern "C" int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styl
     std::vector<uint64_t> pair_prefix(n_styles + 1, 0);
     std::vector<uint32_t> tile_prefix(n_styles + 1, 0);
     std::vector<BatchChunk> vchunks, cchunks;
-    const uint64_t kChunk = 1u << 16;
+    const uint64_t kChunk = 1u << 16;   // voxels per finalize chunk
     const uint32_t bw = 2 * t.tile_w, bh = 2 * t.tile_h;
     const uint32_t tiles_x = (width + bw - 1) / bw, tiles_y = (height + bh - 1) / bh;
     for (uint32_t s = 0; s < n_styles; ++s) {
@@ -103,7 +136,7 @@ extern "C" int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styl
             delete b;
             return rc;
         }
-        if (!(in.counts && in.density && in.cells && in.rgba && (in.n_indices < 2 || (in.xyz && in.indices)))) {
+        if (!((in.counts || in.tangent_acc) && in.density && in.cells && in.rgba && (in.n_indices < 2 || (in.xyz && in.indices)))) {
             delete b;
             set_error("style %u: missing buffer", s);
             return VKHR_ERR_INVALID;
@@ -121,6 +154,7 @@ extern "C" int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styl
         v.density = in.density;
         v.tangent_q = in.tangent_i8x4;
         v.cells = reinterpret_cast<unsigned long long*>(in.cells);
+        v.occupancy = in.occupancy;
         if (v.acc && !(v.tangents && v.tangent_q)) {
             delete b;
             set_error("style %u: tangent_acc needs tangents and tangent_i8x4", s);
@@ -128,13 +162,14 @@ extern "C" int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styl
         }
         pair_prefix[s + 1] = pair_prefix[s] + v.n_pairs;
         uint64_t nvox = uint64_t(in.volume) * in.volume * in.volume;
-        uint64_t ncell = uint64_t(in.volume + 1) * (in.volume + 1) * (in.volume + 1);
         for (uint64_t a = 0; a < nvox; a += kChunk) vchunks.push_back({s, 0, a, std::min(nvox, a + kChunk)});
-        for (uint64_t a = 0; a < ncell; a += kChunk) cchunks.push_back({s, 0, a, std::min(ncell, a + kChunk)});
+        const uint64_t blocks = uint64_t(cell_col_blocks(in.volume)) * bricks_along(in.volume) * (in.volume + 1);
+        for (uint64_t a = 0; a < blocks; ++a) cchunks.push_back({s, 0, a, a + 1});
         vkhr_volume vol;
         std::memset(&vol, 0, sizeof(vol));
         vol.density = in.density;
         vol.cells = in.cells;
+        vol.occupancy = in.occupancy;
         vol.tangents = in.tangent_i8x4;
         vol.nx = vol.ny = vol.nz = in.volume;
         if ((rc = make_raymarch_args(&vol, &in.camera, &in.light, &in.strand, &in.params, width, height, &args[s]))) {

@@ -18,7 +18,7 @@ __device__ __forceinline__ VoxelGrid grid_of(const BatchVolStyle& s) {
     return g;
 }
 
-// Zeroes every style's counts or tangent accumulator, resets the per-style
+// Zeroes every style's counts or tangent accumulator and occupancy, resets the per-style
 // min/max and the raymarch work queue.
 __global__ void __launch_bounds__(256)
 batch_clear_kernel(BatchDevice d) {
@@ -29,10 +29,17 @@ batch_clear_kernel(BatchDevice d) {
         }
         if (threadIdx.x == 0) *d.work_counter = 0u;
     }
+    for (uint32_t s = blockIdx.x; s < d.n_styles; s += gridDim.x) {
+        const BatchVolStyle& st = d.styles[s];
+        if (!st.occupancy) continue;
+        const size_t words = occupancy_words(st.nx, st.ny, st.nz);
+        for (size_t w = threadIdx.x; w < words; w += blockDim.x) st.occupancy[w] = 0u;
+    }
     for (uint32_t c = blockIdx.x; c < d.n_voxel_chunks; c += gridDim.x) {
         const BatchChunk ch = d.voxel_chunks[c];
         const BatchVolStyle& st = d.styles[ch.style];
         // with the packed accumulator the counts are unpacked by finalize
+        // (when they are asked for)
         for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
             if (st.acc) {
                 st.acc[2 * i] = 0;
@@ -44,15 +51,20 @@ batch_clear_kernel(BatchDevice d) {
     }
 }
 
+// Grid-stride over wave-aligned ranges of the concatenated segments: the
+// lanes of a wave may belong to different styles.
 __global__ void __launch_bounds__(256)
 batch_voxelize_kernel(BatchDevice d) {
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; g < d.total_pairs; g += stride) {
-        const uint32_t s = style_of(d.pair_prefix, d.n_styles, g);
+    for (uint64_t base = uint64_t(blockIdx.x) * blockDim.x + (threadIdx.x & ~uint32_t(kWave - 1));
+         base < d.total_pairs; base += stride) {
+        const uint64_t g = base + (threadIdx.x & uint32_t(kWave - 1));
+        const bool has = g < d.total_pairs;
+        const uint32_t s = has ? style_of(d.pair_prefix, d.n_styles, g) : 0u;
         const BatchVolStyle& st = d.styles[s];
-        const uint32_t p = uint32_t(g - d.pair_prefix[s]);
-        voxelize_segment(grid_of(st), st.xyz, st.n_vertices, st.indices[2u * p], st.indices[2u * p + 1u],
-                         st.tangents, st.counts, st.acc);
+        const uint32_t p = has ? uint32_t(g - d.pair_prefix[s]) : 0u;
+        voxelize_segment(grid_of(st), st.xyz, st.n_vertices, has, has ? st.indices[2u * p] : 0u,
+                         has ? st.indices[2u * p + 1u] : 0u, st.tangents, st.counts, st.acc);
     }
 }
 
@@ -65,7 +77,8 @@ batch_finalize_kernel(BatchDevice d) {
         const BatchVolStyle& st = d.styles[ch.style];
         uint32_t lo = 255u, hi = 0u;
         for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
-            uint32_t v = finalize_voxel(st.counts, st.acc, i, st.density, st.tangent_q, st.acc != nullptr);
+            uint32_t v = finalize_voxel(st.acc ? nullptr : st.counts, st.acc, i, st.density, st.tangent_q);
+            if (st.acc && st.counts) st.counts[i] = packed_count(st.acc, i);
             lo = v < lo ? v : lo;
             hi = v > hi ? v : hi;
         }
@@ -88,26 +101,30 @@ batch_finalize_kernel(BatchDevice d) {
     }
 }
 
+// Corner-packed cells of the normalized density and the brick occupancy:
+// a cell chunk is one cells_block (8 cell rows x 256 cell columns at one
+// cell level) of one style.
+__global__ void __launch_bounds__(256)
+batch_cells_kernel(BatchDevice d) {
+    __shared__ uint32_t flags[kCellBlockCols];
+    for (uint32_t c = blockIdx.x; c < d.n_cell_chunks; c += gridDim.x) {
+        const BatchChunk ch = d.cell_chunks[c];
+        const BatchVolStyle& st = d.styles[ch.style];
+        const uint32_t mcs = cell_col_blocks(st.nx), bys = bricks_along(st.ny), b = uint32_t(ch.begin);
+        const uint32_t mc = b % mcs, rest = b / mcs;
+        cells_block(st.density, st.nx, st.ny, st.nz, rest % bys, rest / bys, mc, st.cells, st.occupancy, flags);
+        __syncthreads();  // flags is reused by the next chunk
+    }
+}
+
 __global__ void __launch_bounds__(256)
 batch_normalize_kernel(BatchDevice d) {
     for (uint32_t c = blockIdx.x; c < d.n_voxel_chunks; c += gridDim.x) {
         const BatchChunk ch = d.voxel_chunks[c];
         const BatchVolStyle& st = d.styles[ch.style];
-        const uint32_t lo = d.minmax[2 * ch.style], hi = d.minmax[2 * ch.style + 1];
-        const bool flat = hi == lo;
-        const float scaling = normalize_scaling(lo, hi);
+        const DensityMap m = density_map(&d.minmax[2 * ch.style], true);
         for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x)
-            st.density[i] = uint8_t(normalize_value(st.density[i], lo, scaling, flat));
-    }
-}
-
-__global__ void __launch_bounds__(256)
-batch_cells_kernel(BatchDevice d) {
-    for (uint32_t c = blockIdx.x; c < d.n_cell_chunks; c += gridDim.x) {
-        const BatchChunk ch = d.cell_chunks[c];
-        const BatchVolStyle& st = d.styles[ch.style];
-        for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x)
-            st.cells[i] = pack_cell(st.density, st.nx, st.ny, st.nz, i);
+            st.density[i] = uint8_t(texel_density(st.density[i], m));
     }
 }
 
@@ -118,7 +135,7 @@ void launch_batch_volume(const BatchDevice& d, hipStream_t stream) {
     batch_voxelize_kernel<<<grid_for(d.total_pairs, 256, 65536), 256, 0, stream>>>(d);
     batch_finalize_kernel<<<4096, 256, 0, stream>>>(d);
     batch_normalize_kernel<<<4096, 256, 0, stream>>>(d);
-    batch_cells_kernel<<<4096, 256, 0, stream>>>(d);
+    batch_cells_kernel<<<8192, 256, 0, stream>>>(d);
 }
 
 }  // namespace vkhr

@@ -161,8 +161,11 @@ __device__ __forceinline__ void support_interval(const RaymarchArgs& a, const do
 
 // One pixel of the raymarch pass (volume.frag:31-102).  Every lane of a
 // wave must call it (the optional PPLL insert ballots the wave); lanes
-// outside the image take no fragment.
-__device__ __forceinline__ void raymarch_pixel(const RaymarchArgs& a, int px, int py) {
+// outside the image take no fragment.  occ: the LDS copy of the brick
+// occupancy (stage_occupancy), if any; kF32: the float density volume
+// (a.vol.f32) instead of the cells.
+template <bool kF32>
+__device__ __forceinline__ void raymarch_pixel(const RaymarchArgs& a, int px, int py, OccupancyView occ) {
     // Out-of-image lanes stay converged (the PPLL insert ballots the wave).
     const bool inside = px < a.width && py < a.height;
     const size_t pix = inside ? size_t(py) * a.width + px : 0;
@@ -190,13 +193,26 @@ __device__ __forceinline__ void raymarch_pixel(const RaymarchArgs& a, int px, in
         // there (the depth test before it changes nothing either).
         double t_lo, t_hi;
         support_interval(a, xs, dx, &t_lo, &t_hi);
+        // Below t_depth the depth test cannot end the march (z - db*w is
+        // affine in t and negative there, by a margin far above its
+        // rounding), so it is evaluated only from there on.
+        double t_depth = -1.0;
+        {
+            const double f0 = zs - db * ws, slope = dz - db * dw;
+            if (f0 < 0.0) {
+                t_depth = 2.0;
+                if (slope > 0.0)
+                    t_depth = -f0 / slope -
+                              (1e-9 + 1e-13 * (fabs(zs) + fabs(db * ws) + fabs(dz) + fabs(db * dw)) / slope);
+            }
+        }
         int i = 0;
         for (float t = 0.0f; t < 1.0f; t += a.step_size, ++i) {
             const double td = double(t);
             // depth test: db < z/w  <=>  db*w < z  (w > 0 in front of the eye)
-            if (db * fma(td, dw, ws) < fma(td, dz, zs)) break;
+            if (td >= t_depth && db * fma(td, dw, ws) < fma(td, dz, zs)) break;
             if (td > t_hi) break;
-            float d = density_at(a.vol, fma(td, dx[0], xs[0]), fma(td, dx[1], xs[1]), fma(td, dx[2], xs[2]));
+            float d = march_tap<kF32>(a.vol, fma(td, dx[0], xs[0]), fma(td, dx[1], xs[1]), fma(td, dx[2], xs[2]), occ);
             acc += double(d);
             if (d != 0.0f) {
                 if (acc <= a.iso) { surf = i; t_surf = t; }
@@ -247,8 +263,8 @@ __device__ __forceinline__ void raymarch_pixel(const RaymarchArgs& a, int px, in
                     const double td = double(t);
                     if (td > t_hi) break;
                     if (td < t_lo) continue;
-                    strands += double(density_at(a.vol, fma(td, xl[0], sx[0]), fma(td, xl[1], sx[1]),
-                                                 fma(td, xl[2], sx[2])));
+                    strands += double(march_tap<kF32>(a.vol, fma(td, xl[0], sx[0]), fma(td, xl[1], sx[1]),
+                                                      fma(td, xl[2], sx[2]), occ));
                 }
                 adsm = pow(1.0 - a.alpha, strands * a.adsm_k);
                 occlusion *= adsm;

@@ -25,12 +25,15 @@
 namespace vkhr {
 namespace {
 
+template <bool kF32>
 __global__ void __launch_bounds__(256)
 raymarch_kernel(RaymarchArgs a) {
+    __shared__ uint32_t occ_lds[kOccupancyLdsWords];
+    const OccupancyView occ = {occ_lds, stage_occupancy(a.vol, occ_lds)};
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
     const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
-    raymarch_pixel(a, px, py);
+    raymarch_pixel<kF32>(a, px, py, occ);
 }
 
 void matmul_cm(const float* x, const float* y, double* out) {  // x * y, column-major
@@ -79,7 +82,10 @@ double smoothstep(double e0, double e1, double x) {
 
 void launch_raymarch(const RaymarchArgs& a, hipStream_t stream) {
     dim3 grid((a.width + 15) / 16, (a.height + 15) / 16);
-    raymarch_kernel<<<grid, 256, 0, stream>>>(a);
+    if (a.vol.f32)
+        raymarch_kernel<true><<<grid, 256, 0, stream>>>(a);
+    else
+        raymarch_kernel<false><<<grid, 256, 0, stream>>>(a);
 }
 
 // Shared with the batched and adjoint paths: fills the per-launch arguments.
@@ -91,6 +97,8 @@ int make_raymarch_args(const vkhr_volume* vol, const vkhr_camera* cam, const vkh
                  width, height);
     VKHR_REQUIRE(vol->nx && vol->ny && vol->nz, VKHR_ERR_INVALID, "zero volume resolution");
     VKHR_REQUIRE(vol->cells || vol->density_f32, VKHR_ERR_INVALID, "volume needs cells or density_f32");
+    VKHR_REQUIRE(uint64_t(vol->nx + 1) * (vol->ny + 1) * (vol->nz + 1) < (uint64_t(1) << 32), VKHR_ERR_OVERFLOW,
+                 "volume %ux%ux%u too large", vol->nx, vol->ny, vol->nz);
     VKHR_REQUIRE(vol->tangents || rp->shading_model != 0, VKHR_ERR_INVALID, "Kajiya-Kay needs tangents");
     VKHR_REQUIRE(rp->raycast_steps >= 1.0f && rp->raycast_steps <= 65536.0f, VKHR_ERR_INVALID,
                  "raycast_steps %g out of range", rp->raycast_steps);
@@ -138,6 +146,10 @@ int make_raymarch_args(const vkhr_volume* vol, const vkhr_camera* cam, const vkh
     a->vol.cells = reinterpret_cast<const unsigned long long*>(vol->cells);
     a->vol.f32 = vol->density_f32;
     a->vol.nx = a->nx; a->vol.ny = a->ny; a->vol.nz = a->nz;
+    a->vol.occupancy = vol->occupancy;
+    a->vol.bx = bricks_along(vol->nx);
+    a->vol.by = bricks_along(vol->ny);
+    a->vol.occ_words = uint32_t(occupancy_words(vol->nx, vol->ny, vol->nz));
     a->tangents = reinterpret_cast<const char4*>(vol->tangents);
     return VKHR_OK;
 }

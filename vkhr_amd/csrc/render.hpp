@@ -10,21 +10,59 @@ namespace vkhr {
 constexpr uint32_t kPpllNull = 0xffffffffu;
 
 // The density volume as the kernels read it: corner-packed uint8 cells
-// (one 8-byte gather per trilinear tap) or a float volume.
+// (one 8-byte gather per trilinear tap) or a float volume, and the brick
+// occupancy of the cells (one bit per 8x8x8 cells, nullable).
 struct DensityVolume {
     const unsigned long long* cells;
     const float* f32;
+    const uint32_t* occupancy;
     int nx, ny, nz;
+    uint32_t bx, by;         // bricks along x and y
+    uint32_t occ_words;      // words of the occupancy bitmask
+};
+
+constexpr uint32_t kBrickShift = 3;              // 8x8x8 cells per brick
+constexpr uint32_t kOccupancyLdsWords = 2048;    // largest bitmask staged in LDS (8 KiB)
+
+__host__ __device__ __forceinline__ uint32_t bricks_along(uint32_t n) { return (n + 8u) >> kBrickShift; }
+
+__host__ __device__ __forceinline__ size_t occupancy_words(uint32_t nx, uint32_t ny, uint32_t nz) {
+    return (size_t(bricks_along(nx)) * bricks_along(ny) * bricks_along(nz) + 31) / 32;
+}
+
+// Copies the occupancy bitmask into LDS when it fits; returns whether it did
+// (otherwise no sample is skipped).  All threads of the block call it.
+__device__ __forceinline__ bool stage_occupancy(const DensityVolume& v, uint32_t* lds) {
+    const bool use = v.occupancy && v.cells && !v.f32 && v.occ_words <= kOccupancyLdsWords;
+    if (!use) return false;
+    for (uint32_t i = threadIdx.x; i < v.occ_words; i += blockDim.x) lds[i] = v.occupancy[i];
+    __syncthreads();
+    return true;
+}
+
+// The LDS copy of the occupancy a march reads (use = false: none).
+struct OccupancyView {
+    const uint32_t* lds;
+    bool use;
 };
 
 // texture(density, uvw).r at texel coordinate (x, y, z) = uvw * n - 0.5:
 // exact weights (the fraction of a float64 coordinate), zero border.
-__device__ __forceinline__ float density_at(const DensityVolume& v, double x, double y, double z) {
+// With the occupancy bitmask (staged in LDS), a sample whose cell lies in
+// an empty brick returns 0 without the gather: every texel of its footprint
+// is 0, so the result is the same.
+__device__ __forceinline__ float density_at(const DensityVolume& v, double x, double y, double z,
+                                            OccupancyView occ = {nullptr, false}) {
     if (!(x >= -1.0 && y >= -1.0 && z >= -1.0 && x < double(v.nx) && y < double(v.ny) && z < double(v.nz)))
         return 0.0f;
     double fx = floor(x), fy = floor(y), fz = floor(z);
     float wx = float(x - fx), wy = float(y - fy), wz = float(z - fz);
     int ix = int(fx), iy = int(fy), iz = int(fz);
+    if (occ.use) {
+        const uint32_t b = (uint32_t(ix + 1) >> kBrickShift) +
+                           v.bx * ((uint32_t(iy + 1) >> kBrickShift) + v.by * (uint32_t(iz + 1) >> kBrickShift));
+        if (!((occ.lds[b >> 5] >> (b & 31u)) & 1u)) return 0.0f;
+    }
     float b[8];
     if (v.f32) {
 #pragma unroll
@@ -50,6 +88,49 @@ __device__ __forceinline__ float density_at(const DensityVolume& v, double x, do
     float c1 = fmaf(wy, c11 - c01, c01);
     float r = fmaf(wz, c1 - c0, c0);
     return v.f32 ? r : r * (1.0f / 255.0f);
+}
+
+// density_at for the march loops, whose samples lie inside the support
+// interval of their ray (coordinates within a margin of [-1, n), so the
+// float-to-int conversions are exact): the range test x in [-1, n) is done
+// on the integer cell, floor(x) in [-1, n-1]; the cell index is 32-bit (the
+// callers bound the cell count below 2^32).  kF32 selects the float volume.
+template <bool kF32>
+__device__ __forceinline__ float march_tap(const DensityVolume& v, double x, double y, double z, OccupancyView occ) {
+    const double fx = floor(x), fy = floor(y), fz = floor(z);
+    const int ix = int(fx), iy = int(fy), iz = int(fz);
+    const uint32_t cx1 = uint32_t(ix + 1), cy1 = uint32_t(iy + 1), cz1 = uint32_t(iz + 1);
+    if (cx1 > uint32_t(v.nx) || cy1 > uint32_t(v.ny) || cz1 > uint32_t(v.nz)) return 0.0f;
+    if (occ.use) {
+        const uint32_t b = (cx1 >> kBrickShift) + v.bx * ((cy1 >> kBrickShift) + v.by * (cz1 >> kBrickShift));
+        if (!((occ.lds[b >> 5] >> (b & 31u)) & 1u)) return 0.0f;
+    }
+    const float wx = float(x - fx), wy = float(y - fy), wz = float(z - fz);
+    float b[8];
+    if constexpr (kF32) {
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            const int tx = ix + (n & 1), ty = iy + ((n >> 1) & 1), tz = iz + (n >> 2);
+            const bool in = tx >= 0 && ty >= 0 && tz >= 0 && tx < v.nx && ty < v.ny && tz < v.nz;
+            b[n] = in ? v.f32[uint32_t(tx) + uint32_t(v.nx) * (uint32_t(ty) + uint32_t(v.ny) * uint32_t(tz))] : 0.0f;
+        }
+    } else {
+        const uint32_t c = cx1 + (uint32_t(v.nx) + 1u) * (cy1 + (uint32_t(v.ny) + 1u) * cz1);
+        const unsigned long long q = v.cells[c];
+        const uint32_t lo = uint32_t(q), hi = uint32_t(q >> 32);
+        b[0] = float(lo & 0xffu); b[1] = float((lo >> 8) & 0xffu);
+        b[2] = float((lo >> 16) & 0xffu); b[3] = float(lo >> 24);
+        b[4] = float(hi & 0xffu); b[5] = float((hi >> 8) & 0xffu);
+        b[6] = float((hi >> 16) & 0xffu); b[7] = float(hi >> 24);
+    }
+    const float c00 = fmaf(wx, b[1] - b[0], b[0]);
+    const float c10 = fmaf(wx, b[3] - b[2], b[2]);
+    const float c01 = fmaf(wx, b[5] - b[4], b[4]);
+    const float c11 = fmaf(wx, b[7] - b[6], b[6]);
+    const float c0 = fmaf(wy, c10 - c00, c00);
+    const float c1 = fmaf(wy, c11 - c01, c01);
+    const float r = fmaf(wz, c1 - c0, c0);
+    return kF32 ? r : r * (1.0f / 255.0f);
 }
 
 // local_ambient_occlusion with kernel size 2 (8 taps at +-scale/2 texels)

@@ -3,10 +3,13 @@
 // src/vkhr/rasterizer/hair_style.cc:75-77), plus the corner-packed density
 // layout read by the raymarcher.
 //
-// Pass 1 (per voxel): saturating uint8 view, tangent quantization, and the
-//                     min/max of the uint8 view (one atomic pair per block).
-// Pass 2 (per voxel): HairStyle::Volume::normalize and the float view.
-// Pass 3 (per cell):  corner packing, 8 texels -> one uint64.
+// Pass 1 (per voxel): the saturating uint8 view, tangent quantization, and
+//                     the min/max of the uint8 view (one atomic pair per
+//                     block).
+// Pass 2 (per voxel): HairStyle::Volume::normalize in place and the float
+//                     view.
+// Pass 3 (8 consecutive cells of one row per thread): corner packing, 8
+//                     texels -> one uint64 per cell, and the brick occupancy.
 // Compiled with -ffp-contract=off (quantization and normalize are float
 // expressions whose rounding must match the reference's).
 #include "volume.hpp"
@@ -24,9 +27,11 @@ finalize_counts_kernel(const uint32_t* __restrict__ counts, const long long* __r
                        uint8_t* __restrict__ density, int8_t* __restrict__ tangents,
                        uint32_t* __restrict__ minmax) {
     uint32_t lo = 255u, hi = 0u;
-    size_t stride = size_t(gridDim.x) * blockDim.x;
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint32_t d = finalize_voxel(const_cast<uint32_t*>(counts), sums, i, density, tangents);
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    const size_t first = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (size_t q = first; q < n / 4; q += stride) finalize_quad(counts, sums, 4 * q, density, tangents, &lo, &hi);
+    for (size_t i = (n & ~size_t(3)) + first; i < n; i += stride) {  // the tail of n % 4
+        uint32_t d = finalize_voxel(counts, sums, i, density, tangents);
         lo = d < lo ? d : lo;
         hi = d > hi ? d : hi;
     }
@@ -50,48 +55,49 @@ finalize_counts_kernel(const uint32_t* __restrict__ counts, const long long* __r
     }
 }
 
+// One block per (brick row, cell level, 256 cell columns) (CLAMP_TO_BORDER
+// with an opaque-black border: texels outside the volume are 0), with the
+// brick occupancy.
 __global__ void __launch_bounds__(256)
-normalize_kernel(uint8_t* __restrict__ density, size_t n, const uint32_t* __restrict__ minmax, int normalize,
-                 float* __restrict__ density_f32) {
-    uint32_t lo = minmax ? minmax[0] : 0u, hi = minmax ? minmax[1] : 255u;
-    // a constant volume (division by zero, NaN products) becomes all zero
-    bool flat = (hi == lo);
-    float scaling = normalize_scaling(lo, hi);
-    size_t stride = size_t(gridDim.x) * blockDim.x;
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint32_t d = density[i];
-        if (normalize) {
-            d = normalize_value(d, lo, scaling, flat);
-            density[i] = uint8_t(d);
-        }
-        if (density_f32) density_f32[i] = float(d) / 255.0f;  // R8_UNORM
+cells_kernel(const uint8_t* __restrict__ density, uint32_t nx, uint32_t ny, uint32_t nz,
+             unsigned long long* __restrict__ cells, uint32_t* __restrict__ occupancy) {
+    __shared__ uint32_t flags[kCellBlockCols];
+    const uint32_t mcs = cell_col_blocks(nx), bys = bricks_along(ny);
+    const uint32_t n_blocks = mcs * bys * (nz + 1u);
+    for (uint32_t b = blockIdx.x; b < n_blocks; b += gridDim.x) {
+        const uint32_t mc = b % mcs, rest = b / mcs;
+        cells_block(density, nx, ny, nz, rest % bys, rest / bys, mc, cells, occupancy, flags);
+        __syncthreads();  // flags is reused by the next block
     }
 }
 
-// Cell (i,j,k) for i,j,k in [-1, n-1]: the 8 texels of the trilinear
-// footprint whose lower corner is (i,j,k), texels outside the volume = 0
-// (CLAMP_TO_BORDER with an opaque-black border).
+// Four voxels per thread (uchar4), the tail of n % 4 by the first threads.
 __global__ void __launch_bounds__(256)
-cells_kernel(const uint8_t* __restrict__ density, uint32_t nx, uint32_t ny, uint32_t nz,
-             unsigned long long* __restrict__ cells) {
-    uint32_t cx = nx + 1u, cy = ny + 1u, cz = nz + 1u;
-    size_t total = size_t(cx) * cy * cz;
-    size_t stride = size_t(gridDim.x) * blockDim.x;
-    for (size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x; c < total; c += stride) {
-        cells[c] = pack_cell(density, nx, ny, nz, c);
+normalize_kernel(uint8_t* __restrict__ density, size_t n, const uint32_t* __restrict__ minmax, int normalize,
+                 float* __restrict__ density_f32) {
+    const DensityMap m = density_map(minmax, normalize != 0);
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    const size_t first = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    uchar4* d4 = reinterpret_cast<uchar4*>(density);
+    for (size_t q = first; q < n / 4; q += stride) {
+        uchar4 v = d4[q];
+        v.x = uint8_t(texel_density(v.x, m));
+        v.y = uint8_t(texel_density(v.y, m));
+        v.z = uint8_t(texel_density(v.z, m));
+        v.w = uint8_t(texel_density(v.w, m));
+        if (normalize) d4[q] = v;
+        if (density_f32)  // R8_UNORM
+            reinterpret_cast<float4*>(density_f32)[q] =
+                make_float4(float(v.x) / 255.0f, float(v.y) / 255.0f, float(v.z) / 255.0f, float(v.w) / 255.0f);
+    }
+    for (size_t i = (n & ~size_t(3)) + first; i < n; i += stride) {
+        const uint32_t d = texel_density(density[i], m);
+        if (normalize) density[i] = uint8_t(d);
+        if (density_f32) density_f32[i] = float(d) / 255.0f;
     }
 }
 
 }  // namespace
-
-int build_cells(const uint8_t* density, uint32_t nx, uint32_t ny, uint32_t nz, uint64_t* cells,
-                hipStream_t s) {
-    size_t total = size_t(nx + 1) * (ny + 1) * (nz + 1);
-    cells_kernel<<<grid_for(total, 256, 65536), 256, 0, s>>>(density, nx, ny, nz,
-                                                             reinterpret_cast<unsigned long long*>(cells));
-    VKHR_CHECK_LAUNCH();
-    return VKHR_OK;
-}
 
 }  // namespace vkhr
 
@@ -99,15 +105,23 @@ using namespace vkhr;
 
 extern "C" int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tangent_acc, uint32_t nx, uint32_t ny,
                                     uint32_t nz, int normalize, uint8_t* density_u8, int8_t* tangent_i8x4,
-                                    float* density_f32, uint64_t* cells, vkhr_stream stream) {
+                                    float* density_f32, uint64_t* cells, uint32_t* occupancy,
+                                    vkhr_stream stream) {
     clear_error();
     int rc = ensure_device();
     if (rc) return rc;
-    VKHR_REQUIRE(counts && density_u8, VKHR_ERR_INVALID, "counts/density_u8 is null");
+    VKHR_REQUIRE(density_u8, VKHR_ERR_INVALID, "density_u8 is null");
+    VKHR_REQUIRE(reinterpret_cast<uintptr_t>(density_u8) % 4 == 0 && reinterpret_cast<uintptr_t>(density_f32) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(counts) % 16 == 0 && reinterpret_cast<uintptr_t>(tangent_i8x4) % 16 == 0,
+                 VKHR_ERR_INVALID, "density_u8 / counts, density_f32, tangent_i8x4 must be 4 / 16-byte aligned");
+    VKHR_REQUIRE(counts || tangent_acc, VKHR_ERR_INVALID, "neither counts nor tangent_acc");
+    VKHR_REQUIRE(!occupancy || cells, VKHR_ERR_INVALID, "occupancy needs cells");
     VKHR_REQUIRE(nx && ny && nz, VKHR_ERR_INVALID, "zero volume resolution");
     VKHR_REQUIRE(!tangent_i8x4 || tangent_acc, VKHR_ERR_INVALID, "tangent_i8x4 needs tangent_acc");
     size_t n = size_t(nx) * ny * nz;
     VKHR_REQUIRE(n <= (size_t(1) << 31), VKHR_ERR_OVERFLOW, "volume too large");
+    VKHR_REQUIRE(!cells || size_t(nx + 1) * (ny + 1) * (nz + 1) < (size_t(1) << 32), VKHR_ERR_OVERFLOW,
+                 "too many cells");
     hipStream_t s = as_stream(stream);
     uint32_t* mm = nullptr;
     if (normalize) {
@@ -118,14 +132,21 @@ extern "C" int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tange
         VKHR_CHECK_LAUNCH();
     }
     // A capped grid keeps the min/max atomics at one pair per block.
-    unsigned grid = grid_for(n, 256, 2048);
-    finalize_counts_kernel<<<grid, 256, 0, s>>>(counts, reinterpret_cast<const long long*>(tangent_acc), n, density_u8,
-                                                tangent_i8x4, mm);
+    finalize_counts_kernel<<<grid_for(n, 256, 2048), 256, 0, s>>>(
+        counts, reinterpret_cast<const long long*>(tangent_acc), n, density_u8, tangent_i8x4, mm);
     VKHR_CHECK_LAUNCH();
     if (normalize || density_f32) {
-        normalize_kernel<<<grid, 256, 0, s>>>(density_u8, n, mm, normalize, density_f32);
+        normalize_kernel<<<grid_for(n / 4 + 1, 256, 65536), 256, 0, s>>>(density_u8, n, mm, normalize, density_f32);
         VKHR_CHECK_LAUNCH();
     }
-    if (cells && (rc = build_cells(density_u8, nx, ny, nz, cells, s))) return rc;
+    if (cells) {
+        if (occupancy && (rc = fill_u32(occupancy, occupancy_words(nx, ny, nz), 0u, s))) return rc;
+        const uint32_t blocks = cell_col_blocks(nx) * bricks_along(ny) * (nz + 1);
+        cells_kernel<<<grid_for(blocks, 1, 65536), 256, 0, s>>>(
+            density_u8, nx, ny, nz, reinterpret_cast<unsigned long long*>(cells), occupancy);
+        VKHR_CHECK_LAUNCH();
+    }
     return VKHR_OK;
 }
+
+extern "C" size_t vkhr_occupancy_words(uint32_t nx, uint32_t ny, uint32_t nz) { return occupancy_words(nx, ny, nz); }

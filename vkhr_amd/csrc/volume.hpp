@@ -3,51 +3,89 @@
 // units including this header are compiled with -ffp-contract=off.
 #pragma once
 
-#include "common.hpp"
+#include "render.hpp"
 
 namespace vkhr {
 
-// Saturating uint8 view of voxel i and its quantized tangent; returns the
-// uint8 value.  Tangents: sum / float(count) * 127 truncated toward zero --
-// the reference's expression on sums of every contribution (the reference
-// keeps only the first 255 in its serial order, so saturated voxels may
-// differ).  The sums come from the fixed-point accumulator (voxel.hpp);
-// beyond 32767 contributions they are not decodable and the tangent is 0.
-// With counts_from_acc the count is taken from the accumulator and stored
-// to counts[i] (the batched path unpacks the counts here).
-__device__ __forceinline__ uint32_t finalize_voxel(uint32_t* __restrict__ counts, const long long* __restrict__ acc,
-                                                   size_t i, uint8_t* __restrict__ density,
-                                                   int8_t* __restrict__ tangents, bool counts_from_acc = false) {
-    uint32_t c;
-    if (counts_from_acc) {
-        c = uint32_t((unsigned long long)(acc[2 * i + 1]) & 0xffffffffull);
-        counts[i] = c;
-    } else {
-        c = counts[i];
-    }
-    uint32_t d = c < 255u ? c : 255u;
+// The count of voxel i: from counts, or, without them, from word 1 of the
+// packed accumulator (voxel.hpp).
+__device__ __forceinline__ uint32_t voxel_count(const uint32_t* __restrict__ counts,
+                                                const long long* __restrict__ acc, size_t i) {
+    if (counts) return counts[i];
+    return uint32_t((unsigned long long)(acc[2 * i + 1]) & 0xffffffffull);
+}
+
+// Quantized tangent of voxel i: sum / float(count) * 127 truncated toward
+// zero -- the reference's expression on sums of every contribution (the
+// reference keeps only the first 255 in its serial order, so saturated
+// voxels may differ).  The sums come from the fixed-point accumulator
+// (voxel.hpp); beyond 32767 contributions they are not decodable and the
+// tangent is 0, as for empty voxels.
+__device__ __forceinline__ char4 quantize_tangent(const long long* __restrict__ acc, size_t i, uint32_t c) {
+    if (!(c && c < 32768u)) return make_char4(0, 0, 0, 0);
+    const long long w0 = acc[2 * i + 0], w1 = acc[2 * i + 1];
+    const long long sy = (long long)(int)(unsigned)(w0 & 0xffffffffll);
+    const long long sx = (w0 - sy) / (1ll << 32);
+    const long long sz = w1 >> 32;  // w1 = z * 2^32 + count, 0 <= count < 2^32
+    const float inv = 1.0f / 65536.0f;
+    const float denom = float(c);
+    const float x = float(sx) * inv / denom * 127.0f;
+    const float y = float(sy) * inv / denom * 127.0f;
+    const float z = float(sz) * inv / denom * 127.0f;
+    auto cvt = [](float v) -> signed char {
+        v = fminf(fmaxf(v, -127.0f), 127.0f);
+        return static_cast<signed char>(static_cast<int>(v));
+    };
+    return make_char4(cvt(x), cvt(y), cvt(z), 0);
+}
+
+// First pass over voxel i: the saturating uint8 view min(count, 255)
+// (stored and returned, for the min/max of the normalize) and the quantized
+// tangent.
+__device__ __forceinline__ uint32_t finalize_voxel(const uint32_t* __restrict__ counts,
+                                                   const long long* __restrict__ acc, size_t i,
+                                                   uint8_t* __restrict__ density, int8_t* __restrict__ tangents) {
+    const uint32_t c = voxel_count(counts, acc, i);
+    const uint32_t d = c < 255u ? c : 255u;
     density[i] = uint8_t(d);
-    if (tangents) {
-        char4 q = make_char4(0, 0, 0, 0);
-        if (c && c < 32768u) {
-            long long w0 = acc[2 * i + 0], w1 = acc[2 * i + 1];
-            long long sy = (long long)(int)(unsigned)(w0 & 0xffffffffll);
-            long long sx = (w0 - sy) / (1ll << 32);
-            const float inv = 1.0f / 65536.0f;
-            float denom = float(c);
-            float x = float(sx) * inv / denom * 127.0f;
-            float y = float(sy) * inv / denom * 127.0f;
-            long long sz = w1 >> 32;  // w1 = z * 2^32 + count, 0 <= count < 2^32
-            float z = float(sz) * inv / denom * 127.0f;
-            auto cvt = [](float v) -> signed char {
-                v = fminf(fmaxf(v, -127.0f), 127.0f);
-                return static_cast<signed char>(static_cast<int>(v));
-            };
-            q = make_char4(cvt(x), cvt(y), cvt(z), 0);
-        }
-        reinterpret_cast<char4*>(tangents)[i] = q;
-    }
+    if (tangents) reinterpret_cast<char4*>(tangents)[i] = quantize_tangent(acc, i, c);
     return d;
+}
+
+// finalize_voxel of voxels i .. i+3 (i a multiple of 4, all < n) with
+// vector loads and stores; folds their uint8 views into lo / hi.
+__device__ __forceinline__ void finalize_quad(const uint32_t* __restrict__ counts, const long long* __restrict__ acc,
+                                              size_t i, uint8_t* __restrict__ density,
+                                              int8_t* __restrict__ tangents, uint32_t* lo, uint32_t* hi) {
+    uint32_t c[4];
+    if (counts) {
+        const uint4 v = reinterpret_cast<const uint4*>(counts)[i / 4];
+        c[0] = v.x; c[1] = v.y; c[2] = v.z; c[3] = v.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c[k] = uint32_t((unsigned long long)(acc[2 * (i + k) + 1]) & 0xffffffffull);
+    }
+    uchar4 d;
+    d.x = uint8_t(c[0] < 255u ? c[0] : 255u);
+    d.y = uint8_t(c[1] < 255u ? c[1] : 255u);
+    d.z = uint8_t(c[2] < 255u ? c[2] : 255u);
+    d.w = uint8_t(c[3] < 255u ? c[3] : 255u);
+    reinterpret_cast<uchar4*>(density)[i / 4] = d;
+    if (tangents) {
+        char4 q[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) q[k] = quantize_tangent(acc, i + k, c[k]);
+        int4 packed;
+        packed.x = *reinterpret_cast<int*>(&q[0]);
+        packed.y = *reinterpret_cast<int*>(&q[1]);
+        packed.z = *reinterpret_cast<int*>(&q[2]);
+        packed.w = *reinterpret_cast<int*>(&q[3]);
+        reinterpret_cast<int4*>(tangents)[i / 4] = packed;
+    }
+    const uint32_t mn = min(min(uint32_t(d.x), uint32_t(d.y)), min(uint32_t(d.z), uint32_t(d.w)));
+    const uint32_t mx = max(max(uint32_t(d.x), uint32_t(d.y)), max(uint32_t(d.z), uint32_t(d.w)));
+    *lo = mn < *lo ? mn : *lo;
+    *hi = mx > *hi ? mx : *hi;
 }
 
 // HairStyle::Volume::normalize of one value: (d - min) * (255 / (max - min)),
@@ -62,24 +100,99 @@ __host__ __device__ __forceinline__ float normalize_scaling(uint32_t lo, uint32_
     return hi == lo ? 0.0f : 255.0f / float(int(hi) - int(lo));  // hair_style.cc:351
 }
 
-// Cell c of the corner-packed layout: the 8 texels of the trilinear footprint
-// whose lower corner is (i,j,k) in [-1, n-1]^3, zero outside the volume.
-__device__ __forceinline__ unsigned long long pack_cell(const uint8_t* __restrict__ density, uint32_t nx,
-                                                        uint32_t ny, uint32_t nz, size_t c) {
-    uint32_t cx = nx + 1u, cy = ny + 1u;
-    int i = int(c % cx) - 1;
-    int j = int((c / cx) % cy) - 1;
-    int k = int(c / (size_t(cx) * cy)) - 1;
-    unsigned long long packed = 0ull;
+// The uint8 view -> the density the renderers sample: the normalize when
+// it is on.
+struct DensityMap {
+    uint32_t lo;
+    float scaling;
+    bool flat;
+    bool normalize;
+};
+
+__device__ __forceinline__ DensityMap density_map(const uint32_t* minmax, bool normalize) {
+    DensityMap m;
+    m.lo = normalize ? minmax[0] : 0u;
+    const uint32_t hi = normalize ? minmax[1] : 255u;
+    m.flat = hi == m.lo;
+    m.scaling = normalize_scaling(m.lo, hi);
+    m.normalize = normalize;
+    return m;
+}
+
+__device__ __forceinline__ uint32_t texel_density(uint32_t d, const DensityMap& m) {
+    return m.normalize ? normalize_value(d, m.lo, m.scaling, m.flat) : d;
+}
+
+// One block of the corner-packed layout -- cells (i, j, k), i in [-1, nx-1],
+// index (i+1) + (nx+1) * ((j+1) + (ny+1) * (k+1)), each the 8 texels of the
+// trilinear footprint whose lower corner is (i,j,k) as one uint64 (byte b =
+// texel (i+(b&1), j+(b>>1&1), k+(b>>2)), 0 outside the volume) -- from the
+// final uint8 density: 8 cell rows j+1 in [8 by, 8 by + 8) at cell level
+// k+1 = ck, cell columns i+1 in [256 mc, 256 mc + 256), 256 threads, each 8
+// consecutive cells of one row (one brick wide).  Each thread reads the 9
+// texels of its 4 texel rows once and stores 64 contiguous bytes.  The
+// bricks (8x8x8 cells) holding a nonzero cell are or-ed into occupancy
+// (nullable; two word atomics per block).  All threads of the block call it.
+constexpr uint32_t kCellBlockCols = 32;   // threads along x: 32 x 8 = 256 cells
+
+__host__ __device__ __forceinline__ uint32_t cell_col_blocks(uint32_t nx) {
+    return (nx + 1u + 8u * kCellBlockCols - 1u) / (8u * kCellBlockCols);
+}
+
+__device__ __forceinline__ void cells_block(const uint8_t* __restrict__ density, uint32_t nx, uint32_t ny, uint32_t nz,
+                                            uint32_t by, uint32_t ck, uint32_t mc,
+                                            unsigned long long* __restrict__ cells, uint32_t* occupancy,
+                                            uint32_t* lds_flags) {
+    const uint32_t r = threadIdx.x / kCellBlockCols, mloc = threadIdx.x % kCellBlockCols;
+    const uint32_t cj = (by << kBrickShift) + r;          // cell row j + 1
+    const uint32_t x0 = (mc * kCellBlockCols + mloc) * 8u; // first cell column i + 1
+    if (threadIdx.x < kCellBlockCols) lds_flags[threadIdx.x] = 0u;
+    __syncthreads();
+    if (cj <= ny && x0 <= nx) {
+        uint32_t t[4][9];  // texel rows (j + (q&1), k + (q>>1)), texels x0 - 1 .. x0 + 7
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-        int x = i + (b & 1), y = j + ((b >> 1) & 1), z = k + (b >> 2);
-        if (x >= 0 && y >= 0 && z >= 0 && x < int(nx) && y < int(ny) && z < int(nz)) {
-            unsigned long long t = density[size_t(x) + size_t(nx) * (size_t(y) + size_t(ny) * size_t(z))];
-            packed |= t << (8 * b);
+        for (int q = 0; q < 4; ++q) {
+            const int y = int(cj) - 1 + (q & 1), z = int(ck) - 1 + (q >> 1);
+            const bool row_in = y >= 0 && z >= 0 && y < int(ny) && z < int(nz);
+            const uint8_t* src = density + (row_in ? uint32_t(y) * nx + uint32_t(z) * nx * ny : 0u);
+#pragma unroll
+            for (int e = 0; e < 9; ++e) {
+                const int x = int(x0) - 1 + e;
+                t[q][e] = (row_in && x >= 0 && x < int(nx)) ? uint32_t(src[x]) : 0u;
+            }
+        }
+        unsigned long long* out = cells + size_t(cj + (ny + 1u) * ck) * (nx + 1u) + x0;
+        const uint32_t n_cells = min(8u, nx + 1u - x0);
+        unsigned long long any = 0ull;
+#pragma unroll
+        for (uint32_t c = 0; c < 8; ++c) {
+            if (c < n_cells) {
+                unsigned long long packed = 0ull;
+#pragma unroll
+                for (int b = 0; b < 8; ++b)
+                    packed |= (unsigned long long)(t[b >> 1][c + (b & 1)]) << (8 * b);
+                out[c] = packed;
+                any |= packed;
+            }
+        }
+        if (occupancy && any) lds_flags[mloc] = 1u;
+    }
+    if (!occupancy) return;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // bricks bx = mc * 32 + m (m < 32) at bits first + bx: at most two words
+        const uint32_t bxs = bricks_along(nx), bys = bricks_along(ny);
+        const uint32_t first = bxs * (by + bys * (ck >> kBrickShift)) + mc * kCellBlockCols;
+        unsigned long long mask = 0ull;
+        for (uint32_t m = 0; m < kCellBlockCols && mc * kCellBlockCols + m < bxs; ++m)
+            if (lds_flags[m]) mask |= 1ull << m;
+        if (mask) {
+            const uint32_t w = first >> 5, sh = first & 31u;
+            const unsigned long long bits = mask << sh;  // sh + 32 <= 63
+            if (uint32_t(bits)) atomicOr(&occupancy[w], uint32_t(bits));
+            if (uint32_t(bits >> 32)) atomicOr(&occupancy[w + 1], uint32_t(bits >> 32));
         }
     }
-    return packed;
 }
 
 }  // namespace vkhr

@@ -49,18 +49,31 @@ __device__ __forceinline__ long long tangent_fixed(float v) {
     return (long long)rintf(v * kTangentScale);  // NaN components become 0 via the clamp
 }
 
-// Adds one sample to voxel v: the count alone, or count and tangent packed.
-__device__ __forceinline__ void add_sample(uint32_t* __restrict__ counts, long long* __restrict__ acc, uint32_t v,
-                                           float3 t) {
-    if (!acc) {
-        atomicAdd(&counts[v], 1u);
-        return;
-    }
+// The two packed words of a sample with tangent t (count 1).
+__device__ __forceinline__ void packed_words(float3 t, unsigned long long* w0, unsigned long long* w1) {
     long long x = tangent_fixed(t.x), y = tangent_fixed(t.y), z = tangent_fixed(t.z);
-    unsigned long long w0 = (unsigned long long)(x) * (1ull << 32) + (unsigned long long)(y);
-    unsigned long long w1 = (unsigned long long)(z) * (1ull << 32) + 1ull;
-    atomicAdd(reinterpret_cast<unsigned long long*>(&acc[2ull * v + 0ull]), w0);
-    atomicAdd(reinterpret_cast<unsigned long long*>(&acc[2ull * v + 1ull]), w1);
+    *w0 = (unsigned long long)(x) * (1ull << 32) + (unsigned long long)(y);
+    *w1 = (unsigned long long)(z) * (1ull << 32) + 1ull;
+}
+
+// Adds every lane's sample to its packed accumulator entry (target =
+// &acc[2v], or null for no sample), two lanes per sample: in round r lane L
+// adds word (L & 1) of the sample of lane (L >> 1) + 32r, so that the 16
+// contiguous bytes of one entry travel in one line request.  Every lane of
+// the wave must call it (shuffles).
+__device__ __forceinline__ void add_packed_wave(long long* target, unsigned long long w0, unsigned long long w1) {
+    const unsigned long long live = __ballot(target != nullptr);
+    if (!live) return;
+    const int lane = __lane_id();
+    const unsigned long long addr = reinterpret_cast<unsigned long long>(target);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        if (!((live >> (32 * r)) & 0xffffffffull)) continue;
+        const int src = (lane >> 1) + 32 * r;
+        const unsigned long long a = __shfl(addr, src, kWave);
+        const unsigned long long s0 = __shfl(w0, src, kWave), s1 = __shfl(w1, src, kWave);
+        if (a) atomicAdd(reinterpret_cast<unsigned long long*>(a) + (lane & 1), (lane & 1) ? s1 : s0);
+    }
 }
 
 // The count of voxel v from word 1 of the packed accumulator.
@@ -68,50 +81,70 @@ __device__ __forceinline__ uint32_t packed_count(const long long* __restrict__ a
     return uint32_t((unsigned long long)(acc[2 * v + 1]) & 0xffffffffull);
 }
 
-// One vertex (hair_style.cc:272-281): count + optional tangent sum.
-__device__ __forceinline__ void voxelize_vertex(const VoxelGrid& g, const float* __restrict__ xyz, uint32_t i,
-                                                const float* __restrict__ tangents, uint32_t* __restrict__ counts,
-                                                long long* __restrict__ sums) {
-    float3 p = load_xyz(xyz, i);
-    // (vertex - origin) / voxel_size, per component (hair_style.cc:274)
-    float px = (p.x - g.ox) / g.vx;
-    float py = (p.y - g.oy) / g.vy;
-    float pz = (p.z - g.oz) / g.vz;
-    uint32_t v = voxel_of(g, px, py, pz);
-    add_sample(counts, sums, v, sums ? load_xyz(tangents, i) : make_float3(0.0f, 0.0f, 0.0f));
+// One vertex per lane (hair_style.cc:272-281): count, or count and tangent
+// packed.  Wave-cooperative: every lane calls it, has = false for none.
+__device__ __forceinline__ void voxelize_vertex(const VoxelGrid& g, const float* __restrict__ xyz, bool has,
+                                                uint32_t i, const float* __restrict__ tangents,
+                                                uint32_t* __restrict__ counts, long long* __restrict__ sums) {
+    uint32_t v = 0;
+    if (has) {
+        float3 p = load_xyz(xyz, i);
+        // (vertex - origin) / voxel_size, per component (hair_style.cc:274)
+        float px = (p.x - g.ox) / g.vx;
+        float py = (p.y - g.oy) / g.vy;
+        float pz = (p.z - g.oz) / g.vz;
+        v = voxel_of(g, px, py, pz);
+        if (!sums) atomicAdd(&counts[v], 1u);
+    }
+    unsigned long long w0 = 0, w1 = 0;
+    if (has && sums) packed_words(load_xyz(tangents, i), &w0, &w1);
+    add_packed_wave(has && sums ? &sums[2ull * v] : nullptr, w0, w1);
 }
 
-// One index pair (hair_style.cc:311-329): the sequential float DDA.
+// One index pair per lane (hair_style.cc:311-329): the sequential float
+// DDA.  Wave-cooperative (the lanes step together until the longest
+// segment of the wave is done): every lane calls it, has = false for none.
 __device__ __forceinline__ void voxelize_segment(const VoxelGrid& g, const float* __restrict__ xyz,
-                                                 uint32_t n_vertices, uint32_t ia, uint32_t ib,
+                                                 uint32_t n_vertices, bool has, uint32_t ia, uint32_t ib,
                                                  const float* __restrict__ tangents, uint32_t* __restrict__ counts,
                                                  long long* __restrict__ sums) {
-    uint2 ab = make_uint2(ia, ib);
-    if (ab.x >= n_vertices || ab.y >= n_vertices) return;  // malformed pair: skipped
-    float3 a = load_xyz(xyz, ab.x);
-    float3 b = load_xyz(xyz, ab.y);
-    // root / tip in voxel units (hair_style.cc:312-313)
-    float rx = (a.x - g.ox) / g.vx, ry = (a.y - g.oy) / g.vy, rz = (a.z - g.oz) / g.vz;
-    float tx = (b.x - g.ox) / g.vx, ty = (b.y - g.oy) / g.vy, tz = (b.z - g.oz) / g.vz;
-    float dx = tx - rx, dy = ty - ry, dz = tz - rz;
-    // steps = compMax(abs(direction)); direction /= steps (:315-317)
-    float steps = fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fabsf(dz));
-    // A non-finite or absurd length (vertices far outside the given box)
-    // would never finish; such segments are skipped.
-    if (!(steps <= kMaxSteps)) return;
-    dx /= steps;
-    dy /= steps;
-    dz /= steps;
-    float3 t = make_float3(0.0f, 0.0f, 0.0f);
-    if (sums) t = load_xyz(tangents, ab.x);
+    float rx = 0.0f, ry = 0.0f, rz = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f, steps = 0.0f;
+    unsigned long long w0 = 0, w1 = 0;
+    if (has && ia < n_vertices && ib < n_vertices) {  // a malformed pair is skipped
+        float3 a = load_xyz(xyz, ia);
+        float3 b = load_xyz(xyz, ib);
+        // root / tip in voxel units (hair_style.cc:312-313)
+        rx = (a.x - g.ox) / g.vx; ry = (a.y - g.oy) / g.vy; rz = (a.z - g.oz) / g.vz;
+        float tx = (b.x - g.ox) / g.vx, ty = (b.y - g.oy) / g.vy, tz = (b.z - g.oz) / g.vz;
+        dx = tx - rx; dy = ty - ry; dz = tz - rz;
+        // steps = compMax(abs(direction)); direction /= steps (:315-317)
+        steps = fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fabsf(dz));
+        // A non-finite or absurd length (vertices far outside the given box)
+        // would never finish; such segments are skipped.
+        if (steps <= kMaxSteps) {
+            dx /= steps;
+            dy /= steps;
+            dz /= steps;
+            if (sums) packed_words(load_xyz(tangents, ia), &w0, &w1);
+        } else {
+            steps = 0.0f;
+        }
+    }
     // while (steps-- > 0.0f): ceil(steps) samples, root accumulated (:319-328)
-    while (steps > 0.0f) {
-        steps -= 1.0f;
-        uint32_t v = voxel_of(g, rx, ry, rz);
-        add_sample(counts, sums, v, t);
-        rx += dx;
-        ry += dy;
-        rz += dz;
+    while (__ballot(steps > 0.0f)) {
+        const bool on = steps > 0.0f;
+        uint32_t v = 0;
+        if (on) {
+            steps -= 1.0f;
+            v = voxel_of(g, rx, ry, rz);
+            if (!sums) atomicAdd(&counts[v], 1u);
+        }
+        add_packed_wave(on && sums ? &sums[2ull * v] : nullptr, w0, w1);
+        if (on) {
+            rx += dx;
+            ry += dy;
+            rz += dz;
+        }
     }
 }
 

@@ -16,13 +16,17 @@
 namespace vkhr {
 namespace {
 
+// Grid-stride over wave-aligned ranges so that whole waves stay converged
+// for the wave-cooperative bodies.
 __global__ void __launch_bounds__(256)
 voxelize_vertices_kernel(const float* __restrict__ xyz, uint32_t n_vertices,
                          const float* __restrict__ tangents, VoxelGrid g,
                          uint32_t* __restrict__ counts, long long* __restrict__ sums) {
-    uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_vertices; i += stride) {
-        voxelize_vertex(g, xyz, i, tangents, counts, sums);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~uint32_t(kWave - 1)); base < n_vertices;
+         base += stride) {
+        const uint32_t i = base + (threadIdx.x & uint32_t(kWave - 1));
+        voxelize_vertex(g, xyz, i < n_vertices, i, tangents, counts, sums);
     }
 }
 
@@ -30,9 +34,13 @@ __global__ void __launch_bounds__(256)
 voxelize_segments_kernel(const float* __restrict__ xyz, uint32_t n_vertices,
                          const uint32_t* __restrict__ indices, uint32_t n_pairs, const float* __restrict__ tangents, VoxelGrid g,
                          uint32_t* __restrict__ counts, long long* __restrict__ sums) {
-    uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n_pairs; s += stride) {
-        voxelize_segment(g, xyz, n_vertices, indices[2u * s], indices[2u * s + 1u], tangents, counts, sums);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~uint32_t(kWave - 1)); base < n_pairs;
+         base += stride) {
+        const uint32_t s = base + (threadIdx.x & uint32_t(kWave - 1));
+        const bool has = s < n_pairs;
+        voxelize_segment(g, xyz, n_vertices, has, has ? indices[2u * s] : 0u, has ? indices[2u * s + 1u] : 0u,
+                         tangents, counts, sums);
     }
 }
 
@@ -44,13 +52,14 @@ unpack_counts_kernel(const long long* __restrict__ acc, uint32_t* __restrict__ c
 }
 
 // Zeroes what the voxelizer accumulates into: the packed accumulator when
-// there is one (the counts are unpacked from it afterwards), else the counts.
+// there is one (the counts are unpacked from it afterwards if asked for),
+// else the counts.
 int clear_targets(uint32_t* counts, int64_t* acc, size_t n, hipStream_t s) {
     return acc ? fill_u32(reinterpret_cast<uint32_t*>(acc), 4 * n, 0u, s) : fill_u32(counts, n, 0u, s);
 }
 
 int unpack_counts(uint32_t* counts, const int64_t* acc, size_t n, hipStream_t s) {
-    if (!acc) return VKHR_OK;
+    if (!acc || !counts) return VKHR_OK;
     unpack_counts_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(reinterpret_cast<const long long*>(acc), counts, n);
     VKHR_CHECK_LAUNCH();
     return VKHR_OK;
@@ -69,7 +78,7 @@ extern "C" int vkhr_voxelize_vertices(const float* xyz, uint32_t n_vertices, con
     if (rc) return rc;
     VoxelGrid g;
     if ((rc = make_grid(bounds, nx, ny, nz, &g))) return rc;
-    VKHR_REQUIRE(counts, VKHR_ERR_INVALID, "counts is null");
+    VKHR_REQUIRE(counts || tangent_acc, VKHR_ERR_INVALID, "neither counts nor tangent_acc");
     VKHR_REQUIRE(n_vertices == 0 || xyz, VKHR_ERR_INVALID, "xyz is null");
     VKHR_REQUIRE(!tangent_acc || tangents, VKHR_ERR_INVALID, "tangent_acc needs tangents");
     hipStream_t s = as_stream(stream);
@@ -92,7 +101,7 @@ extern "C" int vkhr_voxelize_segments(const float* xyz, uint32_t n_vertices, con
     if (rc) return rc;
     VoxelGrid g;
     if ((rc = make_grid(bounds, nx, ny, nz, &g))) return rc;
-    VKHR_REQUIRE(counts, VKHR_ERR_INVALID, "counts is null");
+    VKHR_REQUIRE(counts || tangent_acc, VKHR_ERR_INVALID, "neither counts nor tangent_acc");
     uint32_t n_pairs = n_indices / 2;  // the reference loops while i < size-1
     VKHR_REQUIRE(n_pairs == 0 || (xyz && indices && n_vertices), VKHR_ERR_INVALID, "xyz/indices is null");
     VKHR_REQUIRE(!tangent_acc || tangents, VKHR_ERR_INVALID, "tangent_acc needs tangents");

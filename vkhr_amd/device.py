@@ -43,12 +43,16 @@ def generate_tangents(xyz: torch.Tensor, offsets: torch.Tensor) -> torch.Tensor:
 
 
 def voxelize(xyz: torch.Tensor, bounds: dict, res, indices: torch.Tensor | None = None,
-             tangents: torch.Tensor | None = None, want_sums: bool = False, counts=None, sums=None):
-    """Vertex mode when indices is None, else segment mode.  Returns (counts, sums)."""
+             tangents: torch.Tensor | None = None, want_sums: bool = False, counts=None, sums=None,
+             want_counts: bool = True):
+    """Vertex mode when indices is None, else segment mode.  Returns (counts, sums);
+    with want_sums and not want_counts the counts stay packed in sums (counts is None)."""
     nx, ny, nz = res
     n = nx * ny * nz
     dev = xyz.device
-    if counts is None:
+    if not want_sums:
+        want_counts = True
+    if counts is None and want_counts:
         counts = torch.empty(n, dtype=torch.int32, device=dev)
     if want_sums and sums is None:
         sums = torch.empty(2 * n, dtype=torch.int64, device=dev)   # fixed-point tangent accumulator
@@ -62,18 +66,26 @@ def voxelize(xyz: torch.Tensor, bounds: dict, res, indices: torch.Tensor | None 
     return counts, (sums if want_sums else None)
 
 
-def finalize(counts: torch.Tensor, res, sums: torch.Tensor | None = None, normalize: bool = True,
-             want_f32: bool = False, want_cells: bool = True):
+def occupancy_words(res) -> int:
+    return int(capi.lib.vkhr_occupancy_words(*res))
+
+
+def finalize(counts: torch.Tensor | None, res, sums: torch.Tensor | None = None, normalize: bool = True,
+             want_f32: bool = False, want_cells: bool = True, want_occupancy: bool = True):
+    """counts may be None when sums holds them packed.  The brick occupancy
+    comes with the cells unless want_occupancy is False."""
     nx, ny, nz = res
     n = nx * ny * nz
-    dev = counts.device
+    dev = (counts if counts is not None else sums).device
     dens = torch.empty(n, dtype=torch.uint8, device=dev)
     tan = torch.empty(4 * n, dtype=torch.int8, device=dev) if sums is not None else None
     f32 = torch.empty(n, dtype=torch.float32, device=dev) if want_f32 else None
     cells = torch.empty((nx + 1) * (ny + 1) * (nz + 1), dtype=torch.int64, device=dev) if want_cells else None
+    occ = (torch.empty(occupancy_words(res), dtype=torch.int32, device=dev)
+           if want_cells and want_occupancy else None)
     call("vkhr_volume_finalize", ptr(counts), ptr(sums), nx, ny, nz, int(normalize), ptr(dens), ptr(tan),
-         ptr(f32), ptr(cells), stream_of())
-    return dict(density=dens, tangents=tan, density_f32=f32, cells=cells)
+         ptr(f32), ptr(cells), ptr(occ), stream_of())
+    return dict(density=dens, tangents=tan, density_f32=f32, cells=cells, occupancy=occ)
 
 
 def volume_struct(vol: dict, res) -> capi.Volume:
@@ -82,6 +94,7 @@ def volume_struct(vol: dict, res) -> capi.Volume:
     v.cells = ptr(vol.get("cells"))
     v.tangents = ptr(vol.get("tangents"))
     v.density_f32 = ptr(vol.get("density_f32"))
+    v.occupancy = ptr(vol.get("occupancy"))
     v.nx, v.ny, v.nz = res
     return v
 

@@ -148,6 +148,7 @@ HairStyle::Volume make_volume(const AABB& box, std::size_t w, std::size_t h, std
     v.density = DeviceBuffer(n);
     v.tangent = DeviceBuffer(4 * n);
     v.cells = DeviceBuffer((w + 1) * (h + 1) * (d + 1) * sizeof(uint64_t));
+    v.occupancy = DeviceBuffer(vkhr_occupancy_words(uint32_t(w), uint32_t(h), uint32_t(d)) * sizeof(uint32_t));
     return v;
 }
 
@@ -163,7 +164,8 @@ vkhr_aabb to_c(const AABB& b) {
 void finalize(HairStyle::Volume& v, int normalize, vkhr_stream stream) {
     check(vkhr_volume_finalize(v.counts.as<uint32_t>(), v.tangent_sums.as<int64_t>(), uint32_t(v.resolution.x),
                                uint32_t(v.resolution.y), uint32_t(v.resolution.z), normalize, v.density.as<uint8_t>(),
-                               v.tangent.as<int8_t>(), nullptr, v.cells.as<uint64_t>(), stream),
+                               v.tangent.as<int8_t>(), nullptr, v.cells.as<uint64_t>(),
+                               v.occupancy.as<uint32_t>(), stream),
           "vkhr_volume_finalize");
 }
 
@@ -210,6 +212,7 @@ vkhr_volume HairStyle::Volume::view() const {
     std::memset(&v, 0, sizeof(v));
     v.density = density.as<uint8_t>();
     v.cells = cells.as<uint64_t>();
+    v.occupancy = occupancy.as<uint32_t>();
     v.tangents = tangent.as<int8_t>();
     v.nx = uint32_t(resolution.x);
     v.ny = uint32_t(resolution.y);
