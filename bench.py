@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import re
 import os
 import subprocess
 import sys
@@ -75,6 +76,42 @@ def time_adjoint(dev, reps=10):
     e[3].record()
     torch.cuda.synchronize()
     return e[0].elapsed_time(e[1]) / reps, e[2].elapsed_time(e[3]) / reps
+
+
+def time_adjoint_batch(dev):
+    """Batched adjoint: 8 generator styles (128^3 float densities) x 8 ring
+    views at 512x512, 256 steps; returns (per-render ms, graph ms) for the
+    forward + L2 loss gradient + adjoint of all 64 renders."""
+    import torch
+    from vkhr_amd.adjoint_batch import AdjointBatch, AdjointProblem
+    prob = AdjointProblem(dev)
+    batch = AdjointBatch(prob)
+    for _ in range(2):
+        batch.launch()
+        prob.run_plain()
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    e[0].record()
+    prob.run_plain()
+    e[1].record()
+    torch.cuda.synchronize()
+    e[2].record(batch.stream)
+    batch.launch(wait=False)
+    e[3].record(batch.stream)
+    torch.cuda.synchronize()
+    return e[0].elapsed_time(e[1]), e[2].elapsed_time(e[3])
+
+
+TRAFFIC_SUMMARY = Path(__file__).resolve().parent / "profiles" / "raymarch_traffic_config1_1080p.md"
+
+
+def measured_traffic():
+    """HBM bytes per raymarch launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
+    if not TRAFFIC_SUMMARY.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", TRAFFIC_SUMMARY.read_text())
+    return float(m.group(1)) if m else None
 
 
 def time_batch(dev, reps=3):
@@ -253,7 +290,7 @@ def main():
         "segments_per_s": n_seg / t_vox,
         "voxelize_ms": t_vox * 1e3,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "raymarch_kernel", "algorithmic_bytes": ray_bytes},
     }
     if not args.no_extras:
@@ -264,6 +301,11 @@ def main():
         t_plain, t_batch = time_batch(dev)
         result["batched"] = {"workload": "32 styles, 5k-140k strands, 64^3/128^3/256^3, 960x540, 512 steps",
                              "per_style_ms": t_plain, "batched_graph_ms": t_batch, "speedup": t_plain / t_batch}
+        t_plain, t_batch = time_adjoint_batch(dev)
+        result["batched_adjoint"] = {"workload": "8 styles x 8 views, 128^3 float32, 512x512, 256 steps, "
+                                                 "forward + L2 loss gradient + adjoint",
+                                     "per_render_ms": t_plain, "batched_graph_ms": t_batch,
+                                     "speedup": t_plain / t_batch}
     if not args.no_cpu_baseline:
         vol_np = dict(density=vol["density"].cpu().numpy(), tangents=vol["tangents"].cpu().numpy())
         b = {k: np.asarray(v) for k, v in bounds.items()}

@@ -62,7 +62,76 @@ private:
 
 class HairStyle final {
 public:
+    // HairStyle::Error (hair_style.hh:28-54): the state of the last load/save.
+    enum class Error {
+        None,
+
+        OpeningFile,
+        ReadingFileHeader,
+        WritingFileHeader,
+
+        InvalidSignature,
+
+        ReadingSegments,
+        ReadingVertices,
+        ReadingThickness,
+        ReadingTransparency,
+        ReadingColor,
+        ReadingTangents,
+        ReadingIndices,
+
+        WritingSegments,
+        WritingVertices,
+        WritingThickness,
+        WritingTransparency,
+        WritingColor,
+        WritingTangents,
+        WritingIndices,
+
+        InvalidFormat
+    };
+
     HairStyle() = default;
+    explicit HairStyle(const std::string& file_path);    // load(file_path)
+
+    explicit operator bool() const;                        // no error state
+    Error get_last_error_state() const;
+
+    // The .hair format (Cem Yuksel's HAIR files with vkhr's 128-byte header:
+    // hair_style.hh:147-174, hair_style.cc:24-70, 576-665): the header, then
+    // the arrays its bit field announces, in the order segments (uint16 per
+    // strand), vertices, thickness, transparency, colour, tangents, indices.
+    bool load(const std::string& file_path);
+    bool save(const std::string& file_path) const;
+    std::size_t get_size() const;                          // bytes save() writes
+
+    // The strand order shuffle and the random strand reduction
+    // (hair_style.cc:371-449), driven by xorshift64 of `seed` (the reference
+    // seeds it from std::random_device; here it is a member the caller sets).
+    void shuffle();
+    void reduce(float ratio);
+    std::uint64_t seed = 0x2545f4914f6cdd1dull;
+
+    bool has_segments() const { return !segments.empty(); }
+    bool has_vertices() const { return !vertices.empty(); }
+    bool has_thickness() const { return !thickness.empty(); }
+    bool has_transparency() const { return !transparency.empty(); }
+    bool has_color() const { return !color.empty(); }
+    bool has_tangents() const { return !tangents.empty(); }
+    bool has_indices() const { return !indices.empty(); }
+    bool has_bounding_box() const;                         // from the header
+
+    unsigned get_default_segment_count() const;
+    void set_default_segment_count(unsigned default_segment_count);
+    float get_default_thickness() const;
+    void set_default_thickness(float default_thickness);
+    float get_default_transparency() const;
+    void set_default_transparency(float default_transparency);
+    vec3 get_default_color() const;
+    void set_default_color(const vec3& default_color);
+    const char* get_information() const;
+    void set_information(const std::string& information);
+    void set_strand_count(unsigned strand_count);          // for styles without a segment array
 
     // HairStyle::Volume (hair_style.hh:89-101): the host vectors of the
     // reference plus the device volumes the renderers read.
@@ -85,15 +154,15 @@ public:
         bool normalized = false;
     };
 
-    unsigned get_strand_count() const;
-    unsigned get_segment_count() const;
+    unsigned get_strand_count() const;     // segments.size(), or the header's without a segment array
+    unsigned get_segment_count() const;    // vertices - strands
     unsigned get_vertex_count() const;
 
     void generate_thickness(float radius);   // hair_style.cc:155-169
     void generate_tangents();                // hair_style.cc:171-194 (on the device)
     void generate_indices();                 // hair_style.cc:196-213 (on the device)
-    void generate_bounding_box();            // hair_style.cc:215-234 (on the device)
-    AABB get_bounding_box() const;           // hair_style.cc:236-255
+    void generate_bounding_box();            // hair_style.cc:215-234 (into the header, as the reference)
+    AABB get_bounding_box() const;           // hair_style.cc:236-255 (from the header)
 
     Volume voxelize_vertices(std::size_t width, std::size_t height, std::size_t depth) const;
     Volume voxelize_segments(std::size_t width, std::size_t height, std::size_t depth) const;
@@ -102,6 +171,8 @@ public:
     std::vector<unsigned short> segments;
     std::vector<vec3> vertices;
     std::vector<float> thickness;
+    std::vector<float> transparency;
+    std::vector<vec3> color;
     std::vector<vec3> tangents;
     std::vector<unsigned> indices;
 
@@ -114,10 +185,36 @@ public:
 
     vkhr_stream stream = nullptr;
 
+    // The 128-byte file header (hair_style.hh:147-174); the bit field is
+    // kept as its uint32 word (bit 0 has_segments ... bit 7
+    // has_bounding_box, the order of the reference's field declaration).
+    struct FileHeader {
+        char signature[4];
+        std::uint32_t strand_count, vertex_count;
+        std::uint32_t field;
+        std::uint32_t default_segment_count;
+        float default_thickness, default_transparency;
+        float default_color[3];
+        char information[64];
+        float bounding_box_min[3];
+        float bounding_box_max[3];
+    };
+    static_assert(sizeof(FileHeader) == 128, "the .hair header is 128 bytes");
+    enum FieldBit : std::uint32_t {
+        kHasSegments = 1u << 0, kHasVertices = 1u << 1, kHasThickness = 1u << 2, kHasTransparency = 1u << 3,
+        kHasColor = 1u << 4, kHasTangents = 1u << 5, kHasIndices = 1u << 6, kHasBoundingBox = 1u << 7,
+    };
+    const FileHeader& header() const { return file_header_; }
+
 private:
     DeviceBuffer strand_offsets() const;
-    AABB box_{};
-    bool has_box_ = false;
+    std::vector<std::uint32_t> host_strand_offsets() const;
+    bool set_error_state(Error e) const;
+    bool valid_signature() const;
+    bool format_is_valid() const;
+    void complete_header() const;
+    mutable FileHeader file_header_{};
+    mutable Error error_state_ = Error::None;
     mutable DeviceBuffer d_vertices_, d_tangents_, d_thickness_, d_indices_;
 };
 

@@ -430,6 +430,59 @@ int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styles, uint32_
 int vkhr_batch_launch(vkhr_batch* batch, vkhr_stream stream);
 int vkhr_batch_destroy(vkhr_batch* batch);
 
+/* ----------------------------------------------------- batched adjoint form */
+
+/* No counterpart in the reference (which has no gradients): one training
+ * step of the differentiable raymarch for many renders of several styles --
+ * the forward (vkhr_raymarch_record) of every render, the L2 loss against
+ * its target image, and the adjoint, with the density / parameter
+ * gradients of all renders of a style summed -- as one graph launch.
+ *
+ * A style: its float density volume (volume.density_f32 and tangents
+ * required) and its gradient outputs, overwritten by each launch with the
+ * gradient of the sum of its renders' losses: d_density (nx*ny*nz floats)
+ * and d_params (3 floats: absorption, ADSM thickness, Kajiya-Kay exponent). */
+typedef struct vkhr_adjoint_style {
+    vkhr_volume volume;
+    float* d_density;
+    float* d_params;
+} vkhr_adjoint_style;
+
+/* A render of a style: the view and parameters of vkhr_raymarch_record, the
+ * target image (width*height float4), the forward image it writes (rgba,
+ * width*height float4), its record (vkhr_march_record_bytes) and its loss
+ * (1 float, the sum of squared differences over all channels; nullable). */
+typedef struct vkhr_adjoint_render {
+    uint32_t style;
+    uint32_t reserved_;
+    vkhr_camera camera;
+    vkhr_light light;
+    vkhr_strand_params strand;
+    vkhr_render_params params;
+    vkhr_diff_params diff;
+    const float* target;
+    float* rgba;
+    void* record;
+    float* loss;
+} vkhr_adjoint_render;
+
+typedef struct vkhr_adjoint_batch vkhr_adjoint_batch;
+
+/* Builds the device tables and the per-style cell accumulators (8 floats
+ * per cell of each style, owned by the batch). */
+int vkhr_adjoint_batch_create(const vkhr_adjoint_style* styles, uint32_t n_styles,
+                              const vkhr_adjoint_render* renders, uint32_t n_renders, uint32_t width,
+                              uint32_t height, vkhr_adjoint_batch** out);
+
+/* Runs the step as one hipGraph (captured on the first call on a created,
+ * non-NULL stream): clear, forward of all renders, adjoint of all renders
+ * (the loss gradient 2 (rgba - target) is formed in the adjoint), and the
+ * fold of each style's cells into its d_density.  The images and records
+ * are bit-identical to per-render vkhr_raymarch_record; the gradients equal
+ * the sums of per-render vkhr_raymarch_adjoint up to float summation order. */
+int vkhr_adjoint_batch_launch(vkhr_adjoint_batch* batch, vkhr_stream stream);
+int vkhr_adjoint_batch_destroy(vkhr_adjoint_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

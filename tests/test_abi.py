@@ -49,7 +49,8 @@ def test_struct_layouts_match_header():
              ("vkhr_light", capi.Light), ("vkhr_strand_params", capi.StrandParams),
              ("vkhr_render_params", capi.RenderParams), ("vkhr_ppll", capi.Ppll),
              ("vkhr_frag_capture", capi.FragCapture), ("vkhr_diff_params", capi.DiffParams),
-             ("vkhr_batch_style", capi.BatchStyle), ("vkhr_tuning", capi.Tuning)]
+             ("vkhr_batch_style", capi.BatchStyle), ("vkhr_tuning", capi.Tuning),
+             ("vkhr_adjoint_style", capi.AdjointStyle), ("vkhr_adjoint_render", capi.AdjointRender)]
     for cname, cls in pairs:
         hdr = _header_struct_fields(cname)
         names = [f[0] for f in cls._fields_]

@@ -87,6 +87,16 @@ class BatchStyle(C.Structure):
                 ("camera", Camera), ("light", Light), ("strand", StrandParams), ("params", RenderParams)]
 
 
+class AdjointStyle(C.Structure):
+    _fields_ = [("volume", Volume), ("d_density", C.c_void_p), ("d_params", C.c_void_p)]
+
+
+class AdjointRender(C.Structure):
+    _fields_ = [("style", C.c_uint32), ("reserved_", C.c_uint32), ("camera", Camera), ("light", Light),
+                ("strand", StrandParams), ("params", RenderParams), ("diff", DiffParams), ("target", C.c_void_p),
+                ("rgba", C.c_void_p), ("record", C.c_void_p), ("loss", C.c_void_p)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("reserved_", C.c_uint32)]
@@ -123,6 +133,10 @@ SIGNATURES = {
     "vkhr_batch_create": [C.POINTER(BatchStyle), U32, U32, U32, C.POINTER(Tuning), C.POINTER(C.c_void_p)],
     "vkhr_batch_launch": [P, P],
     "vkhr_batch_destroy": [P],
+    "vkhr_adjoint_batch_create": [C.POINTER(AdjointStyle), U32, C.POINTER(AdjointRender), U32, U32, U32,
+                                  C.POINTER(C.c_void_p)],
+    "vkhr_adjoint_batch_launch": [P, P],
+    "vkhr_adjoint_batch_destroy": [P],
 }
 # entry points that do not return a status
 SIZE_FUNCTIONS = {"vkhr_march_record_bytes": [U32, U32], "vkhr_occupancy_words": [U32, U32, U32]}

@@ -22,8 +22,10 @@
 // one atomic each.
 #include "march.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 namespace vkhr {
 namespace {
@@ -37,6 +39,8 @@ struct AdjointArgs {
     const float4* d_rgba;
     float* cell_grad;           // 8 floats per cell, (nx+1)(ny+1)(nz+1) cells
     float* d_params;            // absorption, ADSM thickness, shininess
+    const float4* target;       // with a target: dL/d(rgba) = 2 (rgba - target), L = sum of squares
+    float* loss;                // L is added here (with a target; nullable)
     int n_steps;                // iterations of the shader's float loop
     double ln_transmit;         // log(1 - alpha)
     double light_texel[3];
@@ -137,6 +141,7 @@ gather_cells_kernel(const float* __restrict__ cg, float* __restrict__ grad, int 
 // all 256 threads of a block: 2x2 waves of 8x8 pixels from (x0, y0); red is
 // the block's reduction buffer for the parameter gradients.
 __device__ __forceinline__ void adjoint_tile(const AdjointArgs& a, int x0, int y0, float (*red)[256 / kWave]) {
+    // red: 4 x (waves per block) floats
     const RaymarchArgs& f = a.f;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int px = x0 + (wave & 1) * 8 + (lane & 7);
@@ -147,10 +152,18 @@ __device__ __forceinline__ void adjoint_tile(const AdjointArgs& a, int x0, int y
     bool frag = false;
     MarchRecord r;
     float4 g = make_float4(0, 0, 0, 0);
+    float sq = 0.0f;
     if (inside) {
         r = a.record[pix];
         frag = r.state == 1;
-        g = a.d_rgba[pix];
+        if (a.target) {  // L2 loss against the target image
+            const float4 o = f.rgba[pix], t = a.target[pix];
+            const float4 d = make_float4(o.x - t.x, o.y - t.y, o.z - t.z, o.w - t.w);
+            g = make_float4(2.0f * d.x, 2.0f * d.y, 2.0f * d.z, 2.0f * d.w);
+            sq = d.x * d.x + d.y * d.y + d.z * d.z + d.w * d.w;
+        } else {
+            g = a.d_rgba[pix];
+        }
     }
     double c_primary = 0.0, c_shadow = 0.0, G = 0.0;
     double d_alpha = 0.0, d_k = 0.0, d_p = 0.0;
@@ -180,18 +193,19 @@ __device__ __forceinline__ void adjoint_tile(const AdjointArgs& a, int x0, int y
             d_alpha += G * (f.adsm_k * double(r.shadow_sum)) * (-1.0 / (1.0 - f.alpha));
         }
     }
-    // parameter gradients: wave, then block, then one atomic each
-    double pv[3] = {d_alpha, d_k, d_p};
-    for (int k = 0; k < 3; ++k) {
-        float v = float(pv[k]);
+    // parameter gradients (and the loss): wave, then block, then one atomic each
+    const float pv[4] = {float(d_alpha), float(d_k), float(d_p), sq};
+    for (int k = 0; k < 4; ++k) {
+        float v = pv[k];
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
         if (lane == 0) red[k][wave] = v;
     }
     __syncthreads();
-    if (threadIdx.x < 3) {
+    if (threadIdx.x < 4) {
         float v = 0.0f;
         for (int w = 0; w < int(blockDim.x / kWave); ++w) v += red[threadIdx.x][w];
-        if (v != 0.0f) atomicAdd(&a.d_params[threadIdx.x], v);
+        float* dst = threadIdx.x < 3 ? &a.d_params[threadIdx.x] : a.loss;
+        if (v != 0.0f && dst) atomicAdd(dst, v);
     }
     __syncthreads();  // red is reused by the next tile of a persistent block
 
@@ -237,8 +251,91 @@ __device__ __forceinline__ void adjoint_tile(const AdjointArgs& a, int x0, int y
 
 __global__ void __launch_bounds__(256)
 adjoint_kernel(AdjointArgs a) {
-    __shared__ float red[3][256 / kWave];
+    __shared__ float red[4][256 / kWave];
     adjoint_tile(a, blockIdx.x * 16, blockIdx.y * 16, red);
+}
+
+// ---------------------------------------------------------- batched form
+
+struct AdjointBatchStyle {
+    float* cell_grad;           // 8 floats per cell, owned by the batch
+    float* d_density;
+    float* d_params;
+    uint32_t nx, ny, nz;
+    uint32_t reserved;
+};
+
+// A contiguous range [begin, end) of one style's cell-accumulator floats
+// (clear) or voxels (fold).
+struct AdjointChunk {
+    uint32_t style;
+    uint32_t reserved;
+    uint64_t begin, end;
+};
+
+struct AdjointBatchDevice {
+    AdjointBatchStyle* styles;
+    AdjointChunk* clear_chunks;
+    AdjointChunk* fold_chunks;
+    float** losses;             // per render, nullable entries
+    uint32_t n_styles, n_renders;
+    uint32_t n_clear_chunks, n_fold_chunks;
+    uint32_t tiles_x, tiles_per_render;
+};
+
+__global__ void __launch_bounds__(256)
+adjoint_batch_clear_kernel(AdjointBatchDevice d) {
+    if (blockIdx.x == 0) {
+        for (uint32_t s = threadIdx.x; s < 3 * d.n_styles; s += blockDim.x) d.styles[s / 3].d_params[s % 3] = 0.0f;
+        for (uint32_t r = threadIdx.x; r < d.n_renders; r += blockDim.x)
+            if (d.losses[r]) *d.losses[r] = 0.0f;
+    }
+    for (uint32_t c = blockIdx.x; c < d.n_clear_chunks; c += gridDim.x) {
+        const AdjointChunk ch = d.clear_chunks[c];
+        float* cg = d.styles[ch.style].cell_grad;
+        for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) cg[i] = 0.0f;
+    }
+}
+
+// One block per (render, 16x16 tile): the recording forward.
+__global__ void __launch_bounds__(256)
+adjoint_batch_forward_kernel(AdjointBatchDevice d, const AdjointArgs* __restrict__ args) {
+    __shared__ uint32_t occ_lds[1];  // float volumes carry no occupancy
+    const uint32_t item = uint32_t(__builtin_amdgcn_readfirstlane(int(blockIdx.x)));
+    const uint32_t r = item / d.tiles_per_render, tile = item % d.tiles_per_render;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int px = int(tile % d.tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+    const int py = int(tile / d.tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
+    raymarch_pixel<true>(args[r].f, px, py, OccupancyView{occ_lds, false});
+}
+
+// One block per (render, 16x16 tile): the adjoint with the L2 loss gradient.
+__global__ void __launch_bounds__(256)
+adjoint_batch_backward_kernel(AdjointBatchDevice d, const AdjointArgs* __restrict__ args) {
+    __shared__ float red[4][256 / kWave];
+    const uint32_t item = uint32_t(__builtin_amdgcn_readfirstlane(int(blockIdx.x)));
+    const uint32_t r = item / d.tiles_per_render, tile = item % d.tiles_per_render;
+    adjoint_tile(args[r], int(tile % d.tiles_x) * 16, int(tile / d.tiles_x) * 16, red);
+}
+
+// d_density = the fold of each style's cells (overwritten).
+__global__ void __launch_bounds__(256)
+adjoint_batch_fold_kernel(AdjointBatchDevice d) {
+    for (uint32_t c = blockIdx.x; c < d.n_fold_chunks; c += gridDim.x) {
+        const AdjointChunk ch = d.fold_chunks[c];
+        const AdjointBatchStyle& st = d.styles[ch.style];
+        const size_t cx = size_t(st.nx) + 1, cy = size_t(st.ny) + 1;
+        for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
+            const size_t x = i % st.nx, yz = i / st.nx, y = yz % st.ny, z = yz / st.ny;
+            float sum = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const size_t cell = (x + 1 - (q & 1)) + cx * ((y + 1 - ((q >> 1) & 1)) + cy * (z + 1 - (q >> 2)));
+                sum += st.cell_grad[cell * 8u + size_t(q)];
+            }
+            st.d_density[i] = sum;
+        }
+    }
 }
 
 }  // namespace
@@ -263,6 +360,16 @@ static int diff_args(const vkhr_volume* volume, const vkhr_camera* camera, const
         a->shininess = diff->shininess;
     }
     return VKHR_OK;
+}
+
+// The per-launch constants of the adjoint derived from the forward's.
+static void finish_adjoint_args(AdjointArgs* a) {
+    int n = 0;
+    for (float t = 0.0f; t < 1.0f; t += a->f.step_size) ++n;
+    a->n_steps = n;
+    a->ln_transmit = std::log(1.0 - a->f.alpha);
+    for (int c = 0; c < 3; ++c)
+        a->light_texel[c] = (a->f.light_origin[c] - a->f.origin[c]) / a->f.size[c] * a->f.nres[c] - 0.5;
 }
 
 extern "C" int vkhr_raymarch_record(const vkhr_volume* volume, const vkhr_camera* camera, const vkhr_light* light,
@@ -304,18 +411,8 @@ extern "C" int vkhr_raymarch_adjoint(const vkhr_volume* volume, const vkhr_camer
     a->record = static_cast<const MarchRecord*>(record);
     a->d_rgba = reinterpret_cast<const float4*>(d_rgba);
     a->d_params = d_params;
-    int n = 0;
-    for (float t = 0.0f; t < 1.0f; t += a->f.step_size) ++n;
-    a->n_steps = n;
-    a->ln_transmit = std::log(1.0 - a->f.alpha);
-    for (int c = 0; c < 3; ++c)
-        a->light_texel[c] = (a->f.light_origin[c] - a->f.origin[c]) / a->f.size[c] * a->f.nres[c] - 0.5;
+    finish_adjoint_args(a);
     const size_t n_cells = size_t(volume->nx + 1) * (volume->ny + 1) * (volume->nz + 1);
-    if (n_cells >= (size_t(1) << 32)) {
-        delete a;
-        set_error("volume %ux%ux%u too large for the adjoint", volume->nx, volume->ny, volume->nz);
-        return VKHR_ERR_INVALID;
-    }
     void* cg = nullptr;
     if ((rc = scratch(n_cells * 8 * sizeof(float), &cg))) {
         delete a;
@@ -334,5 +431,142 @@ extern "C" int vkhr_raymarch_adjoint(const vkhr_volume* volume, const vkhr_camer
                                                                       int(volume->ny), int(volume->nz));
     delete a;
     VKHR_CHECK_LAUNCH();
+    return VKHR_OK;
+}
+
+// ------------------------------------------------------------ batched form
+
+struct vkhr_adjoint_batch {
+    AdjointBatchDevice d;
+    AdjointArgs* args = nullptr;      // device, per render
+    std::vector<void*> allocations;
+    uint32_t n_items = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void release_adjoint_batch(vkhr_adjoint_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (void* p : b->allocations) (void)hipFree(p);
+    b->allocations.clear();
+}
+
+template <class T>
+int upload_table(vkhr_adjoint_batch* b, const std::vector<T>& host, T** out) {
+    void* p = nullptr;
+    VKHR_CHECK_HIP(hipMalloc(&p, std::max<size_t>(host.size() * sizeof(T), 16)));
+    b->allocations.push_back(p);
+    if (!host.empty()) VKHR_CHECK_HIP(hipMemcpy(p, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+    *out = static_cast<T*>(p);
+    return VKHR_OK;
+}
+
+int build_adjoint_batch(vkhr_adjoint_batch* b, const vkhr_adjoint_style* styles, uint32_t n_styles,
+                        const vkhr_adjoint_render* renders, uint32_t n_renders, uint32_t width, uint32_t height) {
+    int rc = VKHR_OK;
+    std::vector<AdjointBatchStyle> st(n_styles);
+    std::vector<AdjointChunk> clear, fold;
+    const uint64_t kChunk = 1u << 16;
+    for (uint32_t s = 0; s < n_styles; ++s) {
+        const vkhr_adjoint_style& in = styles[s];
+        VKHR_REQUIRE(in.volume.density_f32 && in.volume.tangents && in.d_density && in.d_params, VKHR_ERR_INVALID,
+                     "style %u: the float density, tangents, d_density and d_params are required", s);
+        VKHR_REQUIRE(in.volume.nx && in.volume.ny && in.volume.nz, VKHR_ERR_INVALID, "style %u: empty volume", s);
+        const uint64_t cells = uint64_t(in.volume.nx + 1) * (in.volume.ny + 1) * (in.volume.nz + 1);
+        VKHR_REQUIRE(cells < (uint64_t(1) << 32), VKHR_ERR_OVERFLOW, "style %u: volume too large", s);
+        void* cg = nullptr;
+        VKHR_CHECK_HIP(hipMalloc(&cg, cells * 8 * sizeof(float)));
+        b->allocations.push_back(cg);
+        st[s] = {static_cast<float*>(cg), in.d_density, in.d_params, in.volume.nx, in.volume.ny, in.volume.nz, 0};
+        for (uint64_t a = 0; a < cells * 8; a += kChunk) clear.push_back({s, 0, a, std::min(cells * 8, a + kChunk)});
+        const uint64_t vox = uint64_t(in.volume.nx) * in.volume.ny * in.volume.nz;
+        for (uint64_t a = 0; a < vox; a += kChunk) fold.push_back({s, 0, a, std::min(vox, a + kChunk)});
+    }
+    std::vector<AdjointArgs> args(n_renders);
+    std::vector<float*> losses(n_renders, nullptr);
+    for (uint32_t r = 0; r < n_renders; ++r) {
+        const vkhr_adjoint_render& in = renders[r];
+        VKHR_REQUIRE(in.style < n_styles, VKHR_ERR_INVALID, "render %u: style %u of %u", r, in.style, n_styles);
+        VKHR_REQUIRE(in.target && in.rgba && in.record, VKHR_ERR_INVALID, "render %u: target/rgba/record is null", r);
+        AdjointArgs& a = args[r];
+        std::memset(&a, 0, sizeof(a));
+        if ((rc = diff_args(&styles[in.style].volume, &in.camera, &in.light, &in.strand, &in.params, &in.diff, width,
+                            height, &a.f)))
+            return rc;
+        a.f.rgba = reinterpret_cast<float4*>(in.rgba);
+        a.f.record = static_cast<MarchRecord*>(in.record);
+        a.record = static_cast<const MarchRecord*>(in.record);
+        a.target = reinterpret_cast<const float4*>(in.target);
+        a.loss = in.loss;
+        a.d_params = styles[in.style].d_params;
+        a.cell_grad = st[in.style].cell_grad;
+        finish_adjoint_args(&a);
+        losses[r] = in.loss;
+    }
+    AdjointBatchDevice& d = b->d;
+    if ((rc = upload_table(b, st, &d.styles)) || (rc = upload_table(b, clear, &d.clear_chunks)) ||
+        (rc = upload_table(b, fold, &d.fold_chunks)) || (rc = upload_table(b, losses, &d.losses)) ||
+        (rc = upload_table(b, args, &b->args)))
+        return rc;
+    d.n_styles = n_styles;
+    d.n_renders = n_renders;
+    d.n_clear_chunks = uint32_t(clear.size());
+    d.n_fold_chunks = uint32_t(fold.size());
+    d.tiles_x = (width + 15) / 16;
+    d.tiles_per_render = d.tiles_x * ((height + 15) / 16);
+    b->n_items = d.tiles_per_render * n_renders;
+    return VKHR_OK;
+}
+
+}  // namespace
+
+extern "C" int vkhr_adjoint_batch_create(const vkhr_adjoint_style* styles, uint32_t n_styles,
+                                         const vkhr_adjoint_render* renders, uint32_t n_renders, uint32_t width,
+                                         uint32_t height, vkhr_adjoint_batch** out) {
+    clear_error();
+    int rc = ensure_device();
+    if (rc) return rc;
+    VKHR_REQUIRE(styles && n_styles && renders && n_renders && out, VKHR_ERR_INVALID, "no styles or renders");
+    VKHR_REQUIRE(width && height, VKHR_ERR_INVALID, "empty image");
+    VKHR_REQUIRE(uint64_t(n_renders) * ((width + 15) / 16) * ((height + 15) / 16) < (uint64_t(1) << 31),
+                 VKHR_ERR_INVALID, "too many tiles");
+    vkhr_adjoint_batch* b = new vkhr_adjoint_batch();
+    std::memset(&b->d, 0, sizeof(b->d));
+    if ((rc = build_adjoint_batch(b, styles, n_styles, renders, n_renders, width, height))) {
+        release_adjoint_batch(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return VKHR_OK;
+}
+
+extern "C" int vkhr_adjoint_batch_launch(vkhr_adjoint_batch* b, vkhr_stream stream) {
+    clear_error();
+    VKHR_REQUIRE(b, VKHR_ERR_INVALID, "null batch");
+    VKHR_REQUIRE(stream, VKHR_ERR_INVALID, "the batched launch is graph-captured: pass a created stream, not NULL");
+    hipStream_t s = as_stream(stream);
+    if (!b->exec) {
+        VKHR_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        adjoint_batch_clear_kernel<<<4096, 256, 0, s>>>(b->d);
+        adjoint_batch_forward_kernel<<<b->n_items, 256, 0, s>>>(b->d, b->args);
+        adjoint_batch_backward_kernel<<<b->n_items, 256, 0, s>>>(b->d, b->args);
+        adjoint_batch_fold_kernel<<<4096, 256, 0, s>>>(b->d);
+        hipError_t e = hipStreamEndCapture(s, &b->graph);
+        VKHR_CHECK_HIP(e);
+        VKHR_CHECK_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    VKHR_CHECK_HIP(hipGraphLaunch(b->exec, s));
+    return VKHR_OK;
+}
+
+extern "C" int vkhr_adjoint_batch_destroy(vkhr_adjoint_batch* b) {
+    clear_error();
+    if (!b) return VKHR_OK;
+    release_adjoint_batch(b);
+    delete b;
     return VKHR_OK;
 }

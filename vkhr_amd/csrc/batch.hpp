@@ -24,7 +24,7 @@ struct BatchVolStyle {
     uint32_t* occupancy;                        // nullable
 };
 
-// A contiguous range of one style's voxels (or one block of its cells).
+// A contiguous range of one style's voxels (or one cell level of its cells).
 struct BatchChunk {
     uint32_t style;
     uint32_t reserved;

@@ -163,8 +163,7 @@ extern "C" int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styl
         pair_prefix[s + 1] = pair_prefix[s] + v.n_pairs;
         uint64_t nvox = uint64_t(in.volume) * in.volume * in.volume;
         for (uint64_t a = 0; a < nvox; a += kChunk) vchunks.push_back({s, 0, a, std::min(nvox, a + kChunk)});
-        const uint64_t blocks = uint64_t(cell_col_blocks(in.volume)) * bricks_along(in.volume) * (in.volume + 1);
-        for (uint64_t a = 0; a < blocks; ++a) cchunks.push_back({s, 0, a, a + 1});
+        for (uint64_t k = 0; k <= in.volume; ++k) cchunks.push_back({s, 0, k, k + 1});  // cell levels
         vkhr_volume vol;
         std::memset(&vol, 0, sizeof(vol));
         vol.density = in.density;

@@ -123,75 +123,56 @@ __device__ __forceinline__ uint32_t texel_density(uint32_t d, const DensityMap& 
     return m.normalize ? normalize_value(d, m.lo, m.scaling, m.flat) : d;
 }
 
-// One block of the corner-packed layout -- cells (i, j, k), i in [-1, nx-1],
+// One tile of the corner-packed layout -- cells (i, j, k), i in [-1, nx-1],
 // index (i+1) + (nx+1) * ((j+1) + (ny+1) * (k+1)), each the 8 texels of the
 // trilinear footprint whose lower corner is (i,j,k) as one uint64 (byte b =
 // texel (i+(b&1), j+(b>>1&1), k+(b>>2)), 0 outside the volume) -- from the
-// final uint8 density: 8 cell rows j+1 in [8 by, 8 by + 8) at cell level
-// k+1 = ck, cell columns i+1 in [256 mc, 256 mc + 256), 256 threads, each 8
-// consecutive cells of one row (one brick wide).  Each thread reads the 9
-// texels of its 4 texel rows once and stores 64 contiguous bytes.  The
-// bricks (8x8x8 cells) holding a nonzero cell are or-ed into occupancy
-// (nullable; two word atomics per block).  All threads of the block call it.
-constexpr uint32_t kCellBlockCols = 32;   // threads along x: 32 x 8 = 256 cells
+// final uint8 density: 64 cell columns i+1 in [64 xb, 64 xb + 64) (the lanes
+// of a wave) x 4 cell rows j+1 in [4 yb, 4 yb + 4) (the waves of a block) at
+// cell level k+1 = ck.  Lanes read consecutive texels and store consecutive
+// cells (coalesced).  The bricks (8x8x8 cells) holding a nonzero cell are
+// or-ed into occupancy (nullable; one word atomic per tile with any).  All
+// threads of the block call it.
+constexpr uint32_t kTileRows = 4;   // cell rows per tile (waves of a 256-thread block)
 
-__host__ __device__ __forceinline__ uint32_t cell_col_blocks(uint32_t nx) {
-    return (nx + 1u + 8u * kCellBlockCols - 1u) / (8u * kCellBlockCols);
-}
-
-__device__ __forceinline__ void cells_block(const uint8_t* __restrict__ density, uint32_t nx, uint32_t ny, uint32_t nz,
-                                            uint32_t by, uint32_t ck, uint32_t mc,
-                                            unsigned long long* __restrict__ cells, uint32_t* occupancy,
-                                            uint32_t* lds_flags) {
-    const uint32_t r = threadIdx.x / kCellBlockCols, mloc = threadIdx.x % kCellBlockCols;
-    const uint32_t cj = (by << kBrickShift) + r;          // cell row j + 1
-    const uint32_t x0 = (mc * kCellBlockCols + mloc) * 8u; // first cell column i + 1
-    if (threadIdx.x < kCellBlockCols) lds_flags[threadIdx.x] = 0u;
-    __syncthreads();
-    if (cj <= ny && x0 <= nx) {
-        uint32_t t[4][9];  // texel rows (j + (q&1), k + (q>>1)), texels x0 - 1 .. x0 + 7
+__device__ __forceinline__ void cells_tile(const uint8_t* __restrict__ density, uint32_t nx, uint32_t ny,
+                                           uint32_t nz, uint32_t xb, uint32_t yb, uint32_t ck,
+                                           unsigned long long* __restrict__ cells, uint32_t* occupancy,
+                                           uint32_t* lds_flags) {
+    const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const uint32_t cxi = xb * kWave + lane;        // cell column i + 1
+    const uint32_t cj = yb * kTileRows + wave;     // cell row j + 1
+    unsigned long long packed = 0ull;
+    if (cxi <= nx && cj <= ny) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < 4; ++q) {  // texel rows (j + (q&1), k + (q>>1))
             const int y = int(cj) - 1 + (q & 1), z = int(ck) - 1 + (q >> 1);
-            const bool row_in = y >= 0 && z >= 0 && y < int(ny) && z < int(nz);
-            const uint8_t* src = density + (row_in ? uint32_t(y) * nx + uint32_t(z) * nx * ny : 0u);
-#pragma unroll
-            for (int e = 0; e < 9; ++e) {
-                const int x = int(x0) - 1 + e;
-                t[q][e] = (row_in && x >= 0 && x < int(nx)) ? uint32_t(src[x]) : 0u;
-            }
+            if (y < 0 || z < 0 || y >= int(ny) || z >= int(nz)) continue;
+            const uint8_t* src = density + uint32_t(y) * nx + uint32_t(z) * nx * ny;
+            const int x = int(cxi) - 1;
+            const uint32_t t0 = x >= 0 ? src[x] : 0u;
+            const uint32_t t1 = cxi < nx ? src[cxi] : 0u;
+            packed |= (unsigned long long)(t0 | (t1 << 8)) << (16 * q);
         }
-        unsigned long long* out = cells + size_t(cj + (ny + 1u) * ck) * (nx + 1u) + x0;
-        const uint32_t n_cells = min(8u, nx + 1u - x0);
-        unsigned long long any = 0ull;
-#pragma unroll
-        for (uint32_t c = 0; c < 8; ++c) {
-            if (c < n_cells) {
-                unsigned long long packed = 0ull;
-#pragma unroll
-                for (int b = 0; b < 8; ++b)
-                    packed |= (unsigned long long)(t[b >> 1][c + (b & 1)]) << (8 * b);
-                out[c] = packed;
-                any |= packed;
-            }
-        }
-        if (occupancy && any) lds_flags[mloc] = 1u;
+        cells[size_t(cj + (ny + 1u) * ck) * (nx + 1u) + cxi] = packed;
     }
     if (!occupancy) return;
+    // lanes 8g..8g+7 of a wave are the cells of brick 8 xb + g
+    const unsigned long long nonzero = __ballot(packed != 0ull);
+    uint32_t m = 0;
+#pragma unroll
+    for (int g = 0; g < 8; ++g) m |= ((nonzero >> (8 * g)) & 0xffull) ? (1u << g) : 0u;
+    if (threadIdx.x == 0) *lds_flags = 0u;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        // bricks bx = mc * 32 + m (m < 32) at bits first + bx: at most two words
+    if (lane == 0 && m) atomicOr(lds_flags, m);
+    __syncthreads();
+    if (threadIdx.x == 0 && *lds_flags) {
+        // bricks 8 xb .. 8 xb + 7 of row (by, bz): bits first .. first + 7
         const uint32_t bxs = bricks_along(nx), bys = bricks_along(ny);
-        const uint32_t first = bxs * (by + bys * (ck >> kBrickShift)) + mc * kCellBlockCols;
-        unsigned long long mask = 0ull;
-        for (uint32_t m = 0; m < kCellBlockCols && mc * kCellBlockCols + m < bxs; ++m)
-            if (lds_flags[m]) mask |= 1ull << m;
-        if (mask) {
-            const uint32_t w = first >> 5, sh = first & 31u;
-            const unsigned long long bits = mask << sh;  // sh + 32 <= 63
-            if (uint32_t(bits)) atomicOr(&occupancy[w], uint32_t(bits));
-            if (uint32_t(bits >> 32)) atomicOr(&occupancy[w + 1], uint32_t(bits >> 32));
-        }
+        const uint32_t first = bxs * ((cj >> kBrickShift) + bys * (ck >> kBrickShift)) + 8u * xb;
+        const unsigned long long bits = (unsigned long long)(*lds_flags) << (first & 31u);
+        if (uint32_t(bits)) atomicOr(&occupancy[first >> 5], uint32_t(bits));
+        if (uint32_t(bits >> 32)) atomicOr(&occupancy[(first >> 5) + 1], uint32_t(bits >> 32));
     }
 }
 

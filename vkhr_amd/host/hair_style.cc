@@ -61,23 +61,37 @@ void DeviceBuffer::upload(const void* host, size_t bytes) {
 
 // ------------------------------------------------------------------ HairStyle
 
-unsigned HairStyle::get_strand_count() const { return static_cast<unsigned>(segments.size()); }
+unsigned HairStyle::get_strand_count() const {
+    return has_segments() ? static_cast<unsigned>(segments.size()) : file_header_.strand_count;
+}
 unsigned HairStyle::get_vertex_count() const { return static_cast<unsigned>(vertices.size()); }
 unsigned HairStyle::get_segment_count() const { return get_vertex_count() - get_strand_count(); }
+
+// First vertex of every strand (+ the end): from the segment array, or the
+// header's default segment count for every strand.
+std::vector<uint32_t> HairStyle::host_strand_offsets() const {
+    const unsigned n = get_strand_count();
+    std::vector<uint32_t> off(size_t(n) + 1, 0);
+    for (unsigned s = 0; s < n; ++s) {
+        const unsigned count = has_segments() ? segments[s] : file_header_.default_segment_count;
+        off[s + 1] = off[s] + count + 1u;
+    }
+    return off;
+}
 
 void HairStyle::generate_thickness(float radius) {
     thickness.clear();
     thickness.reserve(get_vertex_count());
-    for (unsigned short s : segments) {
-        for (unsigned i = 0; i < s; ++i) thickness.push_back(radius);
+    const std::vector<uint32_t> off = host_strand_offsets();
+    for (size_t s = 0; s + 1 < off.size(); ++s) {
+        for (uint32_t i = off[s]; i + 1 < off[s + 1]; ++i) thickness.push_back(radius);
         thickness.push_back(0.0f);  // the tip
     }
     d_thickness_ = DeviceBuffer();
 }
 
 DeviceBuffer HairStyle::strand_offsets() const {
-    std::vector<uint32_t> off(segments.size() + 1, 0);
-    for (size_t s = 0; s < segments.size(); ++s) off[s + 1] = off[s] + segments[s] + 1u;
+    const std::vector<uint32_t> off = host_strand_offsets();
     return DeviceBuffer(off.data(), off.size() * sizeof(uint32_t));
 }
 
@@ -103,14 +117,7 @@ void HairStyle::generate_indices() {
     d_indices_ = std::move(out);
 }
 
-void HairStyle::generate_bounding_box() {
-    vkhr_aabb b;
-    check(vkhr_aabb_compute(device_vertices().as<float>(), get_vertex_count(), &b, stream), "vkhr_aabb_compute");
-    box_ = AABB{{b.origin[0], b.origin[1], b.origin[2]}, b.radius, {b.size[0], b.size[1], b.size[2]}, b.volume};
-    has_box_ = true;
-}
 
-AABB HairStyle::get_bounding_box() const { return box_; }
 
 const DeviceBuffer& HairStyle::device_vertices() const {
     if (!d_vertices_.get() && !vertices.empty()) d_vertices_ = DeviceBuffer(vertices.data(), vertices.size() * sizeof(vec3));
