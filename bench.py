@@ -78,6 +78,73 @@ def time_adjoint(dev, reps=10):
     return e[0].elapsed_time(e[1]) / reps, e[2].elapsed_time(e[3]) / reps
 
 
+def atomic_peak(dev, n_ops=1 << 26):
+    """Device-atomic throughput measured by the build's microbenchmark
+    (libvkhr_microbench.so): uniform-random uint32 adds over 64 MiB (one
+    line request each) and 16-byte pairs like the voxelizer's (two 8-byte
+    adds per line request, random lines); returns requests/s for both."""
+    import ctypes as C
+    import torch
+    lib = C.CDLL(str(ROOT / "vkhr_amd" / "libvkhr_microbench.so"))
+    lib.vkhr_mb_atomics.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p]
+    buf = torch.zeros(16 << 20, dtype=torch.int32, device=dev)
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = {}
+    for name, kind, pattern in (("u32_uniform", 0, 0), ("u32_stream", 0, 3)):
+        lib.vkhr_mb_atomics(kind, pattern, C.c_void_p(buf.data_ptr()), buf.numel(), n_ops, s)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        assert lib.vkhr_mb_atomics(kind, pattern, C.c_void_p(buf.data_ptr()), buf.numel(), n_ops, s) == 0
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = n_ops / (e0.elapsed_time(e1) * 1e-3)
+    return out
+
+
+def time_raster(dev, style, xyz, idx, tan, vol, res, scene, cam, light, bounds, reps=5):
+    """The strand passes at the same workload: 1024^2 shadow depth, strand
+    raster into a 2^25-node PPLL (line-coverage AA, KK, PCF ADSM, LAO) at
+    1920x1080, k=16 resolve; returns (shadow ms, raster ms, resolve ms,
+    fragments)."""
+    import numpy as np
+    import torch
+    from vkhr_amd import params
+    from vkhr_amd import device as D
+    W, H = scene.width, scene.height
+    th = D.to_dev(style.thickness, dev)
+    light_vp = np.asarray(scene.light_vp, np.float64).T.reshape(-1).astype(np.float32)
+    sp = params.strand(bounds, res, strand_radius=1.0)
+    rp = params.render(renderer=0)
+    ppll = D.PPLL(W, H, 1 << 25, dev)
+    smap = torch.empty((1024, 1024), dtype=torch.float32, device=dev)
+    out = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+
+    def shadow():
+        D.shadow_depth(xyz, idx, light_vp, 1.0, (1024, 1024), out=smap)
+
+    def raster():
+        ppll.clear()
+        D.raster_strands(xyz, tan, th, idx, cam, light, sp, rp, vol, res, smap, ppll)
+
+    def resolve():
+        ppll.resolve(out=out)
+
+    shadow()
+    raster()
+    resolve()
+    torch.cuda.synchronize()
+    times = []
+    for fn in (shadow, raster, resolve):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) / reps)
+    return times[0], times[1], times[2], min(ppll.used(), ppll.node_count)
+
+
 def time_adjoint_batch(dev):
     """Batched adjoint: 8 generator styles (128^3 float densities) x 8 ring
     views at 512x512, 256 steps; returns (per-render ms, graph ms) for the
@@ -152,38 +219,76 @@ def embree_present() -> bool:
     return "libembree3" in out
 
 
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores() -> int:
+    """Cores this process may run on (the GPU box grants a share of the host)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def cpu_baselines(style, bounds, res, vol_np, scene_args, covered_pixels, budget):
-    """Oracle legs on the host cores: the strict-IEEE voxelizer (1 thread, as
-    the reference's serial DDA loop) and the float64 raymarch on a bounded
-    sample of covered pixels (1 thread)."""
+    """Oracle legs on the host cores, each for about `budget` seconds: the
+    float64 raymarch transcription on random covered pixels with all cores
+    (OpenMP over pixels, the reported value) and with one; the strict-IEEE
+    voxelizer with one thread (the reference's serial DDA loop) and its
+    OpenMP count-only variant."""
     import numpy as np
     from oracle import raymarch as OR
     from oracle import voxel as OV
+    cores = host_cores()
     idx = style.indices()
-    t0 = time.perf_counter()
-    reps = 0
+    OV.set_threads(1)
+    t0, reps = time.perf_counter(), 0
     while time.perf_counter() - t0 < budget or reps == 0:
         OV.voxelize(style.vertices, bounds, res, indices=idx)
         reps += 1
     seg_rate = reps * (len(idx) // 2) / (time.perf_counter() - t0)
+    OV.set_threads(cores)
+    t0, reps_omp = time.perf_counter(), 0
+    while time.perf_counter() - t0 < budget / 2 or reps_omp == 0:
+        OV.voxelize_counts_omp(style.vertices, bounds, res, idx)
+        reps_omp += 1
+    seg_rate_omp = reps_omp * (len(idx) // 2) / (time.perf_counter() - t0)
     rng = np.random.default_rng(0)
     order = rng.permutation(covered_pixels)
-    done, t0 = 0, time.perf_counter()
     cam, light, sp, rp, w, h = scene_args
-    while time.perf_counter() - t0 < budget or done == 0:
-        chunk = order[done:done + 500]
-        if len(chunk) == 0:
-            break
-        OR.raymarch(vol_np["density"], vol_np["tangents"], res, cam, light, sp, rp, w, h, pixels=chunk)
-        done += len(chunk)
-    ray_rate = done / (time.perf_counter() - t0)
+    rates = {}
+    start = 0
+    for threads, chunk_n in ((1, 500), (cores, 500 * cores)):
+        OV.set_threads(threads)
+        done, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < budget or done == 0:
+            chunk = order[start + done:start + done + chunk_n]
+            if len(chunk) == 0:
+                break
+            OR.raymarch(vol_np["density"], vol_np["tangents"], res, cam, light, sp, rp, w, h, pixels=chunk)
+            done += len(chunk)
+        rates[threads] = (done, done / (time.perf_counter() - t0))
+        start = (start + done) % max(len(order) - chunk_n, 1)
+    OV.set_threads(1)
+    done_all, ray_rate = rates[cores]
+    done_one, ray_rate_one = rates[1]
     return dict(
-        value=ray_rate / 1e6, unit="Mrays/s", cores=1, kind="port",
-        sample=(f"float64 raymarch transcription on {done} random covered pixels of the 1920x1080 frame "
-                f"(512 steps, 256^3), 1 thread; strict-IEEE voxelize_segments restatement: {reps} passes over "
-                f"{len(idx) // 2} segments at {seg_rate:.4g} segments/s, 1 thread; host has {os.cpu_count()} "
-                f"cores; Embree {'present (not timed)' if embree_present() else 'unavailable'}"),
-        segments_per_s=seg_rate)
+        value=ray_rate / 1e6, unit="Mrays/s", cores=cores, kind="port",
+        sample=(f"float64 raymarch transcription (OpenMP over pixels, {cores} threads) on {done_all} random "
+                f"covered pixels of the 1920x1080 frame (512 steps, 256^3); 1 thread: {ray_rate_one / 1e6:.4g} "
+                f"Mrays/s on {done_one} pixels; strict-IEEE voxelize_segments restatement: {reps} passes over "
+                f"{len(idx) // 2} segments, 1 thread (the reference's serial loop); host {cpu_model()}, "
+                f"{cores} cores available; Embree {'present (not timed)' if embree_present() else 'unavailable'}"),
+        raymarch_mrays_per_s_1_thread=ray_rate_one / 1e6,
+        segments_per_s=seg_rate,
+        segments_per_s_openmp_count_only=seg_rate_omp)
 
 
 def main():
@@ -268,6 +373,12 @@ def main():
     ray_bytes = 8 * S * covered + 8 * (S + 8) * frags + 32 * frags + 16 * covered
     achieved = ray_bytes / t_ray / 1e9
     samples = int((sums.view(-1, 2)[:, 1] & 0xFFFFFFFF).sum().item())  # the packed counts
+    # every voxelizer sample is one line request (two 8-byte adds, paired lanes)
+    peak = atomic_peak(dev)
+    voxelize_atomics = {"requests_per_s": samples / t_vox, "peak_uniform_requests_per_s": peak["u32_uniform"],
+                        "frac": samples / t_vox / peak["u32_uniform"],
+                        "peak_lane_contiguous_atomics_per_s": peak["u32_stream"],
+                        "note": "voxelize_ms includes zeroing the 256 MiB accumulator"}
 
     result = {
         "metric": METRIC,
@@ -289,11 +400,16 @@ def main():
         "raymarch_ms": t_ray * 1e3,
         "segments_per_s": n_seg / t_vox,
         "voxelize_ms": t_vox * 1e3,
+        "voxelize_atomics": voxelize_atomics,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "raymarch_kernel", "algorithmic_bytes": ray_bytes},
     }
     if not args.no_extras:
+        t_sh, t_ra, t_re, n_frag = time_raster(dev, style, xyz, idx, tan, vol, res, scene, cam, light, bounds)
+        result["raster"] = {"workload": "1,635,840 segments, 1920x1080, 1024^2 shadow map, k=16, 2^25-node PPLL",
+                            "shadow_ms": t_sh, "raster_ms": t_ra, "resolve_ms": t_re, "fragments": n_frag,
+                            "fragments_per_s": n_frag / (t_ra * 1e-3)}
         t_f, t_b = time_adjoint(dev)
         result["adjoint_fwd_ratio"] = t_b / t_f
         result["adjoint"] = {"workload": "128^3 float32 density, 512x512, 256 steps", "forward_ms": t_f,

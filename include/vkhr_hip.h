@@ -223,6 +223,23 @@ int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tangent_acc,
 /* Words of the brick occupancy bitmask of an nx x ny x nz volume. */
 size_t vkhr_occupancy_words(uint32_t nx, uint32_t ny, uint32_t nz);
 
+/* HairStyle::Volume::downsample (include/vkhr/scene_graph/hair_style.hh:
+ * 228-257) with one of these reductions as its filter functor: voxel (i,j,k)
+ * of out (nx/2 x ny/2 x nz/2) reduces the 2x2x2 neighbourhood (2i+x, 2j+y,
+ * 2k+z) of in (nx x ny x nz, even sizes).  MEAN is the sum / 8 truncated. */
+enum { VKHR_DOWNSAMPLE_MAX = 0, VKHR_DOWNSAMPLE_MIN = 1, VKHR_DOWNSAMPLE_MEAN = 2 };
+int vkhr_volume_downsample(const uint8_t* in, uint32_t nx, uint32_t ny, uint32_t nz, int filter, uint8_t* out,
+                           vkhr_stream stream);
+
+/* filter_volume (share/shaders/volumes/sample_volume.glsl:12-35) evaluated
+ * at every texel centre of a float volume, where its taps fall on texel
+ * centres: out = sum_o w(o) in[v + o] / sum_o w(o) over the kernel_width^3
+ * offsets o (kernel_width odd, zero border), with the shader's weight
+ * exp(-(x^2+y^2+z^2) / 2 * s^2), s = kernel_width / 2 / 2.4 (s^2 multiplies,
+ * as written in the shader). */
+int vkhr_volume_filter(const float* in, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t kernel_width, float* out,
+                       vkhr_stream stream);
+
 /* --------------------------------------------------------------- raymarch */
 
 /* The volumes one hair style is rendered from (device pointers).  Its world

@@ -28,6 +28,8 @@
 #include <cstdint>
 #include <cstring>
 
+#include <omp.h>
+
 #include "../include/vkhr_hip.h"
 
 namespace {
@@ -437,6 +439,11 @@ extern "C" {
 // 2: marched but discarded; surface step (-1: the vec3(0) surface of the shader), entry step (-1 none),
 // found (0/1)}, diag[4] = {accumulated density, ADSM strands, min |acc - iso|
 // over the nonzero samples, |cos(tangent, light)|}.
+// Threads of the OpenMP team used by the pixel loops (the CPU baseline times
+// one and all cores).
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int oracle_max_threads(void) { return omp_get_max_threads(); }
+
 // pixels: list of linear pixel indices to evaluate (n_pixels of them).
 // forced: nullable; per evaluated pixel, {state, surface step, entry step,
 // found} decisions to use instead of the oracle's own (fragment or not, and
@@ -447,8 +454,10 @@ void oracle_raymarch(const uint8_t* density, const int8_t* tangent, const float*
                      const float* depth_buffer, const int64_t* pixels, uint64_t n_pixels, const int32_t* forced,
                      double* rgba, double* depth, int32_t* info, double* diag) {
     Ctx C = make_ctx(density, tangent, density_f32, nx, ny, nz, cam, light, sp, rp, width, height, depth_buffer);
-    PixelOut o;
+    // pixels are independent: the OpenMP team (oracle_set_threads) splits them
+#pragma omp parallel for schedule(dynamic, 16)
     for (uint64_t q = 0; q < n_pixels; ++q) {
+        PixelOut o;
         march_pixel(C, pixels[q], forced ? forced + 4 * q : nullptr, o);
         for (int c = 0; c < 4; ++c) {
             rgba[4 * q + c] = o.rgba[c];

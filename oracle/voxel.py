@@ -89,3 +89,25 @@ def normalize(dens: np.ndarray) -> np.ndarray:
     d = np.ascontiguousarray(dens, np.uint8).copy()
     lib.oracle_normalize(_p(d), C.c_uint64(len(d)))
     return d
+
+
+def voxelize_counts_omp(xyz, bounds: dict, res, indices) -> np.ndarray:
+    """Segment-mode counts only, the DDA split over the OpenMP team (a
+    multi-core timing variant; the same counts as voxelize)."""
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    idx = np.ascontiguousarray(indices, np.uint32)
+    nx, ny, nz = res
+    counts = np.zeros(nx * ny * nz, np.uint32)
+    lib.oracle_voxelize_segments_counts_omp(_p(xyz), _p(idx), C.c_uint64(len(idx)), _p(aabb_array(bounds)),
+                                            C.c_uint32(nx), C.c_uint32(ny), C.c_uint32(nz), _p(counts))
+    return counts
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's OpenMP team (pixel loops, count-only DDA)."""
+    lib.oracle_set_threads(C.c_int(n))
+
+
+def max_threads() -> int:
+    lib.oracle_max_threads.restype = C.c_int
+    return int(lib.oracle_max_threads())

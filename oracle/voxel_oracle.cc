@@ -199,6 +199,35 @@ void oracle_voxelize_segments(const float* xyz, const uint32_t* indices, uint64_
     if (dens_u8) std::memcpy(dens_u8, dens.data(), nv);
 }
 
+// Count-only segment DDA split over the OpenMP team (atomic increments):
+// the same samples as oracle_voxelize_segments, a multi-core CPU timing
+// variant the reference does not have (its loop is serial).
+void oracle_voxelize_segments_counts_omp(const float* xyz, const uint32_t* indices, uint64_t n_indices,
+                                         const float* bounds8, uint32_t w, uint32_t h, uint32_t d,
+                                         uint32_t* counts) {
+    AABB b = from_array(bounds8);
+    vec3 res = {float(w), float(h), float(d)};
+    vec3 voxel_size = b.size / res;
+    vec3 res_m1 = {res.x - 1.0f, res.y - 1.0f, res.z - 1.0f};
+    const uint64_t nv = uint64_t(w) * h * d;
+    std::memset(counts, 0, nv * sizeof(uint32_t));
+#pragma omp parallel for schedule(static, 4096)
+    for (int64_t p = 0; p < int64_t(n_indices / 2); ++p) {
+        const uint64_t i = 2 * uint64_t(p);
+        vec3 root = (at(xyz, indices[i]) - b.origin) / voxel_size;
+        vec3 tip = (at(xyz, indices[i + 1]) - b.origin) / voxel_size;
+        vec3 direction = tip - root;
+        float steps = comp_max(gabs(direction));
+        direction = direction / steps;
+        while (steps-- > 0.0f) {
+            uint64_t pos = voxel_index(root, res_m1, w, h);
+#pragma omp atomic
+            counts[pos] += 1;
+            root += direction;
+        }
+    }
+}
+
 // i8vec3(sum / float(denominator) * 127), w = 0.
 void oracle_quantize_tangents(const float* sums, const uint32_t* denominators, uint64_t n, int8_t* out) {
     for (uint64_t i = 0; i < n; ++i) {

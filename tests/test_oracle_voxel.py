@@ -195,3 +195,14 @@ def test_golden_fixtures_unchanged():
         vtx = OV.voxelize(xyz, b, res)
         assert np.array_equal(seg["counts"], z[f"seg{k}"])
         assert np.array_equal(vtx["counts"], z[f"vtx{k}"])
+
+
+def test_openmp_count_variant_matches():
+    s = synth.config0()
+    b = OV.aabb(s.vertices)
+    idx = s.indices()
+    want = OV.voxelize(s.vertices, b, (64, 64, 64), indices=idx)["counts"]
+    OV.set_threads(4)
+    got = OV.voxelize_counts_omp(s.vertices, b, (64, 64, 64), idx)
+    OV.set_threads(1)
+    assert np.array_equal(got, want)

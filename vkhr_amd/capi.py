@@ -137,6 +137,8 @@ SIGNATURES = {
                                   C.POINTER(C.c_void_p)],
     "vkhr_adjoint_batch_launch": [P, P],
     "vkhr_adjoint_batch_destroy": [P],
+    "vkhr_volume_downsample": [P, U32, U32, U32, I32, P, P],
+    "vkhr_volume_filter": [P, U32, U32, U32, U32, P, P],
 }
 # entry points that do not return a status
 SIZE_FUNCTIONS = {"vkhr_march_record_bytes": [U32, U32], "vkhr_occupancy_words": [U32, U32, U32]}

@@ -97,11 +97,88 @@ normalize_kernel(uint8_t* __restrict__ density, size_t n, const uint32_t* __rest
     }
 }
 
+// Volume::downsample: one thread per output voxel.
+__global__ void __launch_bounds__(256)
+downsample_kernel(const uint8_t* __restrict__ in, uint32_t nx, uint32_t ny, uint32_t ox, uint32_t oy, size_t n_out,
+                  int filter, uint8_t* __restrict__ out) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t o = size_t(blockIdx.x) * blockDim.x + threadIdx.x; o < n_out; o += stride) {
+        const size_t i = o % ox, jk = o / ox, j = jk % oy, k = jk / oy;
+        uint32_t mx = 0u, mn = 255u, sum = 0u;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {  // neighbourhood[x + 2y + 4z]
+            const size_t x = 2 * i + (q & 1), y = 2 * j + ((q >> 1) & 1), z = 2 * k + (q >> 2);
+            const uint32_t v = in[x + nx * (y + size_t(ny) * z)];
+            mx = v > mx ? v : mx;
+            mn = v < mn ? v : mn;
+            sum += v;
+        }
+        out[o] = uint8_t(filter == 0 ? mx : (filter == 1 ? mn : sum / 8u));
+    }
+}
+
+// filter_volume at texel centres: the taps are texels, zero outside.
+__global__ void __launch_bounds__(256)
+filter_kernel(const float* __restrict__ in, int nx, int ny, int nz, int range, float sigma_squared,
+              float* __restrict__ out) {
+    const size_t n = size_t(nx) * ny * nz;
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    for (size_t v = size_t(blockIdx.x) * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const int x0 = int(v % nx), y0 = int((v / nx) % ny), z0 = int(v / (size_t(nx) * ny));
+        float density = 0.0f, total = 0.0f;
+        for (int z = -range; z <= range; ++z)
+            for (int y = -range; y <= range; ++y)
+                for (int x = -range; x <= range; ++x) {
+                    const float r2 = float(x * x + y * y + z * z);
+                    const float exponent = -1.0f * r2 / 2.0f * sigma_squared;
+                    const float w = 1.0f / (2.0f * 3.14159265358979f * sigma_squared) * expf(exponent);
+                    const int tx = x0 + x, ty = y0 + y, tz = z0 + z;
+                    const bool inside = tx >= 0 && ty >= 0 && tz >= 0 && tx < nx && ty < ny && tz < nz;
+                    if (inside) density += in[size_t(tx) + size_t(nx) * (size_t(ty) + size_t(ny) * size_t(tz))] * w;
+                    total += w;
+                }
+        out[v] = density / total;
+    }
+}
+
 }  // namespace
 
 }  // namespace vkhr
 
 using namespace vkhr;
+
+extern "C" int vkhr_volume_downsample(const uint8_t* in, uint32_t nx, uint32_t ny, uint32_t nz, int filter,
+                                      uint8_t* out, vkhr_stream stream) {
+    clear_error();
+    int rc = ensure_device();
+    if (rc) return rc;
+    VKHR_REQUIRE(in && out, VKHR_ERR_INVALID, "in/out is null");
+    VKHR_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2 && nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0, VKHR_ERR_INVALID,
+                 "downsample needs even sizes, got %ux%ux%u", nx, ny, nz);
+    VKHR_REQUIRE(filter >= 0 && filter <= 2, VKHR_ERR_INVALID, "unknown filter %d", filter);
+    const size_t n_out = size_t(nx / 2) * (ny / 2) * (nz / 2);
+    downsample_kernel<<<grid_for(n_out, 256, 65536), 256, 0, as_stream(stream)>>>(in, nx, ny, nx / 2, ny / 2, n_out,
+                                                                                 filter, out);
+    VKHR_CHECK_LAUNCH();
+    return VKHR_OK;
+}
+
+extern "C" int vkhr_volume_filter(const float* in, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t kernel_width,
+                                  float* out, vkhr_stream stream) {
+    clear_error();
+    int rc = ensure_device();
+    if (rc) return rc;
+    VKHR_REQUIRE(in && out && in != out, VKHR_ERR_INVALID, "in/out is null or the same");
+    VKHR_REQUIRE(nx && ny && nz && uint64_t(nx) * ny * nz <= (uint64_t(1) << 31), VKHR_ERR_INVALID, "bad volume");
+    VKHR_REQUIRE(kernel_width % 2 == 1 && kernel_width <= 15, VKHR_ERR_INVALID,
+                 "kernel_width %u: odd and at most 15", kernel_width);
+    const float sigma = (float(kernel_width) / 2.0f) / 2.4f;
+    const size_t n = size_t(nx) * ny * nz;
+    filter_kernel<<<grid_for(n, 256, 65536), 256, 0, as_stream(stream)>>>(in, int(nx), int(ny), int(nz),
+                                                                         int(kernel_width / 2), sigma * sigma, out);
+    VKHR_CHECK_LAUNCH();
+    return VKHR_OK;
+}
 
 extern "C" int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tangent_acc, uint32_t nx, uint32_t ny,
                                     uint32_t nz, int normalize, uint8_t* density_u8, int8_t* tangent_i8x4,
