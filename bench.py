@@ -41,7 +41,24 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=8.0, help="seconds per CPU-baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="skip the adjoint and batched legs")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the 256 MiB copy kernel three times (the reference for the rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE traffic counters) and exit")
     return ap.parse_args()
+
+
+def copy_calibration():
+    import ctypes as C
+    import torch
+    lib = C.CDLL(str(ROOT / "vkhr_amd" / "libvkhr_microbench.so"))
+    lib.vkhr_mb_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
+    src = torch.ones(256 << 20, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for _ in range(3):
+        assert lib.vkhr_mb_copy(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), src.numel(), s) == 0
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": "3 x vkhr_mb_copy of 268435456 bytes"}))
 
 
 def time_adjoint(dev, reps=10):
@@ -183,7 +200,8 @@ def measured_traffic():
 
 def time_batch(dev, reps=3):
     """Batched form: 32 styles of uneven size at 960x540; returns (per-style
-    launches ms, batched graph ms, covered rays per batch)."""
+    launches back to back ms, batched graph ms, sum over styles of one
+    style's launches waited for, ms)."""
     import torch
     from vkhr_amd import synth
     from vkhr_amd.batch import Batch, StyleBuffers, load_tuning
@@ -208,7 +226,14 @@ def time_batch(dev, reps=3):
         batch.launch(wait=False)
     e[3].record(batch.stream)
     torch.cuda.synchronize()
-    return e[0].elapsed_time(e[1]) / reps, e[2].elapsed_time(e[3]) / reps
+    # each style run on its own and waited for (wall clock, host issue included)
+    synced = 0.0
+    for b in bufs:
+        t0 = time.perf_counter()
+        b.run_plain()
+        torch.cuda.synchronize()
+        synced += time.perf_counter() - t0
+    return e[0].elapsed_time(e[1]) / reps, e[2].elapsed_time(e[3]) / reps, synced * 1e3
 
 
 def embree_present() -> bool:
@@ -293,6 +318,9 @@ def cpu_baselines(style, bounds, res, vol_np, scene_args, covered_pixels, budget
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this path runs on one GPU; --gpus must be 1 (the multi-GPU sharding is a "
                                    "design, not built)"}))
@@ -414,9 +442,12 @@ def main():
         result["adjoint_fwd_ratio"] = t_b / t_f
         result["adjoint"] = {"workload": "128^3 float32 density, 512x512, 256 steps", "forward_ms": t_f,
                              "adjoint_ms": t_b}
-        t_plain, t_batch = time_batch(dev)
+        t_plain, t_batch, t_synced = time_batch(dev)
         result["batched"] = {"workload": "32 styles, 5k-140k strands, 64^3/128^3/256^3, 960x540, 512 steps",
-                             "per_style_ms": t_plain, "batched_graph_ms": t_batch, "speedup": t_plain / t_batch}
+                             "per_style_ms": t_plain, "batched_graph_ms": t_batch, "speedup": t_plain / t_batch,
+                             "per_style_synced_ms": t_synced, "speedup_vs_synced": t_synced / t_batch,
+                             "note": "per_style_ms: the 32 per-style sequences issued back to back; "
+                                     "per_style_synced_ms: the sum of each style's sequence timed alone"}
         t_plain, t_batch = time_adjoint_batch(dev)
         result["batched_adjoint"] = {"workload": "8 styles x 8 views, 128^3 float32, 512x512, 256 steps, "
                                                  "forward + L2 loss gradient + adjoint",

@@ -1,5 +1,6 @@
 // Device tables of the batched launch: many hair styles of uneven size,
-// voxelized, finalized and raymarched by one graph of persistent kernels.
+// voxelized, finalized and raymarched by one graph of grid-stride kernels
+// over chunk lists (and a raymarch over (style, tile) work items).
 // The per-segment, per-voxel and per-pixel bodies are the per-style ones
 // (voxel.hpp, volume.hpp, march.hpp), so the outputs are bit-identical to
 // per-style launches.
@@ -24,7 +25,7 @@ struct BatchVolStyle {
     uint32_t* occupancy;                        // nullable
 };
 
-// A contiguous range of one style's voxels (or one cell level of its cells).
+// A contiguous range of one style's voxels (or one cells_block of its cells).
 struct BatchChunk {
     uint32_t style;
     uint32_t reserved;

@@ -1,7 +1,8 @@
-// Batched raymarch (persistent blocks pulling (style, tile) work items from
-// an atomic queue) and the host side of the batched launch: device tables,
-// one hipGraph capturing clear -> voxelize -> finalize -> cells (normalized)
-// -> raymarch, replayed by vkhr_batch_launch.
+// Batched raymarch over (style, tile) work items -- one block per item, or
+// persistent blocks pulling items from an atomic queue -- and the host side
+// of the batched launch: device tables, one hipGraph capturing clear ->
+// voxelize -> finalize -> normalize -> cells -> raymarch, replayed by
+// vkhr_batch_launch.
 #include "batch.hpp"
 #include "volume.hpp"
 #include "voxel.hpp"
@@ -163,7 +164,8 @@ extern "C" int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styl
         pair_prefix[s + 1] = pair_prefix[s] + v.n_pairs;
         uint64_t nvox = uint64_t(in.volume) * in.volume * in.volume;
         for (uint64_t a = 0; a < nvox; a += kChunk) vchunks.push_back({s, 0, a, std::min(nvox, a + kChunk)});
-        for (uint64_t k = 0; k <= in.volume; ++k) cchunks.push_back({s, 0, k, k + 1});  // cell levels
+        const uint64_t cb = cell_blocks(in.volume, in.volume, in.volume);
+        for (uint64_t k = 0; k < cb; ++k) cchunks.push_back({s, 0, k, k + 1});
         vkhr_volume vol;
         std::memset(&vol, 0, sizeof(vol));
         vol.density = in.density;

@@ -102,20 +102,13 @@ batch_finalize_kernel(BatchDevice d) {
 }
 
 // Corner-packed cells of the normalized density and the brick occupancy:
-// a cell chunk is one cell level (a slab of (n+1)^2 cells) of one style,
-// done in tiles of 64 x 4 cells.
+// a cell chunk is one cells_block of one style.
 __global__ void __launch_bounds__(256)
 batch_cells_kernel(BatchDevice d) {
-    __shared__ uint32_t flags;
     for (uint32_t c = blockIdx.x; c < d.n_cell_chunks; c += gridDim.x) {
         const BatchChunk ch = d.cell_chunks[c];
         const BatchVolStyle& st = d.styles[ch.style];
-        const uint32_t xbs = (st.nx + 1u + kWave - 1u) / kWave, ybs = (st.ny + 1u + kTileRows - 1u) / kTileRows;
-        for (uint32_t t = 0; t < xbs * ybs; ++t) {
-            cells_tile(st.density, st.nx, st.ny, st.nz, t % xbs, t / xbs, uint32_t(ch.begin), st.cells, st.occupancy,
-                       &flags);
-            __syncthreads();
-        }
+        cells_block(st.density, st.nx, st.ny, st.nz, uint32_t(ch.begin), st.cells, st.occupancy);
     }
 }
 

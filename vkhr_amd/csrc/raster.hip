@@ -201,12 +201,22 @@ raster_kernel(RasterArgs a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) { pa[c] = q0[c]; pb[c] = q1[c]; }
     }
-    for (int k = 0;; ++k) {
-        const bool live = k < total;
-        if (!__any(live)) break;
+    // Each pass of this loop gives every lane its next covered candidate (the
+    // uncovered ones are skipped inside the lane's own walk), so the shading
+    // and the PPLL insert below run with the wave's lanes busy.
+    int k = 0;
+    for (;;) {
         int px = 0, py = 0;
         double t = 0.0;
-        bool frag = live && candidate(L, k, a.width, a.height, px, py) && covers(L, px, py, a.line_width, t);
+        bool frag = false;
+        while (k < total) {
+            const int kk = k++;
+            if (candidate(L, kk, a.width, a.height, px, py) && covers(L, px, py, a.line_width, t)) {
+                frag = true;
+                break;
+            }
+        }
+        if (!__any(frag)) break;  // every lane has walked all its candidates
         const uint32_t pix = uint32_t(py) * uint32_t(a.width) + uint32_t(px);
         const double z = L.az + t * (L.bz - L.az);
         if (!kColor) {
@@ -243,7 +253,7 @@ raster_kernel(RasterArgs a) {
                 normalize3(e);
                 normalize3(l);
                 double shading[3] = {1.0, 1.0, 1.0};
-                if (a.shading_model == 0) kajiya_kay(a.hair_color, a.intensity, a.shininess, T, l, e, shading);
+                if (a.shading_model == 0) kajiya_kay<true>(a.hair_color, a.intensity, a.shininess, T, l, e, shading);
                 double occlusion = 1.0;
                 if (a.adsm_on == 1 && a.shading_model != 3) {
                     double ls[4];
@@ -256,7 +266,7 @@ raster_kernel(RasterArgs a) {
                         double depth = fmax(light_depth - sd, 0.0);
                         double count = depth * 15000.0;
                         if (depth > 1e-5) count += 1.0;
-                        vis += exp(count * a.ln_transmit) * a.pcf_w[q];
+                        vis += double(expf(float(count * a.ln_transmit))) * a.pcf_w[q];
                         tot += a.pcf_w[q];
                     }
                     occlusion *= vis / tot;
@@ -265,7 +275,7 @@ raster_kernel(RasterArgs a) {
                     double st[3];
 #pragma unroll
                     for (int c = 0; c < 3; ++c) st[c] = (P[c] - a.vorigin[c]) / a.vsize[c] * a.vres[c] - 0.5;
-                    occlusion *= local_ambient_occlusion(a.vol, st, a.ao_scaling, a.ao_clamp, a.ao_exponent);
+                    occlusion *= local_ambient_occlusion_f32pow(a.vol, st, a.ao_scaling, a.ao_clamp, a.ao_exponent);
                 }
                 color = make_float4(float(shading[0] * occlusion), float(shading[1] * occlusion),
                                     float(shading[2] * occlusion), float(coverage));

@@ -148,7 +148,26 @@ __device__ __forceinline__ double local_ambient_occlusion(const DensityVolume& v
     return pow(1.0 - dens / 8.0, exponent);
 }
 
+// The strand rasterizer's per-fragment terms with float32 transcendentals
+// (one power each; the float64 positions and sums stay).  Their relative
+// error, ~1e-7, is far inside the fragment tolerance; the raymarcher keeps
+// the float64 forms below.
+__device__ __forceinline__ double local_ambient_occlusion_f32pow(const DensityVolume& v, const double s[3],
+                                                                 double scale, double clamp_v, double exponent) {
+    double dens = 0.0;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        double ox = ((n & 1) ? 0.5 : -0.5) * scale;
+        double oy = (((n >> 1) & 1) ? 0.5 : -0.5) * scale;
+        double oz = ((n >> 2) ? 0.5 : -0.5) * scale;
+        dens += fmin(double(density_at(v, s[0] + ox, s[1] + oy, s[2] + oz)), clamp_v);
+    }
+    return double(powf(float(1.0 - dens / 8.0), float(exponent)));
+}
+
 // Kajiya-Kay (kajiya-kay.glsl:7-24); sqrt of a negative rounding residue is 0.
+// kF32Pow: the specular power in float32 (the strand rasterizer).
+template <bool kF32Pow = false>
 __device__ __forceinline__ void kajiya_kay(const double diffuse[3], const double specular[3], double p,
                                            const double t[3], const double l[3], const double e[3], double out[3],
                                            double* base_out = nullptr) {
@@ -157,7 +176,7 @@ __device__ __forceinline__ void kajiya_kay(const double diffuse[3], const double
     double stl = sqrt(fmax(1.0 - ctl * ctl, 0.0)), ste = sqrt(fmax(1.0 - cte * cte, 0.0));
     double base = fmax(ctl * cte + stl * ste, 0.0);
     if (base_out) *base_out = base;
-    double spec = pow(base, p);
+    double spec = kF32Pow ? double(powf(float(base), float(p))) : pow(base, p);
 #pragma unroll
     for (int c = 0; c < 3; ++c) out[c] = diffuse[c] * stl + specular[c] * spec;
 }

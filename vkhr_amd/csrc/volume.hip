@@ -8,8 +8,9 @@
 //                     block).
 // Pass 2 (per voxel): HairStyle::Volume::normalize in place and the float
 //                     view.
-// Pass 3 (per cell, tiles of 64 x 4): corner packing, 8 texels -> one
-//                     uint64 per cell, and the brick occupancy.
+// Pass 3 (per cell; blocks of 64 x 8 cells walking 32 cell levels up):
+//                     corner packing, 8 texels -> one uint64 per cell, and
+//                     the brick occupancy.
 // Compiled with -ffp-contract=off (quantization and normalize are float
 // expressions whose rounding must match the reference's).
 #include "volume.hpp"
@@ -55,20 +56,14 @@ finalize_counts_kernel(const uint32_t* __restrict__ counts, const long long* __r
     }
 }
 
-// One block per tile of 64 x 4 cells at one cell level (CLAMP_TO_BORDER
-// with an opaque-black border: texels outside the volume are 0), with the
-// brick occupancy.
+// One block per 64 cell columns x 8 cell rows x kCellSlab cell levels
+// (CLAMP_TO_BORDER with an opaque-black border: texels outside the volume
+// are 0), with the brick occupancy.
 __global__ void __launch_bounds__(256)
 cells_kernel(const uint8_t* __restrict__ density, uint32_t nx, uint32_t ny, uint32_t nz,
              unsigned long long* __restrict__ cells, uint32_t* __restrict__ occupancy) {
-    __shared__ uint32_t flags;
-    const uint32_t xbs = (nx + 1u + kWave - 1u) / kWave, ybs = (ny + 1u + kTileRows - 1u) / kTileRows;
-    const uint32_t n_tiles = xbs * ybs * (nz + 1u);
-    for (uint32_t b = blockIdx.x; b < n_tiles; b += gridDim.x) {
-        const uint32_t xb = b % xbs, rest = b / xbs;
-        cells_tile(density, nx, ny, nz, xb, rest % ybs, rest / ybs, cells, occupancy, &flags);
-        __syncthreads();
-    }
+    const uint32_t n_blocks = cell_blocks(nx, ny, nz);
+    for (uint32_t b = blockIdx.x; b < n_blocks; b += gridDim.x) cells_block(density, nx, ny, nz, b, cells, occupancy);
 }
 
 // Four voxels per thread (uchar4), the tail of n % 4 by the first threads.
@@ -218,8 +213,7 @@ extern "C" int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tange
     }
     if (cells) {
         if (occupancy && (rc = fill_u32(occupancy, occupancy_words(nx, ny, nz), 0u, s))) return rc;
-        const uint32_t tiles = ((nx + 1 + kWave - 1) / kWave) * ((ny + 1 + kTileRows - 1) / kTileRows) * (nz + 1);
-        cells_kernel<<<grid_for(tiles, 1, 1u << 20), 256, 0, s>>>(
+        cells_kernel<<<grid_for(cell_blocks(nx, ny, nz), 1, 1u << 20), 256, 0, s>>>(
             density_u8, nx, ny, nz, reinterpret_cast<unsigned long long*>(cells), occupancy);
         VKHR_CHECK_LAUNCH();
     }

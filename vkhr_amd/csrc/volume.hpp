@@ -123,56 +123,79 @@ __device__ __forceinline__ uint32_t texel_density(uint32_t d, const DensityMap& 
     return m.normalize ? normalize_value(d, m.lo, m.scaling, m.flat) : d;
 }
 
-// One tile of the corner-packed layout -- cells (i, j, k), i in [-1, nx-1],
+// One block of the corner-packed layout -- cells (i, j, k), i in [-1, nx-1],
 // index (i+1) + (nx+1) * ((j+1) + (ny+1) * (k+1)), each the 8 texels of the
 // trilinear footprint whose lower corner is (i,j,k) as one uint64 (byte b =
 // texel (i+(b&1), j+(b>>1&1), k+(b>>2)), 0 outside the volume) -- from the
-// final uint8 density: 64 cell columns i+1 in [64 xb, 64 xb + 64) (the lanes
-// of a wave) x 4 cell rows j+1 in [4 yb, 4 yb + 4) (the waves of a block) at
-// cell level k+1 = ck.  Lanes read consecutive texels and store consecutive
-// cells (coalesced).  The bricks (8x8x8 cells) holding a nonzero cell are
-// or-ed into occupancy (nullable; one word atomic per tile with any).  All
-// threads of the block call it.
-constexpr uint32_t kTileRows = 4;   // cell rows per tile (waves of a 256-thread block)
+// final uint8 density: cell columns i+1 in [64 xb, 64 xb + 64) (the lanes of
+// a wave), cell rows j+1 in [8 by, 8 by + 8) (two per wave), cell levels
+// k+1 in [kCellSlab zc, kCellSlab zc + kCellSlab) walked upwards, so each
+// level loads only its own texel slice and reuses the one below.  Lanes
+// read consecutive texels and store consecutive cells.  The bricks (8x8x8
+// cells) holding a nonzero cell are or-ed into occupancy (nullable; one word
+// atomic per wave and brick level).  All threads of the block call it.
+constexpr uint32_t kCellSlab = 32;   // cell levels per block
 
-__device__ __forceinline__ void cells_tile(const uint8_t* __restrict__ density, uint32_t nx, uint32_t ny,
-                                           uint32_t nz, uint32_t xb, uint32_t yb, uint32_t ck,
-                                           unsigned long long* __restrict__ cells, uint32_t* occupancy,
-                                           uint32_t* lds_flags) {
+__host__ __device__ __forceinline__ uint32_t cell_blocks(uint32_t nx, uint32_t ny, uint32_t nz) {
+    return ((nx + 1u + 63u) / 64u) * bricks_along(ny) * ((nz + 1u + kCellSlab - 1u) / kCellSlab);
+}
+
+__device__ __forceinline__ void cells_block(const uint8_t* __restrict__ density, uint32_t nx, uint32_t ny,
+                                            uint32_t nz, uint32_t block, unsigned long long* __restrict__ cells,
+                                            uint32_t* occupancy) {
     const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-    const uint32_t cxi = xb * kWave + lane;        // cell column i + 1
-    const uint32_t cj = yb * kTileRows + wave;     // cell row j + 1
-    unsigned long long packed = 0ull;
-    if (cxi <= nx && cj <= ny) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {  // texel rows (j + (q&1), k + (q>>1))
-            const int y = int(cj) - 1 + (q & 1), z = int(ck) - 1 + (q >> 1);
-            if (y < 0 || z < 0 || y >= int(ny) || z >= int(nz)) continue;
-            const uint8_t* src = density + uint32_t(y) * nx + uint32_t(z) * nx * ny;
-            const int x = int(cxi) - 1;
-            const uint32_t t0 = x >= 0 ? src[x] : 0u;
-            const uint32_t t1 = cxi < nx ? src[cxi] : 0u;
-            packed |= (unsigned long long)(t0 | (t1 << 8)) << (16 * q);
-        }
-        cells[size_t(cj + (ny + 1u) * ck) * (nx + 1u) + cxi] = packed;
+    const uint32_t xbs = (nx + 1u + 63u) / 64u, bys = bricks_along(ny);
+    const uint32_t xb = block % xbs, by = (block / xbs) % bys, zc = block / (xbs * bys);
+    const uint32_t cxi = xb * 64u + lane;                        // cell column i + 1
+    const bool col = cxi <= nx;
+    const uint32_t k0 = zc * kCellSlab, k1 = min(k0 + kCellSlab, nz + 1u);
+    // the two cell rows of this wave and their texel rows j, j+1
+    uint32_t cj[2];
+    bool row[2];
+    int ty[2][2];
+    for (int r = 0; r < 2; ++r) {
+        cj[r] = (by << kBrickShift) + wave + 4u * uint32_t(r);
+        row[r] = cj[r] <= ny;
+        ty[r][0] = int(cj[r]) - 1;
+        ty[r][1] = int(cj[r]);
     }
-    if (!occupancy) return;
-    // lanes 8g..8g+7 of a wave are the cells of brick 8 xb + g
-    const unsigned long long nonzero = __ballot(packed != 0ull);
-    uint32_t m = 0;
+    // texels (x-1, x) of texel row y in slice z, packed as 2 bytes (0 outside)
+    auto pair = [&](int y, int z) -> uint32_t {
+        if (!col || y < 0 || z < 0 || y >= int(ny) || z >= int(nz)) return 0u;
+        const uint8_t* src = density + uint32_t(y) * nx + uint32_t(z) * nx * ny;
+        const uint32_t lo = cxi >= 1u ? src[cxi - 1u] : 0u;
+        const uint32_t hi = cxi < nx ? src[cxi] : 0u;
+        return lo | (hi << 8);
+    };
+    uint32_t below[2];  // slice k (texel z = k) of each row: rows j, j+1 -> 4 bytes
+    for (int r = 0; r < 2; ++r)
+        below[r] = pair(ty[r][0], int(k0) - 1) | (pair(ty[r][1], int(k0) - 1) << 16);
+    uint32_t any = 0u;  // a nonzero cell in this brick level (per lane)
+    for (uint32_t ck = k0; ck < k1; ++ck) {
 #pragma unroll
-    for (int g = 0; g < 8; ++g) m |= ((nonzero >> (8 * g)) & 0xffull) ? (1u << g) : 0u;
-    if (threadIdx.x == 0) *lds_flags = 0u;
-    __syncthreads();
-    if (lane == 0 && m) atomicOr(lds_flags, m);
-    __syncthreads();
-    if (threadIdx.x == 0 && *lds_flags) {
-        // bricks 8 xb .. 8 xb + 7 of row (by, bz): bits first .. first + 7
-        const uint32_t bxs = bricks_along(nx), bys = bricks_along(ny);
-        const uint32_t first = bxs * ((cj >> kBrickShift) + bys * (ck >> kBrickShift)) + 8u * xb;
-        const unsigned long long bits = (unsigned long long)(*lds_flags) << (first & 31u);
-        if (uint32_t(bits)) atomicOr(&occupancy[first >> 5], uint32_t(bits));
-        if (uint32_t(bits >> 32)) atomicOr(&occupancy[(first >> 5) + 1], uint32_t(bits >> 32));
+        for (int r = 0; r < 2; ++r) {
+            const uint32_t above = pair(ty[r][0], int(ck)) | (pair(ty[r][1], int(ck)) << 16);
+            const unsigned long long packed = (unsigned long long)(below[r]) | ((unsigned long long)(above) << 32);
+            if (col && row[r]) {
+                cells[size_t(cj[r] + (ny + 1u) * ck) * (nx + 1u) + cxi] = packed;
+                any |= packed != 0ull ? 1u : 0u;
+            }
+            below[r] = above;
+        }
+        if (occupancy && ((ck & 7u) == 7u || ck + 1u == k1)) {  // wave-uniform: the end of a brick level
+            const unsigned long long nz_lanes = __ballot(any != 0u);
+            uint32_t m = 0u;
+#pragma unroll
+            for (int g = 0; g < 8; ++g) m |= ((nz_lanes >> (8 * g)) & 0xffull) ? (1u << g) : 0u;
+            if (lane == 0 && m) {
+                // bricks 8 xb .. 8 xb + 7 of brick row (by, ck / 8): bits first .. first + 7
+                const uint32_t first = bricks_along(nx) * (by + bys * (ck >> kBrickShift)) + 8u * xb;
+                const unsigned long long bits = (unsigned long long)(m) << (first & 31u);
+                if (uint32_t(bits)) atomicOr(&occupancy[first >> 5], uint32_t(bits));
+                if (uint32_t(bits >> 32)) atomicOr(&occupancy[(first >> 5) + 1], uint32_t(bits >> 32));
+            }
+            any = 0u;
+        }
     }
 }
 
