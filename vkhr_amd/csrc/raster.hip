@@ -2,15 +2,21 @@
 // strand colour pass into the PPLL) as gfx950 kernels.
 //
 // The reference draws LINE_LIST primitives with a dynamic line width and
-// lets the Vulkan rasterizer pick the fragments.  Here one thread owns one
+// lets the Vulkan rasterizer pick the fragments.  Here one lane owns one
 // index pair: it clips the segment to 0 <= z <= w, maps it to window
-// coordinates, and walks candidate pixels along its major axis -- every
-// column the segment's rectangle touches, a few rows around the line --
-// testing each candidate with the exact rule
+// coordinates, and walks the pixels of its major axis column by column;
+// in each column only the rows of the line's strip (plus a rounding margin)
+// are tested, with the exact rule
 //     0 <= (c-a).(b-a) < |b-a|^2   and   cross(b-a, c-a)^2 <= (w/2)^2 |b-a|^2
-// in float64 (declared in include/vkhr_hip.h).  Candidates are enumerated by
-// index so that all lanes of a wave stay in one loop and can ballot together
-// for the PPLL node allocation (one counter atomic per wave and round).
+// in float64 (declared in include/vkhr_hip.h).
+//
+// Colour pass: segments give from zero to dozens of fragments, so shading
+// one fragment per lane per round would leave most lanes of a wave idle.
+// Instead the lanes push their covered fragments into a per-wave LDS queue
+// (ballot + rank), and the float64 shading runs only on full waves of 64
+// queued fragments, each lane shading a fragment of whichever segment it
+// came from (the segments' shading data sit in LDS).  PPLL nodes are
+// allocated with one counter atomic per wave and round.
 // Window depth interpolates linearly in t, varyings perspective-correctly.
 // Compiled with -ffp-contract=off so the rule evaluates as the float64
 // oracle evaluates it.
@@ -29,11 +35,18 @@ struct LineSetup {
     double wa, wb;                   // clip w of the clipped ends
     double s0, s1;                   // clip interval along the original segment
     double vx, vy, l2;               // b - a, |b - a|^2
+    double lim;                      // (w/2)^2 |b - a|^2
     double h;                        // half-height of the strip across the minor axis
-    int col0, ncols, nrows;          // candidate walk
+    double slope;                    // d(minor) / d(major)
+    double margin;                   // rounding margin of a column's row range (pixels)
+    int col0, col_end;               // columns [col0, col_end) of the major axis
     bool xmajor, valid;
 };
 
+// Walk state of one lane: the current column and its remaining rows.
+struct Walk {
+    int col, row, row_end;
+};
 struct RasterArgs {
     double vp[16];                   // view-projection (column-major)
     double lm[16];                   // light->matrix (biased light view-projection)
@@ -104,6 +117,7 @@ __device__ LineSetup setup_line(const double* m, const double pa[3], const doubl
     if (!(L.l2 > 0.0) || !(L.l2 < 1e30)) return L;
     L.xmajor = fabs(L.vx) >= fabs(L.vy);
     double hw = 0.5 * line_width;
+    L.lim = hw * hw * L.l2;
     double major_a = L.xmajor ? L.ax : L.ay, major_b = L.xmajor ? L.bx : L.by;
     double vmaj = L.xmajor ? L.vx : L.vy;
     int extent = L.xmajor ? width : height;
@@ -113,38 +127,63 @@ __device__ LineSetup setup_line(const double* m, const double pa[3], const doubl
     c_hi = fmin(c_hi, double(extent - 1));
     if (c_lo > c_hi) return L;
     L.col0 = int(c_lo);
-    L.ncols = int(c_hi - c_lo) + 1;
-    // rows: the strip |minor - line(major)| <= hw * |v| / |v_major|, plus a margin
+    L.col_end = int(c_hi) + 1;
+    // The strip |minor - line(major)| <= h holds exactly the pixels with
+    // cross^2 <= (w/2)^2 |v|^2 (the cross product is |v_major| times the
+    // minor-axis distance).  Both that rule and the row range below round in
+    // float64 with errors of a few ulps of the largest window coordinate;
+    // the margin covers them many times over.
     L.h = hw * sqrt(L.l2) / fabs(vmaj);
-    L.nrows = int(floor(2.0 * L.h)) + 3;
+    L.slope = L.xmajor ? L.vy / L.vx : L.vx / L.vy;
+    double mag = fmax(fmax(fabs(L.ax), fabs(L.ay)), fmax(fmax(fabs(L.bx), fabs(L.by)), double(width + height)));
+    L.margin = 1e-6 + 1e-10 * mag;
     L.valid = true;
     return L;
 }
 
-// Candidate k of a segment -> pixel (px, py); false when outside the target.
-__device__ __forceinline__ bool candidate(const LineSetup& L, int k, int width, int height, int& px, int& py) {
-    int col = L.col0 + k / L.nrows;
-    int r = k % L.nrows;
-    double cm = col + 0.5;
-    double line_minor;
-    if (L.xmajor) line_minor = L.ay + (cm - L.ax) * (L.vy / L.vx);
-    else line_minor = L.ax + (cm - L.ay) * (L.vx / L.vy);
-    int row = int(floor(line_minor - L.h - 0.5)) - 1 + r;
-    if (L.xmajor) { px = col; py = row; }
-    else { px = row; py = col; }
-    return px >= 0 && py >= 0 && px < width && py < height;
+__device__ __forceinline__ Walk walk_start(const LineSetup& L) {
+    return Walk{L.valid ? L.col0 - 1 : 0, 1, 0};
 }
 
 // The line rule at pixel centre (px+0.5, py+0.5); t returned on success.
-__device__ __forceinline__ bool covers(const LineSetup& L, int px, int py, double line_width, double& t) {
+__device__ __forceinline__ bool covers(const LineSetup& L, int px, int py, double& t) {
     double cx = px + 0.5 - L.ax, cy = py + 0.5 - L.ay;
     double dp = cx * L.vx + cy * L.vy;
     if (!(dp >= 0.0 && dp < L.l2)) return false;
     double cr = L.vx * cy - L.vy * cx;
-    double hw = 0.5 * line_width;
-    if (!(cr * cr <= hw * hw * L.l2)) return false;
+    if (!(cr * cr <= L.lim)) return false;
     t = dp / L.l2;
     return true;
+}
+
+// Advances a lane's walk to its next covered pixel; false once the walk has
+// passed its last column (and on every later call).
+__device__ __forceinline__ bool next_fragment(const LineSetup& L, Walk& w, int width, int height, int& px, int& py,
+                                              double& t) {
+    if (!L.valid) return false;
+    const int minor_extent = L.xmajor ? height : width;
+    for (;;) {
+        if (w.row > w.row_end) {
+            if (w.col + 1 >= L.col_end) {
+                w.col = L.col_end;
+                return false;
+            }
+            ++w.col;
+            const double cm = w.col + 0.5;
+            const double line_minor = L.xmajor ? L.ay + (cm - L.ax) * L.slope : L.ax + (cm - L.ay) * L.slope;
+            // rows whose centre (row + 0.5) lies within h (+ margin) of the line
+            double r0 = ceil(line_minor - L.h - 0.5 - L.margin), r1 = floor(line_minor + L.h - 0.5 + L.margin);
+            r0 = fmax(r0, 0.0);
+            r1 = fmin(r1, double(minor_extent - 1));
+            if (r0 > r1) continue;
+            w.row = int(r0);
+            w.row_end = int(r1);
+        }
+        const int row = w.row++;
+        px = L.xmajor ? w.col : row;
+        py = L.xmajor ? row : w.col;
+        if (covers(L, px, py, t)) return true;
+    }
 }
 
 // Bilinear, clamp-to-edge sample of the shadow map at uv.
@@ -163,9 +202,52 @@ __device__ __forceinline__ double shadow_tex(const RasterArgs& a, double u, doub
     return r0 + wy * (r1 - r0);
 }
 
-template <bool kColor>
-__global__ void __launch_bounds__(256)
+// Per-wave LDS of the colour pass: the shading data of the wave's 64
+// segments (structure of arrays, so lanes reading different segments hit
+// different banks) and a ring of queued fragments.
+constexpr int kQueue = 128;          // >= 2 x 64: a fill round never overflows
+enum SegField { kPa = 0, kPb = 3, kTa = 6, kTb = 9, kTha = 12, kThb, kWa, kWb, kAz, kBz, kSegFields };
+struct WaveShared {
+    double seg[kSegFields][kWave];
+    uint32_t segment[kWave];
+    double qt[kQueue];
+    uint32_t qpix[kQueue];
+    uint32_t qlane[kQueue];
+};
+constexpr int kRasterBlock = 256;
+
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__global__ void __launch_bounds__(kRasterBlock)
+shadow_raster_kernel(RasterArgs a) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    LineSetup L;
+    L.valid = false;
+    if (s < a.n_pairs) {
+        const uint32_t ia = a.indices[2u * s], ib = a.indices[2u * s + 1u];
+        if (ia < a.n_vertices && ib < a.n_vertices) {
+            float3 fa = load_xyz(a.xyz, ia), fb = load_xyz(a.xyz, ib);
+            double pa[3] = {fa.x, fa.y, fa.z}, pb[3] = {fb.x, fb.y, fb.z};
+            L = setup_line(a.lm, pa, pb, a.width, a.height, a.line_width);
+        }
+    }
+    Walk w = walk_start(L);
+    int px, py;
+    double t;
+    while (next_fragment(L, w, a.width, a.height, px, py, t)) {
+        const double z = L.az + t * (L.bz - L.az);
+        atomicMin(&a.depth_bits[uint32_t(py) * uint32_t(a.width) + uint32_t(px)], __float_as_uint(float(z)));
+    }
+}
+
+__global__ void __launch_bounds__(kRasterBlock)
 raster_kernel(RasterArgs a) {
+    __shared__ WaveShared shared[kRasterBlock / kWave];
+    WaveShared& W = shared[threadIdx.x / kWave];
+    const int lane = __lane_id();
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     LineSetup L;
     L.valid = false;
@@ -178,65 +260,85 @@ raster_kernel(RasterArgs a) {
             float3 fa = load_xyz(a.xyz, ia), fb = load_xyz(a.xyz, ib);
             pa[0] = fa.x; pa[1] = fa.y; pa[2] = fa.z;
             pb[0] = fb.x; pb[1] = fb.y; pb[2] = fb.z;
-            L = setup_line(kColor ? a.vp : a.lm, pa, pb, a.width, a.height, a.line_width);
+            L = setup_line(a.vp, pa, pb, a.width, a.height, a.line_width);
         }
     }
-    const int total = L.valid ? L.ncols * L.nrows : 0;
-    // per-segment attribute ends (clipped)
-    double ta[3] = {0, 0, 0}, tb[3] = {0, 0, 0}, tha = 0, thb = 0;
-    if (kColor && L.valid) {
+    // per-segment attribute ends (clipped), parked in LDS for whichever lane
+    // shades this segment's fragments
+    if (L.valid) {
         float3 ua = load_xyz(a.tangents, ia), ub = load_xyz(a.tangents, ib);
         double t0[3] = {ua.x, ua.y, ua.z}, t1[3] = {ub.x, ub.y, ub.z};
         double h0 = a.thickness[ia], h1 = a.thickness[ib];
-        double q0[3], q1[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            ta[c] = t0[c] + L.s0 * (t1[c] - t0[c]);
-            tb[c] = t0[c] + L.s1 * (t1[c] - t0[c]);
-            q0[c] = pa[c] + L.s0 * (pb[c] - pa[c]);
-            q1[c] = pa[c] + L.s1 * (pb[c] - pa[c]);
+            W.seg[kTa + c][lane] = t0[c] + L.s0 * (t1[c] - t0[c]);
+            W.seg[kTb + c][lane] = t0[c] + L.s1 * (t1[c] - t0[c]);
+            W.seg[kPa + c][lane] = pa[c] + L.s0 * (pb[c] - pa[c]);
+            W.seg[kPb + c][lane] = pa[c] + L.s1 * (pb[c] - pa[c]);
         }
-        tha = h0 + L.s0 * (h1 - h0);
-        thb = h0 + L.s1 * (h1 - h0);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) { pa[c] = q0[c]; pb[c] = q1[c]; }
+        W.seg[kTha][lane] = h0 + L.s0 * (h1 - h0);
+        W.seg[kThb][lane] = h0 + L.s1 * (h1 - h0);
+        W.seg[kWa][lane] = L.wa;
+        W.seg[kWb][lane] = L.wb;
+        W.seg[kAz][lane] = L.az;
+        W.seg[kBz][lane] = L.bz;
+        W.segment[lane] = s;
     }
-    // Each pass of this loop gives every lane its next covered candidate (the
-    // uncovered ones are skipped inside the lane's own walk), so the shading
-    // and the PPLL insert below run with the wave's lanes busy.
-    int k = 0;
+    Walk w = walk_start(L);
+    int head = 0, count = 0;  // wave-uniform ring state
     for (;;) {
-        int px = 0, py = 0;
-        double t = 0.0;
-        bool frag = false;
-        while (k < total) {
-            const int kk = k++;
-            if (candidate(L, kk, a.width, a.height, px, py) && covers(L, px, py, a.line_width, t)) {
-                frag = true;
-                break;
+        // fill: every lane with fragments left queues its next one
+        while (count < kWave) {
+            int px = 0, py = 0;
+            double t = 0.0;
+            const bool found = next_fragment(L, w, a.width, a.height, px, py, t);
+            const unsigned long long m = __ballot(found);
+            if (m == 0ull) break;  // every walk has ended
+            if (found) {
+                const int slot = (head + count + int(__popcll(m & ((1ull << lane) - 1ull)))) & (kQueue - 1);
+                W.qt[slot] = t;
+                W.qpix[slot] = uint32_t(py) * uint32_t(a.width) + uint32_t(px);
+                W.qlane[slot] = uint32_t(lane);
             }
+            count += int(__popcll(m));
         }
-        if (!__any(frag)) break;  // every lane has walked all its candidates
-        const uint32_t pix = uint32_t(py) * uint32_t(a.width) + uint32_t(px);
-        const double z = L.az + t * (L.bz - L.az);
-        if (!kColor) {
-            if (frag) atomicMin(&a.depth_bits[pix], __float_as_uint(float(z)));
-            continue;
+        if (count == 0) break;
+        wave_lds_fence();
+        // shade: lane i takes the i-th queued fragment
+        const int n = count < kWave ? count : kWave;
+        const int slot = (head + lane) & (kQueue - 1);
+        bool frag = lane < n;
+        uint32_t pix = 0, src = 0;
+        double t = 0.0;
+        if (frag) {
+            t = W.qt[slot];
+            pix = W.qpix[slot];
+            src = W.qlane[slot];
         }
-        // early depth test (LESS against the subpass depth, no writes)
-        if (frag) frag = float(z) < (a.depth_buffer ? a.depth_buffer[pix] : 1.0f);
+        head = (head + n) & (kQueue - 1);
+        count -= n;
+        double z = 0.0;
         float4 color = make_float4(0, 0, 0, 0);
         if (frag) {
+            z = W.seg[kAz][src] + t * (W.seg[kBz][src] - W.seg[kAz][src]);
+            // early depth test (LESS against the subpass depth, no writes)
+            frag = float(z) < (a.depth_buffer ? a.depth_buffer[pix] : 1.0f);
+        }
+        if (frag) {
+            const int px = int(pix % uint32_t(a.width)), py = int(pix / uint32_t(a.width));
             // perspective-correct varyings
-            double iw = (1.0 - t) / L.wa + t / L.wb;
-            double sp = (t / L.wb) / iw;
+            const double wa = W.seg[kWa][src], wb = W.seg[kWb][src];
+            double iw = (1.0 - t) / wa + t / wb;
+            double sp = (t / wb) / iw;
             double P[3], T[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                P[c] = pa[c] + sp * (pb[c] - pa[c]);
-                T[c] = ta[c] + sp * (tb[c] - ta[c]);
+                const double p0 = W.seg[kPa + c][src], t0 = W.seg[kTa + c][src];
+                P[c] = p0 + sp * (W.seg[kPb + c][src] - p0);
+                T[c] = t0 + sp * (W.seg[kTb + c][src] - t0);
             }
-            double thick = tha + sp * (thb - tha);
+            const double tha = W.seg[kTha][src];
+            double thick = tha + sp * (W.seg[kThb][src] - tha);
             // gpaa (gpaa.glsl:33-51)
             double cl[4];
             xform4(a.vp, P[0], P[1], P[2], cl);
@@ -264,9 +366,9 @@ raster_kernel(RasterArgs a) {
                     for (int q = 0; q < a.pcf_taps; ++q) {
                         double sd = shadow_tex(a, u0 + a.pcf_dx[q], v0 + a.pcf_dy[q]);
                         double depth = fmax(light_depth - sd, 0.0);
-                        double count = depth * 15000.0;
-                        if (depth > 1e-5) count += 1.0;
-                        vis += double(expf(float(count * a.ln_transmit))) * a.pcf_w[q];
+                        double cnt = depth * 15000.0;
+                        if (depth > 1e-5) cnt += 1.0;
+                        vis += double(expf(float(cnt * a.ln_transmit))) * a.pcf_w[q];
                         tot += a.pcf_w[q];
                     }
                     occlusion *= vis / tot;
@@ -283,10 +385,11 @@ raster_kernel(RasterArgs a) {
         }
         uint32_t node = ppll_insert(a.ppll, frag, pix, color, float(z));
         if (node != kPpllNull) {
-            if (a.cap.segment) a.cap.segment[node] = s;
+            if (a.cap.segment) a.cap.segment[node] = W.segment[src];
             if (a.cap.pixel) a.cap.pixel[node] = pix;
             if (a.cap.color) a.cap.color[node] = color;
         }
+        wave_lds_fence();
     }
 }
 
@@ -341,7 +444,7 @@ extern "C" int vkhr_shadow_depth(const float* xyz, uint32_t n_vertices, const ui
     a->n_pairs = n_pairs;
     a->n_vertices = n_vertices;
     a->depth_bits = reinterpret_cast<uint32_t*>(depth_map);
-    raster_kernel<false><<<(n_pairs + 255) / 256, 256, 0, st>>>(*a);
+    shadow_raster_kernel<<<(n_pairs + kRasterBlock - 1) / kRasterBlock, kRasterBlock, 0, st>>>(*a);
     delete a;
     VKHR_CHECK_LAUNCH();
     return VKHR_OK;
@@ -437,7 +540,7 @@ extern "C" int vkhr_raster_strands(const float* xyz, const float* tangents, cons
     a->n_vertices = n_vertices;
     a->ppll = PpllView{ppll->heads, ppll->nodes, ppll->counter, ppll->node_count};
     if (capture) a->cap = CaptureView{capture->segment, capture->pixel, reinterpret_cast<float4*>(capture->color)};
-    raster_kernel<true><<<(n_pairs + 255) / 256, 256, 0, as_stream(stream)>>>(*a);
+    raster_kernel<<<(n_pairs + kRasterBlock - 1) / kRasterBlock, kRasterBlock, 0, as_stream(stream)>>>(*a);
     delete a;
     VKHR_CHECK_LAUNCH();
     return VKHR_OK;
