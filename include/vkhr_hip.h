@@ -197,9 +197,9 @@ int vkhr_voxelize_segments(const float* xyz, uint32_t n_vertices,
  *                  for voxels of 32768 or more contributions (nullable;
  *                  needs tangent_acc);
  *   density_f32  = density_u8 / 255 (nullable; the R8_UNORM value);
- *                  density_u8 must be 4-byte and counts, density_f32
- *                  and tangent_i8x4 16-byte aligned (device allocations
- *                  are);
+ *                  density_u8 must be 4-byte and counts, tangent_acc,
+ *                  density_f32 and tangent_i8x4 16-byte aligned (device
+ *                  allocations are);
  *   cells        = corner-packed density (nullable): for every cell
  *                  (i,j,k), i,j,k in [-1, n-1], the 8 texels of its
  *                  trilinear footprint as one uint64 (byte b = texel
@@ -408,10 +408,10 @@ typedef struct vkhr_batch_style {
     vkhr_aabb bounds;             /* the style's AABB (vkhr_aabb_compute) */
     uint32_t volume;              /* n of the n^3 volume */
     uint32_t reserved_;
-    uint32_t* counts;             /* n^3; nullable with tangent_acc */
-    int64_t* tangent_acc;         /* 2 n^3, nullable */
-    uint8_t* density;             /* n^3, normalized */
-    int8_t* tangent_i8x4;         /* 4 n^3, nullable */
+    uint32_t* counts;             /* n^3; nullable with tangent_acc; 16-byte aligned */
+    int64_t* tangent_acc;         /* 2 n^3, nullable; 16-byte aligned */
+    uint8_t* density;             /* n^3, normalized; 4-byte aligned */
+    int8_t* tangent_i8x4;         /* 4 n^3, nullable; 16-byte aligned */
     uint64_t* cells;              /* (n+1)^3 */
     uint32_t* occupancy;          /* vkhr_occupancy_words(n, n, n), nullable */
     float* rgba;                  /* width*height float4 */

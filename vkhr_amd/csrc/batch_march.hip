@@ -142,6 +142,13 @@ extern "C" int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styl
             set_error("style %u: missing buffer", s);
             return VKHR_ERR_INVALID;
         }
+        const auto misaligned = [](const void* p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) % a) != 0; };
+        if (misaligned(in.tangent_acc, 16) || misaligned(in.counts, 16) || misaligned(in.density, 4) ||
+            misaligned(in.tangent_i8x4, 16)) {
+            delete b;
+            set_error("style %u: tangent_acc, counts, tangent_i8x4 must be 16-byte and density 4-byte aligned", s);
+            return VKHR_ERR_INVALID;
+        }
         BatchVolStyle& v = vs[s];
         v.ox = g.ox; v.oy = g.oy; v.oz = g.oz; v.vx = g.vx; v.vy = g.vy; v.vz = g.vz;
         v.mx = g.mx; v.my = g.my; v.mz = g.mz; v.nx = g.nx; v.ny = g.ny; v.nz = g.nz;

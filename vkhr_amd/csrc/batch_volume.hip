@@ -19,7 +19,8 @@ __device__ __forceinline__ VoxelGrid grid_of(const BatchVolStyle& s) {
 }
 
 // Zeroes every style's counts or tangent accumulator and occupancy, resets the per-style
-// min/max and the raymarch work queue.
+// min/max and the raymarch work queue (vector stores: 16 B per voxel of the
+// accumulator, 4 counts per store).
 __global__ void __launch_bounds__(256)
 batch_clear_kernel(BatchDevice d) {
     if (blockIdx.x == 0) {
@@ -40,13 +41,14 @@ batch_clear_kernel(BatchDevice d) {
         const BatchVolStyle& st = d.styles[ch.style];
         // with the packed accumulator the counts are unpacked by finalize
         // (when they are asked for)
-        for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
-            if (st.acc) {
-                st.acc[2 * i] = 0;
-                st.acc[2 * i + 1] = 0;
-            } else {
-                st.counts[i] = 0u;
-            }
+        if (st.acc) {
+            longlong2* a = reinterpret_cast<longlong2*>(st.acc);
+            for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) a[i] = make_longlong2(0, 0);
+        } else {
+            const uint64_t q1 = ch.end / 4;  // chunks start at multiples of 4
+            for (uint64_t q = ch.begin / 4 + threadIdx.x; q < q1; q += blockDim.x)
+                reinterpret_cast<uint4*>(st.counts)[q] = make_uint4(0u, 0u, 0u, 0u);
+            for (uint64_t i = 4 * q1 + threadIdx.x; i < ch.end; i += blockDim.x) st.counts[i] = 0u;
         }
     }
 }
@@ -75,10 +77,15 @@ batch_finalize_kernel(BatchDevice d) {
     for (uint32_t c = blockIdx.x; c < d.n_voxel_chunks; c += gridDim.x) {
         const BatchChunk ch = d.voxel_chunks[c];
         const BatchVolStyle& st = d.styles[ch.style];
+        const uint32_t* counts_in = st.acc ? nullptr : st.counts;
+        uint32_t* counts_out = st.acc ? st.counts : nullptr;
         uint32_t lo = 255u, hi = 0u;
-        for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
-            uint32_t v = finalize_voxel(st.acc ? nullptr : st.counts, st.acc, i, st.density, st.tangent_q);
-            if (st.acc && st.counts) st.counts[i] = packed_count(st.acc, i);
+        const uint64_t q1 = ch.end / 4;  // chunks start at multiples of 4
+        for (uint64_t q = ch.begin / 4 + threadIdx.x; q < q1; q += blockDim.x)
+            finalize_quad(counts_in, st.acc, 4 * q, st.density, st.tangent_q, &lo, &hi, counts_out);
+        for (uint64_t i = 4 * q1 + threadIdx.x; i < ch.end; i += blockDim.x) {
+            const uint32_t v = finalize_voxel(counts_in, st.acc, i, st.density, st.tangent_q);
+            if (counts_out) counts_out[i] = packed_count(st.acc, i);
             lo = v < lo ? v : lo;
             hi = v > hi ? v : hi;
         }
@@ -118,7 +125,17 @@ batch_normalize_kernel(BatchDevice d) {
         const BatchChunk ch = d.voxel_chunks[c];
         const BatchVolStyle& st = d.styles[ch.style];
         const DensityMap m = density_map(&d.minmax[2 * ch.style], true);
-        for (uint64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x)
+        uchar4* d4 = reinterpret_cast<uchar4*>(st.density);
+        const uint64_t q1 = ch.end / 4;  // chunks start at multiples of 4
+        for (uint64_t q = ch.begin / 4 + threadIdx.x; q < q1; q += blockDim.x) {
+            uchar4 v = d4[q];
+            v.x = uint8_t(texel_density(v.x, m));
+            v.y = uint8_t(texel_density(v.y, m));
+            v.z = uint8_t(texel_density(v.z, m));
+            v.w = uint8_t(texel_density(v.w, m));
+            d4[q] = v;
+        }
+        for (uint64_t i = 4 * q1 + threadIdx.x; i < ch.end; i += blockDim.x)
             st.density[i] = uint8_t(texel_density(st.density[i], m));
     }
 }

@@ -20,7 +20,10 @@ namespace {
 
 __global__ void __launch_bounds__(256)
 minmax_init_kernel(uint32_t* mm) {
-    if (threadIdx.x == 0) { mm[0] = 255u; mm[1] = 0u; }
+    if (threadIdx.x < kMinMaxSlots) {
+        mm[threadIdx.x * kMinMaxStride] = 255u;
+        mm[threadIdx.x * kMinMaxStride + 1] = 0u;
+    }
 }
 
 __global__ void __launch_bounds__(256)
@@ -51,12 +54,13 @@ finalize_counts_kernel(const uint32_t* __restrict__ counts, const long long* __r
             lo = red[0][w] < lo ? red[0][w] : lo;
             hi = red[1][w] > hi ? red[1][w] : hi;
         }
-        atomicMin(&minmax[0], lo);
-        atomicMax(&minmax[1], hi);
+        uint32_t* slot = minmax + (blockIdx.x % kMinMaxSlots) * kMinMaxStride;
+        atomicMin(&slot[0], lo);
+        atomicMax(&slot[1], hi);
     }
 }
 
-// One block per 64 cell columns x 8 cell rows x kCellSlab cell levels
+// One block per 64 cell columns x 8 cell rows x cell_slab() cell levels
 // (CLAMP_TO_BORDER with an opaque-black border: texels outside the volume
 // are 0), with the brick occupancy.
 __global__ void __launch_bounds__(256)
@@ -70,7 +74,7 @@ cells_kernel(const uint8_t* __restrict__ density, uint32_t nx, uint32_t ny, uint
 __global__ void __launch_bounds__(256)
 normalize_kernel(uint8_t* __restrict__ density, size_t n, const uint32_t* __restrict__ minmax, int normalize,
                  float* __restrict__ density_f32) {
-    const DensityMap m = density_map(minmax, normalize != 0);
+    const DensityMap m = density_map_slots(minmax, normalize != 0);
     const size_t stride = size_t(gridDim.x) * blockDim.x;
     const size_t first = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     uchar4* d4 = reinterpret_cast<uchar4*>(density);
@@ -184,8 +188,10 @@ extern "C" int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tange
     if (rc) return rc;
     VKHR_REQUIRE(density_u8, VKHR_ERR_INVALID, "density_u8 is null");
     VKHR_REQUIRE(reinterpret_cast<uintptr_t>(density_u8) % 4 == 0 && reinterpret_cast<uintptr_t>(density_f32) % 16 == 0 &&
-                     reinterpret_cast<uintptr_t>(counts) % 16 == 0 && reinterpret_cast<uintptr_t>(tangent_i8x4) % 16 == 0,
-                 VKHR_ERR_INVALID, "density_u8 / counts, density_f32, tangent_i8x4 must be 4 / 16-byte aligned");
+                     reinterpret_cast<uintptr_t>(counts) % 16 == 0 && reinterpret_cast<uintptr_t>(tangent_i8x4) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(tangent_acc) % 16 == 0,
+                 VKHR_ERR_INVALID,
+                 "density_u8 / counts, tangent_acc, density_f32, tangent_i8x4 must be 4 / 16-byte aligned");
     VKHR_REQUIRE(counts || tangent_acc, VKHR_ERR_INVALID, "neither counts nor tangent_acc");
     VKHR_REQUIRE(!occupancy || cells, VKHR_ERR_INVALID, "occupancy needs cells");
     VKHR_REQUIRE(nx && ny && nz, VKHR_ERR_INVALID, "zero volume resolution");
@@ -198,17 +204,19 @@ extern "C" int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tange
     uint32_t* mm = nullptr;
     if (normalize) {
         void* scr = nullptr;
-        if ((rc = scratch(64, &scr))) return rc;
+        if ((rc = scratch(kMinMaxBytes, &scr))) return rc;
         mm = static_cast<uint32_t*>(scr);
         minmax_init_kernel<<<1, 64, 0, s>>>(mm);
         VKHR_CHECK_LAUNCH();
     }
-    // A capped grid keeps the min/max atomics at one pair per block.
-    finalize_counts_kernel<<<grid_for(n, 256, 2048), 256, 0, s>>>(
+    // One block per 1024 voxels (capped): one min/max atomic pair per block,
+    // spread over the slots.
+    finalize_counts_kernel<<<grid_for(n / 4 + 1, 256, 2048), 256, 0, s>>>(
         counts, reinterpret_cast<const long long*>(tangent_acc), n, density_u8, tangent_i8x4, mm);
     VKHR_CHECK_LAUNCH();
     if (normalize || density_f32) {
-        normalize_kernel<<<grid_for(n / 4 + 1, 256, 65536), 256, 0, s>>>(density_u8, n, mm, normalize, density_f32);
+        // capped: each wave first reduces the min/max slots
+        normalize_kernel<<<grid_for(n / 4 + 1, 256, 2048), 256, 0, s>>>(density_u8, n, mm, normalize, density_f32);
         VKHR_CHECK_LAUNCH();
     }
     if (cells) {

@@ -21,9 +21,8 @@ __device__ __forceinline__ uint32_t voxel_count(const uint32_t* __restrict__ cou
 // voxels may differ).  The sums come from the fixed-point accumulator
 // (voxel.hpp); beyond 32767 contributions they are not decodable and the
 // tangent is 0, as for empty voxels.
-__device__ __forceinline__ char4 quantize_tangent(const long long* __restrict__ acc, size_t i, uint32_t c) {
+__device__ __forceinline__ char4 quantize_tangent_words(long long w0, long long w1, uint32_t c) {
     if (!(c && c < 32768u)) return make_char4(0, 0, 0, 0);
-    const long long w0 = acc[2 * i + 0], w1 = acc[2 * i + 1];
     const long long sy = (long long)(int)(unsigned)(w0 & 0xffffffffll);
     const long long sx = (w0 - sy) / (1ll << 32);
     const long long sz = w1 >> 32;  // w1 = z * 2^32 + count, 0 <= count < 2^32
@@ -37,6 +36,11 @@ __device__ __forceinline__ char4 quantize_tangent(const long long* __restrict__ 
         return static_cast<signed char>(static_cast<int>(v));
     };
     return make_char4(cvt(x), cvt(y), cvt(z), 0);
+}
+
+__device__ __forceinline__ char4 quantize_tangent(const long long* __restrict__ acc, size_t i, uint32_t c) {
+    if (!(c && c < 32768u)) return make_char4(0, 0, 0, 0);
+    return quantize_tangent_words(acc[2 * i + 0], acc[2 * i + 1], c);
 }
 
 // First pass over voxel i: the saturating uint8 view min(count, 255)
@@ -53,17 +57,25 @@ __device__ __forceinline__ uint32_t finalize_voxel(const uint32_t* __restrict__ 
 }
 
 // finalize_voxel of voxels i .. i+3 (i a multiple of 4, all < n) with
-// vector loads and stores; folds their uint8 views into lo / hi.
+// vector loads and stores (the accumulator 16-byte aligned: one load per
+// voxel); folds their uint8 views into lo / hi.  With acc and counts_out the
+// unpacked counts are stored too.
 __device__ __forceinline__ void finalize_quad(const uint32_t* __restrict__ counts, const long long* __restrict__ acc,
                                               size_t i, uint8_t* __restrict__ density,
-                                              int8_t* __restrict__ tangents, uint32_t* lo, uint32_t* hi) {
+                                              int8_t* __restrict__ tangents, uint32_t* lo, uint32_t* hi,
+                                              uint32_t* __restrict__ counts_out = nullptr) {
     uint32_t c[4];
+    longlong2 w[4];
     if (counts) {
         const uint4 v = reinterpret_cast<const uint4*>(counts)[i / 4];
         c[0] = v.x; c[1] = v.y; c[2] = v.z; c[3] = v.w;
     } else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) c[k] = uint32_t((unsigned long long)(acc[2 * (i + k) + 1]) & 0xffffffffull);
+        for (int k = 0; k < 4; ++k) {
+            w[k] = reinterpret_cast<const longlong2*>(acc)[i + k];
+            c[k] = uint32_t((unsigned long long)(w[k].y) & 0xffffffffull);
+        }
+        if (counts_out) reinterpret_cast<uint4*>(counts_out)[i / 4] = make_uint4(c[0], c[1], c[2], c[3]);
     }
     uchar4 d;
     d.x = uint8_t(c[0] < 255u ? c[0] : 255u);
@@ -74,7 +86,8 @@ __device__ __forceinline__ void finalize_quad(const uint32_t* __restrict__ count
     if (tangents) {
         char4 q[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) q[k] = quantize_tangent(acc, i + k, c[k]);
+        for (int k = 0; k < 4; ++k)
+            q[k] = counts ? quantize_tangent(acc, i + k, c[k]) : quantize_tangent_words(w[k].x, w[k].y, c[k]);
         int4 packed;
         packed.x = *reinterpret_cast<int*>(&q[0]);
         packed.y = *reinterpret_cast<int*>(&q[1]);
@@ -109,14 +122,41 @@ struct DensityMap {
     bool normalize;
 };
 
-__device__ __forceinline__ DensityMap density_map(const uint32_t* minmax, bool normalize) {
+__device__ __forceinline__ DensityMap density_map_of(uint32_t lo, uint32_t hi, bool normalize) {
     DensityMap m;
-    m.lo = normalize ? minmax[0] : 0u;
-    const uint32_t hi = normalize ? minmax[1] : 255u;
+    m.lo = normalize ? lo : 0u;
+    hi = normalize ? hi : 255u;
     m.flat = hi == m.lo;
     m.scaling = normalize_scaling(m.lo, hi);
     m.normalize = normalize;
     return m;
+}
+
+__device__ __forceinline__ DensityMap density_map(const uint32_t* minmax, bool normalize) {
+    return density_map_of(normalize ? minmax[0] : 0u, normalize ? minmax[1] : 255u, normalize);
+}
+
+// The per-volume min/max of the finalize is kept in kMinMaxSlots slots, one
+// per 64-byte line (block b updates slot b % kMinMaxSlots), so that the
+// blocks' atomics do not serialize on one address.
+constexpr int kMinMaxSlots = 32;
+constexpr int kMinMaxStride = 16;  // words between slots
+constexpr size_t kMinMaxBytes = size_t(kMinMaxSlots) * kMinMaxStride * sizeof(uint32_t);
+
+// Min/max over the slots; called by every lane of a wave (shuffles).
+__device__ __forceinline__ DensityMap density_map_slots(const uint32_t* minmax, bool normalize) {
+    const int lane = threadIdx.x % kWave;
+    uint32_t lo = 255u, hi = 0u;
+    if (normalize && lane < kMinMaxSlots) {
+        lo = minmax[lane * kMinMaxStride];
+        hi = minmax[lane * kMinMaxStride + 1];
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const uint32_t a = __shfl_xor(lo, o, kWave), b = __shfl_xor(hi, o, kWave);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    return density_map_of(lo, hi, normalize);
 }
 
 __device__ __forceinline__ uint32_t texel_density(uint32_t d, const DensityMap& m) {
@@ -129,15 +169,24 @@ __device__ __forceinline__ uint32_t texel_density(uint32_t d, const DensityMap& 
 // texel (i+(b&1), j+(b>>1&1), k+(b>>2)), 0 outside the volume) -- from the
 // final uint8 density: cell columns i+1 in [64 xb, 64 xb + 64) (the lanes of
 // a wave), cell rows j+1 in [8 by, 8 by + 8) (two per wave), cell levels
-// k+1 in [kCellSlab zc, kCellSlab zc + kCellSlab) walked upwards, so each
+// k+1 in [slab zc, slab zc + slab) walked upwards (slab = cell_slab), so each
 // level loads only its own texel slice and reuses the one below.  Lanes
 // read consecutive texels and store consecutive cells.  The bricks (8x8x8
 // cells) holding a nonzero cell are or-ed into occupancy (nullable; one word
 // atomic per wave and brick level).  All threads of the block call it.
-constexpr uint32_t kCellSlab = 32;   // cell levels per block
+// Cell levels per block: 32, halved (down to 4) while the volume would give
+// fewer than 2048 blocks, so small volumes still spread over the chip
+// instead of walking long columns on a few compute units.
+__host__ __device__ __forceinline__ uint32_t cell_slab(uint32_t nx, uint32_t ny, uint32_t nz) {
+    const uint32_t plane = ((nx + 1u + 63u) / 64u) * bricks_along(ny);
+    uint32_t slab = 32u;
+    while (slab > 4u && plane * ((nz + 1u + slab - 1u) / slab) < 2048u) slab >>= 1;
+    return slab;
+}
 
 __host__ __device__ __forceinline__ uint32_t cell_blocks(uint32_t nx, uint32_t ny, uint32_t nz) {
-    return ((nx + 1u + 63u) / 64u) * bricks_along(ny) * ((nz + 1u + kCellSlab - 1u) / kCellSlab);
+    const uint32_t slab = cell_slab(nx, ny, nz);
+    return ((nx + 1u + 63u) / 64u) * bricks_along(ny) * ((nz + 1u + slab - 1u) / slab);
 }
 
 __device__ __forceinline__ void cells_block(const uint8_t* __restrict__ density, uint32_t nx, uint32_t ny,
@@ -148,7 +197,8 @@ __device__ __forceinline__ void cells_block(const uint8_t* __restrict__ density,
     const uint32_t xb = block % xbs, by = (block / xbs) % bys, zc = block / (xbs * bys);
     const uint32_t cxi = xb * 64u + lane;                        // cell column i + 1
     const bool col = cxi <= nx;
-    const uint32_t k0 = zc * kCellSlab, k1 = min(k0 + kCellSlab, nz + 1u);
+    const uint32_t slab = cell_slab(nx, ny, nz);
+    const uint32_t k0 = zc * slab, k1 = min(k0 + slab, nz + 1u);
     // the two cell rows of this wave and their texel rows j, j+1
     uint32_t cj[2];
     bool row[2];
@@ -171,30 +221,44 @@ __device__ __forceinline__ void cells_block(const uint8_t* __restrict__ density,
     for (int r = 0; r < 2; ++r)
         below[r] = pair(ty[r][0], int(k0) - 1) | (pair(ty[r][1], int(k0) - 1) << 16);
     uint32_t any = 0u;  // a nonzero cell in this brick level (per lane)
-    for (uint32_t ck = k0; ck < k1; ++ck) {
+    // Levels go in groups of kLevelBatch whose texel loads are all issued
+    // before the first is used: one memory latency per group, not per level.
+    constexpr uint32_t kLevelBatch = 4;
+    for (uint32_t cg = k0; cg < k1; cg += kLevelBatch) {
+        uint32_t above[kLevelBatch][2];
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            const uint32_t above = pair(ty[r][0], int(ck)) | (pair(ty[r][1], int(ck)) << 16);
-            const unsigned long long packed = (unsigned long long)(below[r]) | ((unsigned long long)(above) << 32);
-            if (col && row[r]) {
-                cells[size_t(cj[r] + (ny + 1u) * ck) * (nx + 1u) + cxi] = packed;
-                any |= packed != 0ull ? 1u : 0u;
-            }
-            below[r] = above;
-        }
-        if (occupancy && ((ck & 7u) == 7u || ck + 1u == k1)) {  // wave-uniform: the end of a brick level
-            const unsigned long long nz_lanes = __ballot(any != 0u);
-            uint32_t m = 0u;
+        for (uint32_t u = 0; u < kLevelBatch; ++u)
 #pragma unroll
-            for (int g = 0; g < 8; ++g) m |= ((nz_lanes >> (8 * g)) & 0xffull) ? (1u << g) : 0u;
-            if (lane == 0 && m) {
-                // bricks 8 xb .. 8 xb + 7 of brick row (by, ck / 8): bits first .. first + 7
-                const uint32_t first = bricks_along(nx) * (by + bys * (ck >> kBrickShift)) + 8u * xb;
-                const unsigned long long bits = (unsigned long long)(m) << (first & 31u);
-                if (uint32_t(bits)) atomicOr(&occupancy[first >> 5], uint32_t(bits));
-                if (uint32_t(bits >> 32)) atomicOr(&occupancy[(first >> 5) + 1], uint32_t(bits >> 32));
+            for (int r = 0; r < 2; ++r)
+                above[u][r] = cg + u < k1 ? pair(ty[r][0], int(cg + u)) | (pair(ty[r][1], int(cg + u)) << 16) : 0u;
+#pragma unroll
+        for (uint32_t u = 0; u < kLevelBatch; ++u) {
+            const uint32_t ck = cg + u;
+            if (ck >= k1) break;  // wave-uniform
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const unsigned long long packed =
+                    (unsigned long long)(below[r]) | ((unsigned long long)(above[u][r]) << 32);
+                if (col && row[r]) {
+                    cells[size_t(cj[r] + (ny + 1u) * ck) * (nx + 1u) + cxi] = packed;
+                    any |= packed != 0ull ? 1u : 0u;
+                }
+                below[r] = above[u][r];
             }
-            any = 0u;
+            if (occupancy && ((ck & 7u) == 7u || ck + 1u == k1)) {  // wave-uniform: the end of a brick level
+                const unsigned long long nz_lanes = __ballot(any != 0u);
+                uint32_t m = 0u;
+#pragma unroll
+                for (int g = 0; g < 8; ++g) m |= ((nz_lanes >> (8 * g)) & 0xffull) ? (1u << g) : 0u;
+                if (lane == 0 && m) {
+                    // bricks 8 xb .. 8 xb + 7 of brick row (by, ck / 8): bits first .. first + 7
+                    const uint32_t first = bricks_along(nx) * (by + bys * (ck >> kBrickShift)) + 8u * xb;
+                    const unsigned long long bits = (unsigned long long)(m) << (first & 31u);
+                    if (uint32_t(bits)) atomicOr(&occupancy[first >> 5], uint32_t(bits));
+                    if (uint32_t(bits >> 32)) atomicOr(&occupancy[(first >> 5) + 1], uint32_t(bits >> 32));
+                }
+                any = 0u;
+            }
         }
     }
 }
