@@ -263,7 +263,9 @@ typedef struct vkhr_volume {
  * (src/vkhr/rasterizer/volume.cc:104-203).  A pixel gets a fragment when its
  * centre ray from camera->position enters the AABB in front of the eye
  * (CW front faces, back faces culled: no fragments with the eye inside).
- * Trilinear filtering is computed exactly (not by a texture unit).
+ * Trilinear filtering is computed exactly (not by a texture unit).  Volume
+ * limits (VKHR_ERR_OVERFLOW beyond): (nx+1)(ny+1)(nz+1) < 2^32,
+ * (ny+1)(nz+1) < 2^24 and nx+1 < 2^24.
  *   depth_buffer  nullable per-pixel depth read by the march's early break
  *                 (the subpass depth input); NULL reads 1.0, the cleared value
  *   rgba          W*H float4: the fragment colour clamped to [0,1] (what

@@ -206,19 +206,29 @@ __device__ __forceinline__ void raymarch_pixel(const RaymarchArgs& a, int px, in
                               (1e-9 + 1e-13 * (fabs(zs) + fabs(db * ws) + fabs(dz) + fabs(db * dw)) / slope);
             }
         }
+        // loop invariants in registers (not re-read from the argument block)
+        const DensityVolume vol = a.vol;
+        const double iso = a.iso;
+        const float step = a.step_size;
         int i = 0;
-        for (float t = 0.0f; t < 1.0f; t += a.step_size, ++i) {
+        for (float t = 0.0f; t < 1.0f; t += step, ++i) {
             const double td = double(t);
-            // depth test: db < z/w  <=>  db*w < z  (w > 0 in front of the eye)
-            if (td >= t_depth && db * fma(td, dw, ws) < fma(td, dz, zs)) break;
-            if (td > t_hi) break;
-            float d = march_tap<kF32>(a.vol, fma(td, dx[0], xs[0]), fma(td, dx[1], xs[1]), fma(td, dx[2], xs[2]), occ);
+            // depth test: db < z/w  <=>  db*w < z  (w > 0 in front of the eye);
+            // evaluated as a predicate (no branch around the products)
+            const bool behind = (td >= t_depth) & (db * fma(td, dw, ws) < fma(td, dz, zs));
+            if (behind | (td > t_hi)) break;
+            const float d = march_tap<kF32>(vol, fma(td, dx[0], xs[0]), fma(td, dx[1], xs[1]), fma(td, dx[2], xs[2]),
+                                            occ);
             acc += double(d);
-            if (d != 0.0f) {
-                if (acc <= a.iso) { surf = i; t_surf = t; }
-                if (acc >= a.iso) found = true;
-                if (entry < 0) { entry = i; t_entry = t; }
-            }
+            // surface / entry bookkeeping as selects
+            const bool nonzero = d != 0.0f;
+            const bool below = nonzero & (acc <= iso);
+            surf = below ? i : surf;
+            t_surf = below ? t : t_surf;
+            found = found | (nonzero & (acc >= iso));
+            const bool first = nonzero & (entry < 0);
+            entry = first ? i : entry;
+            t_entry = first ? t : t_entry;
         }
         info = make_int4(2, surf, entry, found ? 1 : 0);  // marched; 1 once a fragment is written
         if (a.record) {
@@ -259,11 +269,13 @@ __device__ __forceinline__ void raymarch_pixel(const RaymarchArgs& a, int px, in
                 // only the samples inside the volume's support add anything
                 double t_lo, t_hi;
                 support_interval(a, sx, xl, &t_lo, &t_hi);
-                for (float t = 0.0f; t < 1.0f; t += a.step_size) {
+                const DensityVolume vol = a.vol;
+                const float step = a.step_size;
+                for (float t = 0.0f; t < 1.0f; t += step) {
                     const double td = double(t);
                     if (td > t_hi) break;
                     if (td < t_lo) continue;
-                    strands += double(march_tap<kF32>(a.vol, fma(td, xl[0], sx[0]), fma(td, xl[1], sx[1]),
+                    strands += double(march_tap<kF32>(vol, fma(td, xl[0], sx[0]), fma(td, xl[1], sx[1]),
                                                       fma(td, xl[2], sx[2]), occ));
                 }
                 adsm = pow(1.0 - a.alpha, strands * a.adsm_k);

@@ -93,29 +93,37 @@ __device__ __forceinline__ float density_at(const DensityVolume& v, double x, do
 // density_at for the march loops, whose samples lie inside the support
 // interval of their ray (coordinates within a margin of [-1, n), so the
 // float-to-int conversions are exact): the range test x in [-1, n) is done
-// on the integer cell, floor(x) in [-1, n-1]; the cell index is 32-bit (the
-// callers bound the cell count below 2^32).  kF32 selects the float volume.
+// on the integer cell, floor(x) in [-1, n-1], and folded with the brick
+// occupancy into one predicate (one branch around the gather).  Indices
+// use 24-bit multiplies: the callers bound n + 1 and (ny + 1)(nz + 1)
+// below 2^24 (and the cell count below 2^32).  kF32 selects the float
+// volume.
 template <bool kF32>
 __device__ __forceinline__ float march_tap(const DensityVolume& v, double x, double y, double z, OccupancyView occ) {
     const double fx = floor(x), fy = floor(y), fz = floor(z);
     const int ix = int(fx), iy = int(fy), iz = int(fz);
     const uint32_t cx1 = uint32_t(ix + 1), cy1 = uint32_t(iy + 1), cz1 = uint32_t(iz + 1);
-    if (cx1 > uint32_t(v.nx) || cy1 > uint32_t(v.ny) || cz1 > uint32_t(v.nz)) return 0.0f;
+    bool live = (cx1 <= uint32_t(v.nx)) & (cy1 <= uint32_t(v.ny)) & (cz1 <= uint32_t(v.nz));
     if (occ.use) {
-        const uint32_t b = (cx1 >> kBrickShift) + v.bx * ((cy1 >> kBrickShift) + v.by * (cz1 >> kBrickShift));
-        if (!((occ.lds[b >> 5] >> (b & 31u)) & 1u)) return 0.0f;
+        const uint32_t b = (cx1 >> kBrickShift) +
+                           __umul24(v.bx, (cy1 >> kBrickShift) + __umul24(v.by, cz1 >> kBrickShift));
+        const uint32_t word = occ.lds[live ? (b >> 5) : 0u];
+        live = live & (((word >> (b & 31u)) & 1u) != 0u);
     }
+    if (!live) return 0.0f;
     const float wx = float(x - fx), wy = float(y - fy), wz = float(z - fz);
     float b[8];
     if constexpr (kF32) {
 #pragma unroll
         for (int n = 0; n < 8; ++n) {
             const int tx = ix + (n & 1), ty = iy + ((n >> 1) & 1), tz = iz + (n >> 2);
-            const bool in = tx >= 0 && ty >= 0 && tz >= 0 && tx < v.nx && ty < v.ny && tz < v.nz;
-            b[n] = in ? v.f32[uint32_t(tx) + uint32_t(v.nx) * (uint32_t(ty) + uint32_t(v.ny) * uint32_t(tz))] : 0.0f;
+            const bool in = (tx >= 0) & (ty >= 0) & (tz >= 0) & (tx < v.nx) & (ty < v.ny) & (tz < v.nz);
+            b[n] = in ? v.f32[uint32_t(tx) + __umul24(uint32_t(v.nx), uint32_t(ty) + __umul24(uint32_t(v.ny),
+                                                                                             uint32_t(tz)))]
+                      : 0.0f;
         }
     } else {
-        const uint32_t c = cx1 + (uint32_t(v.nx) + 1u) * (cy1 + (uint32_t(v.ny) + 1u) * cz1);
+        const uint32_t c = cx1 + __umul24(uint32_t(v.nx) + 1u, cy1 + __umul24(uint32_t(v.ny) + 1u, cz1));
         const unsigned long long q = v.cells[c];
         const uint32_t lo = uint32_t(q), hi = uint32_t(q >> 32);
         b[0] = float(lo & 0xffu); b[1] = float((lo >> 8) & 0xffu);
