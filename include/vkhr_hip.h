@@ -502,6 +502,87 @@ int vkhr_adjoint_batch_create(const vkhr_adjoint_style* styles, uint32_t n_style
 int vkhr_adjoint_batch_launch(vkhr_adjoint_batch* batch, vkhr_stream stream);
 int vkhr_adjoint_batch_destroy(vkhr_adjoint_batch* batch);
 
+/* ------------------------------------------------------------------------
+ * Strand ray tracer: the reference's Embree renderer (vkhr::Raytracer,
+ * src/vkhr/ray_tracer.cc:28-169; embree::HairStyle, ray_tracer/hair_style.cc:
+ * 11-110), a ground truth for the raymarcher's shadows and AO.  Segments
+ * are flat linear curves (ray-facing ribbons, radius linear along the
+ * segment) -- RTC_GEOMETRY_TYPE_FLAT_LINEAR_CURVE with vertex positions and
+ * the per-vertex thickness as radius (HairStyle::create_position_thickness_
+ * data, scene_graph/hair_style.cc:451-464).  A ray (o, d) hits segment
+ * (p0, r0)-(p1, r1) when, with the segment projected on the plane through o
+ * perpendicular to d, the point at u = clamp(-x0.(x1-x0) / |x1-x0|^2, 0, 1)
+ * lies within r0 + u (r1 - r0) of o and its depth t = (p(u)-o).d / d.d is in
+ * [tnear, tfar]; the closest hit breaks ties to the lowest segment index.
+ * ---------------------------------------------------------------------- */
+
+/* Camera::ViewingPlane (include/vkhr/scene_graph/camera.hh:70-75, built by
+ * camera.cc:251-266): pixel (i, j) of the frame looks along
+ * (i + jx) x + (j + jy) y + z from point. */
+typedef struct vkhr_viewing_plane {
+    float x[3], y[3], z[3], point[3];
+} vkhr_viewing_plane;
+
+typedef struct vkhr_trace_params {
+    uint32_t method;     /* Raytracer::VisualizationMethod: 0 shaded, 1 combined shadows,
+                            2 direct shadows, 3 ambient occlusion */
+    int32_t shadows_on;  /* Raytracer::shadows_on */
+    float ao_radius;     /* Raytracer::ao_radius (2.5 in the reference) */
+    float exponent;      /* embree::HairStyle::hair_exponent (50) */
+    float diffuse[3];    /* hair_diffuse: the style's default colour */
+    uint32_t seed;       /* random draws: a counter-based hash of (seed, pixel, frame, draw) */
+    uint32_t frame;      /* sample index */
+    uint32_t reserved_;
+} vkhr_trace_params;
+
+typedef struct vkhr_bvh vkhr_bvh;
+
+/* Builds the BVH of the segments of one style on the device (a linear BVH:
+ * Morton-sorted leaves, binary radix tree, bottom-up boxes); the positions
+ * and radii are copied into it, so the inputs may change afterwards.
+ * Synchronises the stream (load-time call); allocates. */
+int vkhr_bvh_build(const float* xyz, const float* radius, uint32_t n_vertices, const uint32_t* indices,
+                   uint32_t n_indices, vkhr_stream stream, vkhr_bvh** out);
+int vkhr_bvh_destroy(vkhr_bvh* bvh);
+
+/* Segment count and the box of the segment centroids (min xyz, max xyz). */
+int vkhr_bvh_info(const vkhr_bvh* bvh, uint32_t* n_segments, float* bounds);
+
+/* Copies the tree (for structural checks; each pointer nullable):
+ *   nodes         (n-1) x 16 floats: left box lo xyz + left child, left box
+ *                 hi xyz + right child, right box lo xyz + 0, right box hi
+ *                 xyz + 0 (children as uint32 bits, 0x80000000 | leaf for
+ *                 leaves; node 0 is the root)
+ *   parent        (n-1) internal then n leaf parents (0xffffffff at the root)
+ *   leaf_segment  n: the index pair of each leaf */
+int vkhr_bvh_nodes(const vkhr_bvh* bvh, float* nodes, uint32_t* parent, uint32_t* leaf_segment,
+                   vkhr_stream stream);
+
+/* Traces n rays (8 floats each: origin xyz, tnear, direction xyz, tfar):
+ * the closest hit (any_hit == 0: t, u, segment) or whether any hit exists
+ * (any_hit != 0: segment of some hit).  Misses give segment 0xffffffff; t_out
+ * and u_out are nullable. */
+int vkhr_trace_rays(const vkhr_bvh* bvh, const float* rays, uint32_t n_rays, int any_hit, float* t_out,
+                    float* u_out, uint32_t* segment_out, vkhr_stream stream);
+
+/* Adds one sample per pixel of Raytracer::draw (ray_tracer.cc:76-113) to
+ * accum (width x height x 3 doubles, row 0 at the top: the back buffer):
+ * a jittered primary ray; at its hit a shadow ray (light_shading,
+ * :115-140 -- its direction is the jittered light position itself, as in the
+ * reference) and, except for direct shadows, one ambient-occlusion ray in a
+ * random direction that scores 2 when its closest hit is farther than
+ * ao_radius and 0 otherwise (:142-160); a miss scores (1, 1, 1).  The
+ * average over samples is the reference's frame.  Differences: the draws
+ * come from a seeded counter hash (the reference shares one unsynchronised
+ * xorshift state among threads), and where the reference would produce NaN
+ * (a negative 1 - cos^2 from rounding, or a negative cosine sum raised to
+ * the exponent) the sine or the specular term is 0.  indices are the
+ * style's index pairs (segment -> vertices, for the tangent at the hit);
+ * tangents are needed for method 0. */
+int vkhr_raytrace(const vkhr_bvh* bvh, const float* tangents, const uint32_t* indices,
+                  const vkhr_viewing_plane* plane, const vkhr_light* light, const vkhr_trace_params* params,
+                  uint32_t width, uint32_t height, double* accum, vkhr_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

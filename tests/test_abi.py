@@ -3,6 +3,8 @@ declares; struct layouts match the header.  No compute calls (CPU only)."""
 import ctypes as C
 import re
 
+import pytest
+
 from vkhr_amd import capi
 
 
@@ -69,3 +71,14 @@ def test_init_without_gpu_reports_an_error_code():
     rc = capi.lib.vkhr_init()
     assert rc < 0
     assert capi.lib.vkhr_last_error()
+
+
+def test_misplaced_arguments_are_refused_before_the_call():
+    """A pointer-sized value in a 32-bit slot or a small integer in a pointer
+    slot raises on the host (no device work is enqueued)."""
+    from vkhr_amd import capi
+    ptr = 0x7F0000001000
+    with pytest.raises(ValueError, match="not a pointer"):
+        capi.call("vkhr_bvh_build", ptr, 4000, ptr, ptr, 8, None, None)
+    with pytest.raises(ValueError, match="does not fit 32 bits"):
+        capi.call("vkhr_bvh_build", ptr, ptr, ptr, ptr, 8, None, None)

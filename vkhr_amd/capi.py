@@ -102,6 +102,16 @@ class Tuning(C.Structure):
                 ("reserved_", C.c_uint32)]
 
 
+class ViewingPlane(C.Structure):
+    _fields_ = [("x", C.c_float * 3), ("y", C.c_float * 3), ("z", C.c_float * 3), ("point", C.c_float * 3)]
+
+
+class TraceParams(C.Structure):
+    _fields_ = [("method", C.c_uint32), ("shadows_on", C.c_int32), ("ao_radius", C.c_float),
+                ("exponent", C.c_float), ("diffuse", C.c_float * 3), ("seed", C.c_uint32), ("frame", C.c_uint32),
+                ("reserved_", C.c_uint32)]
+
+
 P = C.c_void_p
 U32 = C.c_uint32
 I32 = C.c_int
@@ -138,6 +148,12 @@ SIGNATURES = {
     "vkhr_adjoint_batch_launch": [P, P],
     "vkhr_adjoint_batch_destroy": [P],
     "vkhr_volume_downsample": [P, U32, U32, U32, I32, P, P],
+    "vkhr_bvh_build": [P, P, U32, P, U32, P, C.POINTER(C.c_void_p)],
+    "vkhr_bvh_destroy": [P],
+    "vkhr_bvh_info": [P, P, P],
+    "vkhr_bvh_nodes": [P, P, P, P, P],
+    "vkhr_trace_rays": [P, P, U32, I32, P, P, P, P],
+    "vkhr_raytrace": [P, P, P, C.POINTER(ViewingPlane), C.POINTER(Light), C.POINTER(TraceParams), U32, U32, P, P],
     "vkhr_volume_filter": [P, U32, U32, U32, U32, P, P],
 }
 # entry points that do not return a status
@@ -176,9 +192,24 @@ def _load() -> C.CDLL:
 lib = _load()
 
 
+def _check_args(name: str, fn, args) -> None:
+    """Catches arguments passed in the wrong position before they reach the
+    device: a small integer where a pointer goes, or a pointer-sized value
+    where a 32-bit integer goes."""
+    for i, (v, t) in enumerate(zip(args, fn.argtypes or ())):
+        if not isinstance(v, int) or isinstance(v, bool):
+            continue
+        if t is C.c_void_p and 0 < v < 65536:
+            raise ValueError(f"{name}: argument {i} is {v}, not a pointer")
+        if t in (C.c_uint32, C.c_int) and not (-(1 << 31) <= v < (1 << 32)):
+            raise ValueError(f"{name}: argument {i} ({v:#x}) does not fit 32 bits")
+
+
 def call(name: str, *args) -> None:
     """Calls a status-returning entry point and raises VkhrError on failure."""
-    rc = getattr(lib, name)(*args)
+    fn = getattr(lib, name)
+    _check_args(name, fn, args)
+    rc = fn(*args)
     if rc != VKHR_OK:
         raise VkhrError(name, rc, lib.vkhr_last_error().decode(errors="replace"))
 
