@@ -162,6 +162,47 @@ def time_raster(dev, style, xyz, idx, tan, vol, res, scene, cam, light, bounds, 
     return times[0], times[1], times[2], min(ppll.used(), ppll.node_count)
 
 
+def time_trace(dev, style, xyz, idx, tan, scene, bounds, frames=4):
+    """The strand ray tracer at the same workload: BVH build over the
+    1,635,840 segments (radius 0.004 x thickness), then samples of the
+    reference's Raytracer frame at 1920x1080 -- shaded (primary, shadow and
+    AO rays) and AO only; returns a dict of times and ray rates."""
+    import math
+
+    import torch
+    from vkhr_amd import params, synth
+    from vkhr_amd import device as D
+    from vkhr_amd.trace import METHODS, Bvh, viewing_plane
+    W, H = scene.width, scene.height
+    rad = D.to_dev(style.thickness, dev) * 0.004
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        bvh = Bvh(xyz, rad, idx)
+    t_build = (time.perf_counter() - t0) / 3
+    plane = viewing_plane(scene.eye, scene.look_at, (0, 1, 0), math.radians(synth.CAMERA_FOV_DEG), W, H)
+    light = params.light(scene)
+    ao_radius = 2.5 * float(max(bounds["size"])) / 256.0  # the LAO's 2.5 voxels, in world units
+    out = {"workload": "1,635,840 segments (flat linear curves, radius 0.004 x thickness), 1920x1080, "
+                       "1 sample per pixel per frame", "build_ms": t_build * 1e3, "ao_radius": ao_radius}
+    for name in ("shaded", "ambient_occlusion"):
+        acc = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
+        bvh.raytrace(tan, idx, plane, light, W, H, acc, method=METHODS[name], ao_radius=ao_radius, frame=0)
+        torch.cuda.synchronize()
+        hits = int((acc.reshape(-1, 3) != 1.0).any(dim=1).sum().item())
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for f in range(frames):
+            bvh.raytrace(tan, idx, plane, light, W, H, acc, method=METHODS[name], ao_radius=ao_radius, frame=f + 1)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / frames
+        # rays per sample: one primary per pixel; per hit a shadow ray (shaded) and an AO ray
+        rays = W * H + hits * (2 if name == "shaded" else 1)
+        out[name] = {"frame_ms": ms, "hit_pixels": hits, "rays_per_frame": rays, "mrays_per_s": rays / ms / 1e3}
+    return out
+
+
 def time_adjoint_batch(dev):
     """Batched adjoint: 8 generator styles (128^3 float densities) x 8 ring
     views at 512x512, 256 steps; returns (per-render ms, graph ms) for the
@@ -438,6 +479,7 @@ def main():
         result["raster"] = {"workload": "1,635,840 segments, 1920x1080, 1024^2 shadow map, k=16, 2^25-node PPLL",
                             "shadow_ms": t_sh, "raster_ms": t_ra, "resolve_ms": t_re, "fragments": n_frag,
                             "fragments_per_s": n_frag / (t_ra * 1e-3)}
+        result["raytrace"] = time_trace(dev, style, xyz, idx, tan, scene, bounds)
         t_f, t_b = time_adjoint(dev)
         result["adjoint_fwd_ratio"] = t_b / t_f
         result["adjoint"] = {"workload": "128^3 float32 density, 512x512, 256 steps", "forward_ms": t_f,
