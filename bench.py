@@ -203,6 +203,51 @@ def time_trace(dev, style, xyz, idx, tan, scene, bounds, frames=4):
     return out
 
 
+def time_reference_frames(dev, style, xyz, idx, tan, vol, res, bounds, reps=5):
+    """The reference's published setup (ponytail, 1280x720, 1024^2 3x3 PCF
+    shadow map, 256^3 volume, 512 steps; docs/vkhr.tex:352,387): the frame
+    of the strand rasterizer (shadow map, PPLL clear, strands, resolve) and
+    of the raymarcher (PPLL clear, raymarch into the PPLL, resolve), with
+    this ponytail-sized synthetic style.  Returns ms per frame."""
+    import numpy as np
+    import torch
+    from vkhr_amd import params, synth
+    from vkhr_amd import device as D
+    W, H = 1280, 720
+    scene = synth.make_scene(bounds, W, H)
+    cam, light = params.camera(scene), params.light(scene)
+    th = D.to_dev(style.thickness, dev)
+    light_vp = np.asarray(scene.light_vp, np.float64).T.reshape(-1).astype(np.float32)
+    sp = params.strand(bounds, res, strand_radius=1.0)
+    ppll = D.PPLL(W, H, 1 << 25, dev)
+    smap = torch.empty((1024, 1024), dtype=torch.float32, device=dev)
+    out = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+
+    def raster_frame():
+        D.shadow_depth(xyz, idx, light_vp, 1.0, (1024, 1024), out=smap)
+        ppll.clear()
+        D.raster_strands(xyz, tan, th, idx, cam, light, sp, params.render(renderer=0), vol, res, smap, ppll)
+        ppll.resolve(out=out)
+
+    def raymarch_frame():
+        ppll.clear()
+        D.raymarch_ppll(vol, res, cam, light, sp, params.render(renderer=2, raycast_steps=512.0), ppll)
+        ppll.resolve(out=out)
+
+    times = {}
+    for name, fn in (("rasterizer", raster_frame), ("raymarcher", raymarch_frame)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        times[name] = e0.elapsed_time(e1) / reps
+    return times
+
+
 def time_adjoint_batch(dev):
     """Batched adjoint: 8 generator styles (128^3 float densities) x 8 ring
     views at 512x512, 256 steps; returns (per-render ms, graph ms) for the
@@ -479,6 +524,12 @@ def main():
         result["raster"] = {"workload": "1,635,840 segments, 1920x1080, 1024^2 shadow map, k=16, 2^25-node PPLL",
                             "shadow_ms": t_sh, "raster_ms": t_ra, "resolve_ms": t_re, "fragments": n_frag,
                             "fragments_per_s": n_frag / (t_ra * 1e-3)}
+        frames = time_reference_frames(dev, style, xyz, idx, tan, vol, res, bounds)
+        result["reference_setup_720p"] = {
+            "workload": "136,320 strands x 13 vertices, 1280x720, 1024^2 3x3 PCF shadow map, 256^3, 512 steps",
+            "rasterizer_frame_ms": frames["rasterizer"], "raymarcher_frame_ms": frames["raymarcher"],
+            "published": "ponytail rasterized 7 ms per frame on a Radeon Pro WX 9100 (readme.md:8); the "
+                         "raymarcher about 2x faster near (docs/vkhr.tex:403) -- other hardware, not a ratio"}
         result["raytrace"] = time_trace(dev, style, xyz, idx, tan, scene, bounds)
         t_f, t_b = time_adjoint(dev)
         result["adjoint_fwd_ratio"] = t_b / t_f

@@ -73,7 +73,11 @@ def _fragment_parity(s: Scene, ppll, cap, pairs=None):
                    res=s.res, shadow_map=ni["smap"], pairs=pairs)
     kept = o["kept"]
     o_key = o["segment"][kept].astype(np.uint64) << np.uint64(32) | o["pixel"][kept].astype(np.uint64)
-    sel = np.ones(n, bool) if pairs is None else np.isin(seg, np.asarray(pairs, np.uint32))
+    # nodes a block reserved for fragments its shading discarded are never
+    # linked nor captured (segment stays 0xffffffff)
+    sel = seg != np.uint32(0xFFFFFFFF)
+    if pairs is not None:
+        sel &= np.isin(seg, np.asarray(pairs, np.uint32))
     g_key = seg[sel].astype(np.uint64) << np.uint64(32) | pix[sel].astype(np.uint64)
     assert len(np.unique(g_key)) == len(g_key), "a (segment, pixel) fragment inserted twice"
     common, gi, oi = np.intersect1d(g_key, o_key, return_indices=True)

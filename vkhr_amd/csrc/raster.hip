@@ -15,8 +15,10 @@
 // Instead the lanes push their covered fragments into a per-wave LDS queue
 // (ballot + rank), and the float64 shading runs only on full waves of 64
 // queued fragments, each lane shading a fragment of whichever segment it
-// came from (the segments' shading data sit in LDS).  PPLL nodes are
-// allocated with one counter atomic per wave and round.
+// came from (the segments' shading data sit in LDS).  Each block reserves
+// its PPLL node ids with one counter atomic after a counting walk (a
+// counter atomic per wave and round serialized on the one address), and
+// links each node with an exchange on its pixel's head.
 // Window depth interpolates linearly in t, varyings perspective-correctly.
 // Compiled with -ffp-contract=off so the rule evaluates as the float64
 // oracle evaluates it.
@@ -43,9 +45,11 @@ struct LineSetup {
     bool xmajor, valid;
 };
 
-// Walk state of one lane: the current column and its remaining rows.
+// Walk state of one lane: the current column and its remaining rows, and
+// the pixel box the walk is clipped to (along the major / minor axis).
 struct Walk {
     int col, row, row_end;
+    int col_end, minor_lo, minor_hi;  // columns [.., col_end), rows [minor_lo, minor_hi]
 };
 struct RasterArgs {
     double vp[16];                   // view-projection (column-major)
@@ -141,8 +145,16 @@ __device__ LineSetup setup_line(const double* m, const double pa[3], const doubl
     return L;
 }
 
-__device__ __forceinline__ Walk walk_start(const LineSetup& L) {
-    return Walk{L.valid ? L.col0 - 1 : 0, 1, 0};
+// The walk over the pixels of the box [x0, x1) x [y0, y1) only.
+__device__ __forceinline__ Walk walk_start(const LineSetup& L, int x0, int y0, int x1, int y1) {
+    Walk w{0, 1, 0, 0, 0, -1};
+    if (!L.valid) return w;
+    const int mlo = L.xmajor ? x0 : y0, mhi = L.xmajor ? x1 : y1;
+    w.col = max(L.col0, mlo) - 1;
+    w.col_end = min(L.col_end, mhi);
+    w.minor_lo = L.xmajor ? y0 : x0;
+    w.minor_hi = (L.xmajor ? y1 : x1) - 1;
+    return w;
 }
 
 // The line rule at pixel centre (px+0.5, py+0.5); t returned on success.
@@ -158,14 +170,12 @@ __device__ __forceinline__ bool covers(const LineSetup& L, int px, int py, doubl
 
 // Advances a lane's walk to its next covered pixel; false once the walk has
 // passed its last column (and on every later call).
-__device__ __forceinline__ bool next_fragment(const LineSetup& L, Walk& w, int width, int height, int& px, int& py,
-                                              double& t) {
+__device__ __forceinline__ bool next_fragment(const LineSetup& L, Walk& w, int& px, int& py, double& t) {
     if (!L.valid) return false;
-    const int minor_extent = L.xmajor ? height : width;
     for (;;) {
         if (w.row > w.row_end) {
-            if (w.col + 1 >= L.col_end) {
-                w.col = L.col_end;
+            if (w.col + 1 >= w.col_end) {
+                w.col = w.col_end;
                 return false;
             }
             ++w.col;
@@ -173,8 +183,8 @@ __device__ __forceinline__ bool next_fragment(const LineSetup& L, Walk& w, int w
             const double line_minor = L.xmajor ? L.ay + (cm - L.ax) * L.slope : L.ax + (cm - L.ay) * L.slope;
             // rows whose centre (row + 0.5) lies within h (+ margin) of the line
             double r0 = ceil(line_minor - L.h - 0.5 - L.margin), r1 = floor(line_minor + L.h - 0.5 + L.margin);
-            r0 = fmax(r0, 0.0);
-            r1 = fmin(r1, double(minor_extent - 1));
+            r0 = fmax(r0, double(w.minor_lo));
+            r1 = fmin(r1, double(w.minor_hi));
             if (r0 > r1) continue;
             w.row = int(r0);
             w.row_end = int(r1);
@@ -186,10 +196,15 @@ __device__ __forceinline__ bool next_fragment(const LineSetup& L, Walk& w, int w
     }
 }
 
-// Bilinear, clamp-to-edge sample of the shadow map at uv.
+// Bilinear, clamp-to-edge sample of the shadow map at uv.  Branch-free, so
+// that the taps' loads can all be in flight together: a coordinate beyond
+// +-1e9 texels is replaced by 0, whose weights (0) return texel (0, 0) --
+// the value returned for it before.
 __device__ __forceinline__ double shadow_tex(const RasterArgs& a, double u, double v) {
     double x = u * a.sw - 0.5, y = v * a.sh - 0.5;
-    if (!(fabs(x) < 1e9 && fabs(y) < 1e9)) return double(a.shadow_map[0]);
+    const bool sane = fabs(x) < 1e9 && fabs(y) < 1e9;
+    x = sane ? x : 0.0;
+    y = sane ? y : 0.0;
     double fx = floor(x), fy = floor(y);
     double wx = x - fx, wy = y - fy;
     int x0 = int(fx), y0 = int(fy);
@@ -200,6 +215,16 @@ __device__ __forceinline__ double shadow_tex(const RasterArgs& a, double u, doub
     double t01 = a.shadow_map[size_t(y1) * a.sw + x0], t11 = a.shadow_map[size_t(y1) * a.sw + x1];
     double r0 = t00 + wx * (t10 - t00), r1 = t01 + wx * (t11 - t01);
     return r0 + wy * (r1 - r0);
+}
+
+// One PCF tap of approximate_deep_shadow (approximate_deep_shadows.glsl:
+// 12-21): the light-space depth gap -> strand count -> transmittance.
+__device__ __forceinline__ double pcf_tap(const RasterArgs& a, double u0, double v0, double light_depth, int q) {
+    double sd = shadow_tex(a, u0 + a.pcf_dx[q], v0 + a.pcf_dy[q]);
+    double depth = fmax(light_depth - sd, 0.0);
+    double cnt = depth * 15000.0;
+    if (depth > 1e-5) cnt += 1.0;
+    return double(expf(float(cnt * a.ln_transmit))) * a.pcf_w[q];
 }
 
 // Per-wave LDS of the colour pass: the shading data of the wave's 64
@@ -234,26 +259,51 @@ shadow_raster_kernel(RasterArgs a) {
             L = setup_line(a.lm, pa, pb, a.width, a.height, a.line_width);
         }
     }
-    Walk w = walk_start(L);
+    Walk w = walk_start(L, 0, 0, a.width, a.height);
     int px, py;
     double t;
-    while (next_fragment(L, w, a.width, a.height, px, py, t)) {
+    while (next_fragment(L, w, px, py, t)) {
         const double z = L.az + t * (L.bz - L.az);
         atomicMin(&a.depth_bits[uint32_t(py) * uint32_t(a.width) + uint32_t(px)], __float_as_uint(float(z)));
     }
 }
 
-__global__ void __launch_bounds__(kRasterBlock)
-raster_kernel(RasterArgs a) {
-    __shared__ WaveShared shared[kRasterBlock / kWave];
-    WaveShared& W = shared[threadIdx.x / kWave];
+// The PPLL insert of the colour pass (ppll.glsl:23-48).  Node ids come
+// from the block's reservation (one counter atomic per block, made after a
+// counting walk; the wanting lanes of a round take consecutive ids through
+// an LDS cursor): a device-wide atomic on the one counter per wave and round
+// serializes -- 261 k of them measured 3.4 ms at 16.7 M fragments.  Nodes at
+// or past the pool end are dropped, as in the shader; the node is linked by
+// exchanging the pixel's head.
+__device__ __forceinline__ uint32_t block_insert(const PpllView& p, bool want, uint32_t pixel, float4 color,
+                                                 float depth, uint32_t block_base, uint32_t* block_next) {
+    unsigned long long mask = __ballot(want);
+    if (mask == 0ull) return kPpllNull;
     const int lane = __lane_id();
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    uint32_t off = 0;
+    if (lane == leader) off = atomicAdd(block_next, uint32_t(__popcll(mask)));  // LDS
+    off = __shfl(off, leader, kWave);
+    if (!want) return kPpllNull;
+    const uint32_t node = block_base + off + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+    if (node >= p.node_count) return kPpllNull;  // pool exhausted: fragment dropped
+    uint32_t* n = p.nodes + 3ull * node;
+    n[0] = pack_unorm4x8(color);
+    n[1] = __float_as_uint(depth);
+    n[2] = atomicExch(&p.heads[pixel], node);
+    return node;
+}
+
+// The colour pass over one segment per lane (s, when has).  Every thread
+// of the block calls it (it synchronises the block once, for the
+// reservation).
+__device__ void colour_segments(const RasterArgs& a, WaveShared& W, uint32_t s, bool has, uint32_t* block_sums) {
+    const int lane = __lane_id();
     LineSetup L;
     L.valid = false;
     uint32_t ia = 0, ib = 0;
     double pa[3] = {0, 0, 0}, pb[3] = {0, 0, 0};
-    if (s < a.n_pairs) {
+    if (has && s < a.n_pairs) {
         ia = a.indices[2u * s];
         ib = a.indices[2u * s + 1u];
         if (ia < a.n_vertices && ib < a.n_vertices) {
@@ -284,14 +334,32 @@ raster_kernel(RasterArgs a) {
         W.seg[kBz][lane] = L.bz;
         W.segment[lane] = s;
     }
-    Walk w = walk_start(L);
+    // The block reserves its node ids at once: a counting walk (the walk is
+    // repeated below), one LDS atomic per wave, one counter atomic per block.
+    // The count covers every covered candidate; nodes reserved for fragments
+    // the shading then discards (coverage below 0.001, the depth test) stay
+    // unlinked.
+    {
+        uint32_t mine = 0;
+        Walk cw = walk_start(L, 0, 0, a.width, a.height);
+        int px, py;
+        double t;
+        while (next_fragment(L, cw, px, py, t)) ++mine;
+        for (int o = kWave / 2; o > 0; o >>= 1) mine += __shfl_xor(mine, o, kWave);
+        if (lane == 0) atomicAdd(&block_sums[0], mine);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) block_sums[1] = block_sums[0] ? atomicAdd(a.ppll.counter, block_sums[0]) : 0u;
+    __syncthreads();
+    const uint32_t block_base = block_sums[1];
+    Walk w = walk_start(L, 0, 0, a.width, a.height);
     int head = 0, count = 0;  // wave-uniform ring state
     for (;;) {
         // fill: every lane with fragments left queues its next one
         while (count < kWave) {
             int px = 0, py = 0;
             double t = 0.0;
-            const bool found = next_fragment(L, w, a.width, a.height, px, py, t);
+            const bool found = next_fragment(L, w, px, py, t);
             const unsigned long long m = __ballot(found);
             if (m == 0ull) break;  // every walk has ended
             if (found) {
@@ -363,13 +431,17 @@ raster_kernel(RasterArgs a) {
                     double light_depth = ls[2] / ls[3];
                     double u0 = ls[0] / ls[3], v0 = ls[1] / ls[3];
                     double vis = 0.0, tot = 0.0;
-                    for (int q = 0; q < a.pcf_taps; ++q) {
-                        double sd = shadow_tex(a, u0 + a.pcf_dx[q], v0 + a.pcf_dy[q]);
-                        double depth = fmax(light_depth - sd, 0.0);
-                        double cnt = depth * 15000.0;
-                        if (depth > 1e-5) cnt += 1.0;
-                        vis += double(expf(float(cnt * a.ln_transmit))) * a.pcf_w[q];
-                        tot += a.pcf_w[q];
+                    if (a.pcf_taps == 9) {  // the reference's 3x3 kernel, unrolled: 36 loads in flight
+#pragma unroll
+                        for (int q = 0; q < 9; ++q) {
+                            vis += pcf_tap(a, u0, v0, light_depth, q);
+                            tot += a.pcf_w[q];
+                        }
+                    } else {
+                        for (int q = 0; q < a.pcf_taps; ++q) {
+                            vis += pcf_tap(a, u0, v0, light_depth, q);
+                            tot += a.pcf_w[q];
+                        }
                     }
                     occlusion *= vis / tot;
                 }
@@ -383,7 +455,7 @@ raster_kernel(RasterArgs a) {
                                     float(shading[2] * occlusion), float(coverage));
             }
         }
-        uint32_t node = ppll_insert(a.ppll, frag, pix, color, float(z));
+        const uint32_t node = block_insert(a.ppll, frag, pix, color, float(z), block_base, &block_sums[2]);
         if (node != kPpllNull) {
             if (a.cap.segment) a.cap.segment[node] = W.segment[src];
             if (a.cap.pixel) a.cap.pixel[node] = pix;
@@ -391,6 +463,17 @@ raster_kernel(RasterArgs a) {
         }
         wave_lds_fence();
     }
+}
+
+
+// The colour pass: one segment per lane, the whole target.
+__global__ void __launch_bounds__(kRasterBlock) raster_kernel(RasterArgs a) {
+    __shared__ WaveShared shared[kRasterBlock / kWave];
+    __shared__ uint32_t block_sums[3];  // fragments counted, reservation base, LDS allocation cursor
+    if (threadIdx.x < 3) block_sums[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    colour_segments(a, shared[threadIdx.x / kWave], s, s < a.n_pairs, block_sums);
 }
 
 void load_cm(const float* m, double* out) {

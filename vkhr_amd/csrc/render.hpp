@@ -156,6 +156,36 @@ __device__ __forceinline__ double local_ambient_occlusion(const DensityVolume& v
     return pow(1.0 - dens / 8.0, exponent);
 }
 
+// density_at of the corner-packed cells without branches: a sample outside
+// [-1, n) gathers cell 0 and returns 0, so the gathers of a loop of taps
+// can all be in flight together.  Same value as density_at inside.
+__device__ __forceinline__ float density_at_cells_select(const DensityVolume& v, double x, double y, double z) {
+    const bool in = (x >= -1.0) & (y >= -1.0) & (z >= -1.0) & (x < double(v.nx)) & (y < double(v.ny)) &
+                    (z < double(v.nz));
+    x = in ? x : -1.0;
+    y = in ? y : -1.0;
+    z = in ? z : -1.0;
+    const double fx = floor(x), fy = floor(y), fz = floor(z);
+    const float wx = float(x - fx), wy = float(y - fy), wz = float(z - fz);
+    const int ix = int(fx), iy = int(fy), iz = int(fz);
+    const size_t cx = size_t(v.nx) + 1, cy = size_t(v.ny) + 1;
+    const unsigned long long c = v.cells[size_t(ix + 1) + cx * (size_t(iy + 1) + cy * size_t(iz + 1))];
+    const uint32_t lo = uint32_t(c), hi = uint32_t(c >> 32);
+    float b[8];
+    b[0] = float(lo & 0xffu); b[1] = float((lo >> 8) & 0xffu);
+    b[2] = float((lo >> 16) & 0xffu); b[3] = float(lo >> 24);
+    b[4] = float(hi & 0xffu); b[5] = float((hi >> 8) & 0xffu);
+    b[6] = float((hi >> 16) & 0xffu); b[7] = float(hi >> 24);
+    const float c00 = fmaf(wx, b[1] - b[0], b[0]);
+    const float c10 = fmaf(wx, b[3] - b[2], b[2]);
+    const float c01 = fmaf(wx, b[5] - b[4], b[4]);
+    const float c11 = fmaf(wx, b[7] - b[6], b[6]);
+    const float c0 = fmaf(wy, c10 - c00, c00);
+    const float c1 = fmaf(wy, c11 - c01, c01);
+    const float r = fmaf(wz, c1 - c0, c0);
+    return in ? r * (1.0f / 255.0f) : 0.0f;
+}
+
 // The strand rasterizer's per-fragment terms with float32 transcendentals
 // (one power each; the float64 positions and sums stay).  Their relative
 // error, ~1e-7, is far inside the fragment tolerance; the raymarcher keeps
@@ -168,7 +198,9 @@ __device__ __forceinline__ double local_ambient_occlusion_f32pow(const DensityVo
         double ox = ((n & 1) ? 0.5 : -0.5) * scale;
         double oy = (((n >> 1) & 1) ? 0.5 : -0.5) * scale;
         double oz = ((n >> 2) ? 0.5 : -0.5) * scale;
-        dens += fmin(double(density_at(v, s[0] + ox, s[1] + oy, s[2] + oz)), clamp_v);
+        const float d = v.f32 ? density_at(v, s[0] + ox, s[1] + oy, s[2] + oz)
+                              : density_at_cells_select(v, s[0] + ox, s[1] + oy, s[2] + oz);
+        dens += fmin(double(d), clamp_v);
     }
     return double(powf(float(1.0 - dens / 8.0), float(exponent)));
 }
