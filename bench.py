@@ -248,6 +248,57 @@ def time_reference_frames(dev, style, xyz, idx, tan, vol, res, bounds, reps=5):
     return times
 
 
+def ao_ground_truth(dev, style, xyz, idx, tan, bounds, res, W=960, H=540, samples=64):
+    """The ray tracer as the quality oracle of the raymarcher's local ambient
+    occlusion: per pixel, the raymarcher's LAO factor at its surface point
+    (from the march record) against the ray-traced AO at the strand hit
+    (the unoccluded fraction of `samples` AO rays of the LAO's radius).
+    Returns the Pearson correlation over pixels where both hit, and means."""
+    import math
+
+    import numpy as np
+    import torch
+    from vkhr_amd import params, synth
+    from vkhr_amd import device as D
+    from vkhr_amd.trace import METHODS, Bvh, viewing_plane
+    sums = torch.empty(2 * res[0] * res[1] * res[2], dtype=torch.int64, device=dev)
+    D.voxelize(xyz, bounds, res, indices=idx, tangents=tan, want_sums=True, sums=sums, want_counts=False)
+    vol = D.finalize(None, res, sums=sums, normalize=True, want_f32=True, want_cells=False)
+    vol = dict(density=vol["density"], cells=None, tangents=vol["tangents"], density_f32=vol["density_f32"],
+               occupancy=None)
+    scene = synth.make_scene(bounds, W, H)
+    cam, light = params.camera(scene), params.light(scene)
+    sp, rp = params.strand(bounds, res), params.render(raycast_steps=512.0)
+    _, rec = D.raymarch_record(vol, res, cam, light, sp, rp, D.diff_params(0.8), W, H)
+    r = rec.view(-1, 64).cpu().numpy()
+    lao = r[:, 36:40].copy().view(np.float32).reshape(H, W)
+    state = r[:, 60:64].copy().view(np.int32).reshape(H, W)
+    rad = D.to_dev(style.thickness, dev) * 0.004
+    bvh = Bvh(xyz, rad, idx)
+    plane = viewing_plane(scene.eye, scene.look_at, (0, 1, 0), math.radians(synth.CAMERA_FOV_DEG), W, H)
+    ao_radius = float(rp.occlusion_radius) * float(max(bounds["size"])) / res[0]
+    def unoccluded(ignore_self):
+        acc = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
+        for f in range(samples):
+            bvh.raytrace(tan, idx, plane, light, W, H, acc, method=METHODS["ambient_occlusion"], ao_radius=ao_radius,
+                         frame=f, ignore_self=ignore_self)
+        a = acc[:, :, 0].cpu().numpy() / samples
+        hit = a != 1.0  # a miss scores 1 on every sample
+        return np.where(hit, a / 2.0, np.nan)[:, ::-1]  # the viewing plane's x runs to the left: mirror it
+
+    rt = unoccluded(True)
+    rt_ref = unoccluded(False)
+    both = (state == 1) & np.isfinite(rt)
+    x, y = lao[both].astype(np.float64), rt[both]
+    corr = float(np.corrcoef(x, y)[0, 1]) if both.sum() > 2 else float("nan")
+    return {"workload": f"{W}x{H}, {samples} AO samples per pixel, radius {ao_radius:.4f} (the LAO's "
+                        f"{rp.occlusion_radius} voxels), AO rays skip the segment they start on",
+            "pixels": int(both.sum()), "pearson_lao_vs_raytraced": corr,
+            "mean_lao": float(x.mean()) if len(x) else None,
+            "mean_raytraced_unoccluded": float(y.mean()) if len(y) else None,
+            "mean_raytraced_unoccluded_reference_rule": float(np.nanmean(rt_ref[both])) if both.any() else None}
+
+
 def time_adjoint_batch(dev):
     """Batched adjoint: 8 generator styles (128^3 float densities) x 8 ring
     views at 512x512, 256 steps; returns (per-render ms, graph ms) for the
@@ -531,6 +582,7 @@ def main():
             "published": "ponytail rasterized 7 ms per frame on a Radeon Pro WX 9100 (readme.md:8); the "
                          "raymarcher about 2x faster near (docs/vkhr.tex:403) -- other hardware, not a ratio"}
         result["raytrace"] = time_trace(dev, style, xyz, idx, tan, scene, bounds)
+        result["ao_ground_truth"] = ao_ground_truth(dev, style, xyz, idx, tan, bounds, res)
         t_f, t_b = time_adjoint(dev)
         result["adjoint_fwd_ratio"] = t_b / t_f
         result["adjoint"] = {"workload": "128^3 float32 density, 512x512, 256 steps", "forward_ms": t_f,

@@ -522,7 +522,9 @@ int vkhr_adjoint_batch_destroy(vkhr_adjoint_batch* batch);
 
 /* Camera::ViewingPlane (include/vkhr/scene_graph/camera.hh:70-75, built by
  * camera.cc:251-266): pixel (i, j) of the frame looks along
- * (i + jx) x + (j + jy) y + z from point. */
+ * (i + jx) x + (j + jy) y + z from point.  x is the camera's left, as in the
+ * reference, so column 0 of the traced frame is the right edge of the
+ * rasterized one. */
 typedef struct vkhr_viewing_plane {
     float x[3], y[3], z[3], point[3];
 } vkhr_viewing_plane;
@@ -536,7 +538,8 @@ typedef struct vkhr_trace_params {
     float diffuse[3];    /* hair_diffuse: the style's default colour */
     uint32_t seed;       /* random draws: a counter-based hash of (seed, pixel, frame, draw) */
     uint32_t frame;      /* sample index */
-    uint32_t reserved_;
+    int32_t ignore_self; /* 0: the reference's rule (shadow and AO rays may hit the segment they
+                            start on, beyond tnear = 1e-6); 1: they skip it */
 } vkhr_trace_params;
 
 typedef struct vkhr_bvh vkhr_bvh;

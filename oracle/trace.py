@@ -60,7 +60,8 @@ def any_hit(xyz, radius, indices, rays):
 
 
 def raytrace_frame(xyz, radius, tangents, indices, plane, light_origin, intensity, diffuse, exponent=50.0,
-                   method=0, shadows_on=True, ao_radius=2.5, seed=0, frame=0, width=64, height=64, accum=None):
+                   method=0, shadows_on=True, ao_radius=2.5, seed=0, frame=0, width=64, height=64, accum=None,
+                   ignore_self=False):
     """Adds one sample per pixel (Raytracer::draw) to accum (H, W, 3) float64."""
     xyz = np.ascontiguousarray(xyz, np.float32)
     rad = np.ascontiguousarray(radius, np.float32)
@@ -74,5 +75,5 @@ def raytrace_frame(xyz, radius, tangents, indices, plane, light_origin, intensit
     lib.oracle_raytrace_frame(_p(xyz), _p(rad), _p(tan), _p(idx), C.c_uint64(len(idx)), _p(pl), _p(light), _p(dif),
                               C.c_float(exponent), C.c_int(method), C.c_int(1 if shadows_on else 0),
                               C.c_float(ao_radius), C.c_uint32(seed), C.c_uint32(frame), C.c_uint32(width),
-                              C.c_uint32(height), _p(accum))
+                              C.c_uint32(height), C.c_int(1 if ignore_self else 0), _p(accum))
     return accum

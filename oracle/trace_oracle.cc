@@ -87,10 +87,11 @@ struct Scene {
 };
 
 // Closest hit, ties to the lowest segment index.
-Hit closest(const Scene& s, const float o[3], const float d[3], float tnear, float tfar) {
+Hit closest(const Scene& s, const float o[3], const float d[3], float tnear, float tfar, uint64_t skip = ~0ull) {
     Hit best{0.0f, 0.0f, 0u, false};
     const float dd = dot3(d, d);
     for (uint64_t k = 0; k < s.n_pairs; ++k) {
+        if (k == skip) continue;
         const uint32_t a = s.indices[2 * k], b = s.indices[2 * k + 1];
         float t, u;
         if (!flat_curve(o, d, dd, s.xyz + 3 * size_t(a), s.radius[a], s.xyz + 3 * size_t(b), s.radius[b], tnear,
@@ -101,9 +102,10 @@ Hit closest(const Scene& s, const float o[3], const float d[3], float tnear, flo
     return best;
 }
 
-bool any_hit(const Scene& s, const float o[3], const float d[3], float tnear, float tfar) {
+bool any_hit(const Scene& s, const float o[3], const float d[3], float tnear, float tfar, uint64_t skip = ~0ull) {
     const float dd = dot3(d, d);
     for (uint64_t k = 0; k < s.n_pairs; ++k) {
+        if (k == skip) continue;
         const uint32_t a = s.indices[2 * k], b = s.indices[2 * k + 1];
         float t, u;
         if (flat_curve(o, d, dd, s.xyz + 3 * size_t(a), s.radius[a], s.xyz + 3 * size_t(b), s.radius[b], tnear, tfar,
@@ -181,7 +183,7 @@ void oracle_trace_any(const float* xyz, const float* radius, const uint32_t* ind
 void oracle_raytrace_frame(const float* xyz, const float* radius, const float* tangents, const uint32_t* indices,
                            uint64_t n_indices, const float* plane, const float* light, const float* diffuse,
                            float exponent, int method, int shadows_on, float ao_radius, uint32_t seed,
-                           uint32_t frame, uint32_t width, uint32_t height, double* accum) {
+                           uint32_t frame, uint32_t width, uint32_t height, int ignore_self, double* accum) {
     const Scene s{xyz, radius, indices, n_indices / 2};
     const float* px = plane;
     const float* py = plane + 3;
@@ -201,13 +203,14 @@ void oracle_raytrace_frame(const float* xyz, const float* radius, const float* t
             if (h.hit) {
                 float P[3];
                 for (int c = 0; c < 3; ++c) P[c] = eye[c] + d[c] * h.t;
+                const uint64_t skip = ignore_self ? uint64_t(h.seg) : ~0ull;
                 // light_shading (ray_tracer.cc:116-140): the shadow ray's
                 // direction is the jittered light position itself
                 float sd[3];
                 for (int c = 0; c < 3; ++c) sd[c] = light[c] + sample(draw(seed, pix, frame, 2 + c), -16.0f, 16.0f);
                 float L[3] = {1.0f, 1.0f, 1.0f};
                 if (method != 3) {
-                    const bool occluded = any_hit(s, P, sd, kEpsilon, INFINITY);
+                    const bool occluded = any_hit(s, P, sd, kEpsilon, INFINITY, skip);
                     if (!occluded || !shadows_on) {
                         if (method == 0) {
                             const uint32_t a = indices[2 * size_t(h.seg)], b = indices[2 * size_t(h.seg) + 1];
@@ -236,7 +239,7 @@ void oracle_raytrace_frame(const float* xyz, const float* radius, const float* t
                     const float len = std::sqrt(dot3(rd, rd));
                     // every hit within ao_radius has t <= ao_radius / |rd|
                     const float t_max = len > 0.0f ? (ao_radius / len) * 1.001f + kEpsilon : INFINITY;
-                    const Hit a = closest(s, P, rd, kEpsilon, t_max);
+                    const Hit a = closest(s, P, rd, kEpsilon, t_max, skip);
                     float ao = 2.0f;
                     if (a.hit) {
                         float q[3];

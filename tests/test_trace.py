@@ -146,6 +146,27 @@ def test_viewing_plane_closed_forms():
     assert np.allclose(X, np.cross([0, 1, 0], fwd), atol=1e-6)  # left = up x forward
 
 
+def test_oracle_ignore_self_removes_self_hits():
+    """An AO ray from a point on an isolated segment can hit only that
+    segment: with ignore_self every AO sample is unoccluded (scores 2)."""
+    from vkhr_amd.trace import plane_array, viewing_plane
+    xyz = np.array([[-0.5, 0.0, 0.0], [0.5, 0.0, 0.0]], np.float32)
+    rad = np.full(2, 0.05, np.float32)
+    idx = np.array([0, 1], np.uint32)
+    tan = np.array([[1, 0, 0], [1, 0, 0]], np.float32)
+    pl = plane_array(viewing_plane((0.0, 0.0, 2.0), (0, 0, 0), (0, 1, 0), math.radians(20.0), 16, 16))
+    kw = dict(method=3, ao_radius=10.0, width=16, height=16)
+    with_self = sum(OT.raytrace_frame(xyz, rad, tan, idx, pl, (2, 4, 2), (1, 1, 1), (1, 1, 1), frame=f, **kw)
+                    for f in range(16))
+    without = sum(OT.raytrace_frame(xyz, rad, tan, idx, pl, (2, 4, 2), (1, 1, 1), (1, 1, 1), frame=f,
+                                    ignore_self=True, **kw) for f in range(16))
+    # per sample: 1 a miss, 2 a hit whose AO ray is clear, 0 one that is
+    # occluded; with ignore_self nothing can occlude, so no sum drops below 16
+    assert (without[..., 0] > 16.0).any()
+    assert without.min() >= 16.0
+    assert with_self.min() < 16.0
+
+
 def test_oracle_frame_scores():
     """AO-only samples score 0 or 2 on the hair, 1 on the background; direct
     shadows 0 or 1."""
@@ -245,16 +266,19 @@ def test_frame_matches_the_oracle(device):
     scene = synth.make_scene(OV.aabb(xyz), W, H)
     light = params.light(scene)
     lo, li = np.array(light.origin[:], np.float32), np.array(light.intensity[:3], np.float32)
-    for method in range(4):
+    for method, ignore_self in [(m, False) for m in range(4)] + [(0, True), (3, True)]:
         acc = torch.zeros((H, W, 3), dtype=torch.float64, device=device)
         ref = np.zeros((H, W, 3))
         for frame in range(2):
-            bvh.raytrace(gt, gi, plane, light, W, H, acc, method=method, ao_radius=0.08, seed=5, frame=frame)
+            bvh.raytrace(gt, gi, plane, light, W, H, acc, method=method, ao_radius=0.08, seed=5, frame=frame,
+                         ignore_self=ignore_self)
             OT.raytrace_frame(xyz, rad, tan, idx, plane_array(plane), lo, li, (0.32, 0.228, 0.128), method=method,
-                              ao_radius=0.08, seed=5, frame=frame, width=W, height=H, accum=ref)
+                              ao_radius=0.08, seed=5, frame=frame, width=W, height=H, accum=ref,
+                              ignore_self=ignore_self)
         got = acc.cpu().numpy()
         err = np.abs(got - ref).max()
-        print(f"method {method}: max abs difference {err:.3g}, {np.count_nonzero(ref != 2.0)} non-background values")
+        print(f"method {method} (ignore_self {ignore_self}): max abs difference {err:.3g}, "
+              f"{np.count_nonzero(ref != 2.0)} non-background values")
         # methods 1-3 use only correctly rounded operations; the shaded one a float power
         assert err <= (1e-5 if method == 0 else 0.0)
         assert (ref != 2.0).any() and np.isfinite(got).all()

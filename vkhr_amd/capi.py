@@ -109,7 +109,7 @@ class ViewingPlane(C.Structure):
 class TraceParams(C.Structure):
     _fields_ = [("method", C.c_uint32), ("shadows_on", C.c_int32), ("ao_radius", C.c_float),
                 ("exponent", C.c_float), ("diffuse", C.c_float * 3), ("seed", C.c_uint32), ("frame", C.c_uint32),
-                ("reserved_", C.c_uint32)]
+                ("ignore_self", C.c_int32)]
 
 
 P = C.c_void_p

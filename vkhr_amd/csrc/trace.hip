@@ -103,15 +103,16 @@ __device__ __forceinline__ uint32_t parent_of(const TraceBvh& b, uint32_t x) {
 // Closest (kAny = false) or any (kAny = true) hit of the ray in [tnear, tfar];
 // closest ties go to the lowest segment index.
 template <bool kAny>
-__device__ Hit trace(const TraceBvh& b, float3 o, float3 d, float tnear, float tfar) {
+__device__ Hit trace(const TraceBvh& b, float3 o, float3 d, float tnear, float tfar, uint32_t skip = kNone) {
     Hit best{tfar, 0.0f, kNone, kNone};
     if (b.n == 0u) return best;
     const float dd = dot3(d, d);
     const float3 inv = safe_inverse(d);
     auto test_leaf = [&](uint32_t leaf) -> bool {
         float t, u;
-        if (!flat_curve(o, d, dd, b.prim_a[leaf], b.prim_b[leaf], tnear, best.t, &t, &u)) return false;
         const uint32_t seg = b.prim_seg[leaf];
+        if (seg == skip) return false;
+        if (!flat_curve(o, d, dd, b.prim_a[leaf], b.prim_b[leaf], tnear, best.t, &t, &u)) return false;
         if (best.seg == kNone || t < best.t || (t == best.t && seg < best.seg)) best = Hit{t, u, seg, leaf};
         return true;
     };
@@ -383,7 +384,7 @@ struct FrameArgs {
     float light[6];
     float diffuse[3];
     float exponent, ao_radius;
-    int method, shadows_on;
+    int method, shadows_on, ignore_self;
     uint32_t seed, frame, width, height;
     double* accum;
 };
@@ -428,13 +429,14 @@ __global__ void __launch_bounds__(256) frame_kernel(FrameArgs a) {
     const Hit h = trace<false>(a.b, eye, d, 0.0f, INFINITY);
     if (h.seg != kNone) {
         const float3 P = make_float3(eye.x + d.x * h.t, eye.y + d.y * h.t, eye.z + d.z * h.t);
+        const uint32_t skip = a.ignore_self ? h.seg : kNone;  // secondary rays may skip their own segment
         float L[3] = {1.0f, 1.0f, 1.0f};
         if (a.method != 3) {
             // the shadow ray's direction is the jittered light position itself
             const float3 sd = make_float3(a.light[0] + sample(a.seed, pix, a.frame, 2, -16.0f, 16.0f),
                                           a.light[1] + sample(a.seed, pix, a.frame, 3, -16.0f, 16.0f),
                                           a.light[2] + sample(a.seed, pix, a.frame, 4, -16.0f, 16.0f));
-            const bool occluded = trace<true>(a.b, P, sd, kEpsilon, INFINITY).seg != kNone;
+            const bool occluded = trace<true>(a.b, P, sd, kEpsilon, INFINITY, skip).seg != kNone;
             if (!occluded || !a.shadows_on) {
                 if (a.method == 0) {
                     const uint32_t ia = a.indices[2ull * h.seg], ib = a.indices[2ull * h.seg + 1];
@@ -463,7 +465,7 @@ __global__ void __launch_bounds__(256) frame_kernel(FrameArgs a) {
                                           sample(a.seed, pix, a.frame, 7, -1.0f, 1.0f));
             const float len = sqrtf(dot3(rd, rd));
             const float t_max = len > 0.0f ? (a.ao_radius / len) * 1.001f + kEpsilon : INFINITY;
-            const Hit o = trace<false>(a.b, P, rd, kEpsilon, t_max);
+            const Hit o = trace<false>(a.b, P, rd, kEpsilon, t_max, skip);
             float ao = 2.0f;
             if (o.seg != kNone) {
                 const float3 q = make_float3((P.x + rd.x * o.t) - P.x, (P.y + rd.y * o.t) - P.y,
@@ -711,6 +713,7 @@ extern "C" int vkhr_raytrace(const vkhr_bvh* b, const float* tangents, const uin
     a.ao_radius = params->ao_radius;
     a.method = int(params->method);
     a.shadows_on = params->shadows_on;
+    a.ignore_self = params->ignore_self ? 1 : 0;
     a.seed = params->seed;
     a.frame = params->frame;
     a.width = width;

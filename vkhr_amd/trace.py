@@ -78,12 +78,13 @@ class Bvh:
 
     def raytrace(self, tangents, indices, plane: capi.ViewingPlane, light: capi.Light, width: int, height: int,
                  accum: torch.Tensor, method: int = 0, shadows_on: bool = True, ao_radius: float = 2.5,
-                 exponent: float = 50.0, diffuse=(0.32, 0.228, 0.128), seed: int = 0, frame: int = 0):
+                 exponent: float = 50.0, diffuse=(0.32, 0.228, 0.128), seed: int = 0, frame: int = 0,
+                 ignore_self: bool = False):
         """Adds one sample per pixel (Raytracer::draw) to accum (H, W, 3) float64."""
         p = capi.TraceParams()
         p.method, p.shadows_on, p.ao_radius, p.exponent = method, 1 if shadows_on else 0, ao_radius, exponent
         p.diffuse[:] = list(diffuse)
-        p.seed, p.frame = seed, frame
+        p.seed, p.frame, p.ignore_self = seed, frame, 1 if ignore_self else 0
         call("vkhr_raytrace", self.handle, ptr(tangents), ptr(indices), C.byref(plane), C.byref(light), C.byref(p),
              width, height, ptr(accum), capi.stream_of())
 
