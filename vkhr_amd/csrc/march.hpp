@@ -38,6 +38,8 @@ struct RaymarchArgs {
     double ao_scaling;         // LAO tap offset in texels per kernel unit
     double ao_clamp, ao_exponent;
     float step_size;           // 1 / raycast_steps, in float as the shader
+    float steps_per_unit;      // raycast_steps when it is a power of two (the loop parameter is then
+                               // exactly i * step_size), else 0
     int shading_model, adsm_on;
     int nx, ny, nz;
     int width, height;
@@ -159,6 +161,30 @@ __device__ __forceinline__ void support_interval(const RaymarchArgs& a, const do
     *hi = h;
 }
 
+// Parameter t where the ray x(t) = s + t / inv leaves brick b of cell c1
+// (cells 8b .. 8b+7 cover x in [8b-1, 8b+7)) along one axis, with the plane
+// taken 1e-6 texels inside the brick (float64 rounding of a sample's
+// position cannot cross it); +inf when the ray is parallel to the planes.
+__device__ __forceinline__ double brick_exit(uint32_t c1, double s, double inv) {
+    if (inv == 0.0) return INFINITY;
+    const double b = double(c1 >> kBrickShift);
+    const double plane = inv > 0.0 ? 8.0 * b + 7.0 - 1e-6 : 8.0 * b - 1.0 + 1e-6;
+    return (plane - s) * inv;
+}
+
+// The index to continue at after sample i read an empty brick (cell c):
+// the last index whose t does not pass the brick's exit (nor t_cap); every
+// sample in between lies at least one step inside the brick and reads 0.
+// steps is raycast_steps (a power of two: t_j = j / steps exactly); s is
+// the ray's start and inv the reciprocals of its direction (texel units).
+__device__ __forceinline__ int skip_empty_brick(int i, uint3 c, double s0, double s1, double s2, double inv0,
+                                                double inv1, double inv2, double t_cap, double steps) {
+    double te = fmin(fmin(brick_exit(c.x, s0, inv0), brick_exit(c.y, s1, inv1)), brick_exit(c.z, s2, inv2));
+    te = fmin(te, t_cap);
+    const double j = floor(te * steps);
+    return j > double(i + 1) ? int(j) : i + 1;
+}
+
 // One pixel of the raymarch pass (volume.frag:31-102).  Every lane of a
 // wave must call it (the optional PPLL insert ballots the wave); lanes
 // outside the image take no fragment.  occ: the LDS copy of the brick
@@ -210,15 +236,30 @@ __device__ __forceinline__ void raymarch_pixel(const RaymarchArgs& a, int px, in
         const DensityVolume vol = a.vol;
         const double iso = a.iso;
         const float step = a.step_size;
+        // Empty bricks are stepped over when the loop parameter is exactly
+        // i * step: their samples read 0 and change neither the sum nor the
+        // surface choices; the jump stops short of t_hi and of t_depth (where
+        // the depth test may end the march), so the march ends at the same
+        // sample as without it.
+        const bool jump = occ.use && a.steps_per_unit > 0.0f;
+        const double steps = a.steps_per_unit;
+        const double inv0 = jump && dx[0] != 0.0 ? 1.0 / dx[0] : 0.0;
+        const double inv1 = jump && dx[1] != 0.0 ? 1.0 / dx[1] : 0.0;
+        const double inv2 = jump && dx[2] != 0.0 ? 1.0 / dx[2] : 0.0;
+        const double t_cap = fmin(t_hi, t_depth);
         int i = 0;
-        for (float t = 0.0f; t < 1.0f; t += step, ++i) {
+        for (float t = 0.0f; t < 1.0f;) {
             const double td = double(t);
             // depth test: db < z/w  <=>  db*w < z  (w > 0 in front of the eye);
             // evaluated as a predicate (no branch around the products)
             const bool behind = (td >= t_depth) & (db * fma(td, dw, ws) < fma(td, dz, zs));
             if (behind | (td > t_hi)) break;
+            bool empty = false;
+            uint3 cell;
             const float d = march_tap<kF32>(vol, fma(td, dx[0], xs[0]), fma(td, dx[1], xs[1]), fma(td, dx[2], xs[2]),
-                                            occ);
+                                            occ, &empty, &cell);
+            int next = i + 1;
+            if (jump && empty) next = skip_empty_brick(i, cell, xs[0], xs[1], xs[2], inv0, inv1, inv2, t_cap, steps);
             acc += double(d);
             // surface / entry bookkeeping as selects
             const bool nonzero = d != 0.0f;
@@ -229,6 +270,8 @@ __device__ __forceinline__ void raymarch_pixel(const RaymarchArgs& a, int px, in
             const bool first = nonzero & (entry < 0);
             entry = first ? i : entry;
             t_entry = first ? t : t_entry;
+            i = next;
+            t = jump ? float(i) * step : t + step;
         }
         info = make_int4(2, surf, entry, found ? 1 : 0);  // marched; 1 once a fragment is written
         if (a.record) {
@@ -271,12 +314,26 @@ __device__ __forceinline__ void raymarch_pixel(const RaymarchArgs& a, int px, in
                 support_interval(a, sx, xl, &t_lo, &t_hi);
                 const DensityVolume vol = a.vol;
                 const float step = a.step_size;
-                for (float t = 0.0f; t < 1.0f; t += step) {
+                const bool jump = occ.use && a.steps_per_unit > 0.0f;
+                const double steps = a.steps_per_unit;
+                const double inv0 = jump && xl[0] != 0.0 ? 1.0 / xl[0] : 0.0;
+                const double inv1 = jump && xl[1] != 0.0 ? 1.0 / xl[1] : 0.0;
+                const double inv2 = jump && xl[2] != 0.0 ? 1.0 / xl[2] : 0.0;
+                int j = 0;
+                for (float t = 0.0f; t < 1.0f;) {
                     const double td = double(t);
                     if (td > t_hi) break;
-                    if (td < t_lo) continue;
-                    strands += double(march_tap<kF32>(vol, fma(td, xl[0], sx[0]), fma(td, xl[1], sx[1]),
-                                                      fma(td, xl[2], sx[2]), occ));
+                    int next = j + 1;
+                    if (td >= t_lo) {
+                        bool empty = false;
+                        uint3 cell;
+                        strands += double(march_tap<kF32>(vol, fma(td, xl[0], sx[0]), fma(td, xl[1], sx[1]),
+                                                          fma(td, xl[2], sx[2]), occ, &empty, &cell));
+                        if (jump && empty)
+                            next = skip_empty_brick(j, cell, sx[0], sx[1], sx[2], inv0, inv1, inv2, t_hi, steps);
+                    }
+                    j = next;
+                    t = jump ? float(j) * step : t + step;
                 }
                 adsm = pow(1.0 - a.alpha, strands * a.adsm_k);
                 occlusion *= adsm;

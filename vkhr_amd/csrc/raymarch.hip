@@ -141,6 +141,13 @@ int make_raymarch_args(const vkhr_volume* vol, const vkhr_camera* cam, const vkh
     a->ao_clamp = rp->ao_clamp;
     a->ao_exponent = rp->ao_exponent;
     a->step_size = 1.0f / rp->raycast_steps;
+    {
+        // a power of two up to 2^20: the float loop parameter is then i / steps exactly
+        const float s = rp->raycast_steps;
+        int e = 0;
+        const float m = std::frexp(s, &e);
+        a->steps_per_unit = (m == 0.5f && s >= 1.0f && s <= 1048576.0f) ? s : 0.0f;
+    }
     a->shading_model = rp->shading_model;
     a->adsm_on = rp->adsm_on;
     a->width = int(width);

@@ -98,17 +98,24 @@ __device__ __forceinline__ float density_at(const DensityVolume& v, double x, do
 // use 24-bit multiplies: the callers bound n + 1 and (ny + 1)(nz + 1)
 // below 2^24 (and the cell count below 2^32).  kF32 selects the float
 // volume.
+// With empty/cell: *empty is set when the cell is inside the volume but its
+// brick is empty (the sample reads 0), and cell[] receives the cell (i+1,
+// j+1, k+1).
 template <bool kF32>
-__device__ __forceinline__ float march_tap(const DensityVolume& v, double x, double y, double z, OccupancyView occ) {
+__device__ __forceinline__ float march_tap(const DensityVolume& v, double x, double y, double z, OccupancyView occ,
+                                           bool* empty = nullptr, uint3* cell = nullptr) {
     const double fx = floor(x), fy = floor(y), fz = floor(z);
     const int ix = int(fx), iy = int(fy), iz = int(fz);
     const uint32_t cx1 = uint32_t(ix + 1), cy1 = uint32_t(iy + 1), cz1 = uint32_t(iz + 1);
     bool live = (cx1 <= uint32_t(v.nx)) & (cy1 <= uint32_t(v.ny)) & (cz1 <= uint32_t(v.nz));
+    if (cell) *cell = make_uint3(cx1, cy1, cz1);
     if (occ.use) {
         const uint32_t b = (cx1 >> kBrickShift) +
                            __umul24(v.bx, (cy1 >> kBrickShift) + __umul24(v.by, cz1 >> kBrickShift));
         const uint32_t word = occ.lds[live ? (b >> 5) : 0u];
-        live = live & (((word >> (b & 31u)) & 1u) != 0u);
+        const bool occupied = ((word >> (b & 31u)) & 1u) != 0u;
+        if (empty) *empty = live & !occupied;
+        live = live & occupied;
     }
     if (!live) return 0.0f;
     const float wx = float(x - fx), wy = float(y - fy), wz = float(z - fz);
