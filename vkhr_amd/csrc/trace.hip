@@ -332,7 +332,30 @@ radix_tree_kernel(const uint32_t* __restrict__ codes, int n, float4* nodes, uint
 }
 
 // Bottom-up boxes: each leaf climbs; at every node the second arrival
-// (atomic counter) takes the union of the two child boxes upwards.
+// (atomic counter) takes the union of the two child boxes upwards.  The
+// child boxes pass between threads through device-scope atomic stores and
+// loads (coherent across the XCDs' L2s without flushing them); the store is
+// complete before the arrival is counted.  A device-wide fence instead
+// writes back and invalidates the whole L2 at each step: 11.6 ms for 1.6 M
+// leaves.
+__device__ __forceinline__ void store_box(float4* slo, float4* shi, float4 lo, float4 hi) {
+    float* a = reinterpret_cast<float*>(slo);
+    float* b = reinterpret_cast<float*>(shi);
+    __hip_atomic_store(a + 0, lo.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(a + 1, lo.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(a + 2, lo.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(b + 0, hi.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(b + 1, hi.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(b + 2, hi.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ float3 load3(const float4* p) {
+    const float* a = reinterpret_cast<const float*>(p);
+    return make_float3(__hip_atomic_load(a + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                       __hip_atomic_load(a + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                       __hip_atomic_load(a + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
 __global__ void __launch_bounds__(256)
 refit_kernel(int n, const float4* __restrict__ leaf_lo, const float4* __restrict__ leaf_hi, float4* nodes,
              const uint32_t* __restrict__ parent, uint32_t* arrivals) {
@@ -343,15 +366,12 @@ refit_kernel(int n, const float4* __restrict__ leaf_lo, const float4* __restrict
     uint32_t p = parent[uint32_t(n - 1) + uint32_t(l)];
     for (;;) {
         float4* nd = nodes + 4ull * p;
-        const bool is_left = __float_as_uint(nd[0].w) == child;
-        float4* slo = is_left ? &nd[0] : &nd[2];
-        float4* shi = is_left ? &nd[1] : &nd[3];
-        slo->x = lo.x; slo->y = lo.y; slo->z = lo.z;
-        shi->x = hi.x; shi->y = hi.y; shi->z = hi.z;
-        __threadfence();
-        if (atomicAdd(&arrivals[p], 1u) == 0u) return;  // the sibling's box is not in yet
-        __threadfence();
-        const float4 a0 = nd[0], a1 = nd[1], b0 = nd[2], b1 = nd[3];
+        const bool is_left = __float_as_uint(nd[0].w) == child;  // written by the tree kernel
+        store_box(is_left ? &nd[0] : &nd[2], is_left ? &nd[1] : &nd[3], lo, hi);
+        __builtin_amdgcn_s_waitcnt(0);  // the box stores are complete before the arrival
+        if (__hip_atomic_fetch_add(&arrivals[p], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+            return;  // the sibling's box is not in yet
+        const float3 a0 = load3(&nd[0]), a1 = load3(&nd[1]), b0 = load3(&nd[2]), b1 = load3(&nd[3]);
         lo = make_float4(fminf(a0.x, b0.x), fminf(a0.y, b0.y), fminf(a0.z, b0.z), 0.0f);
         hi = make_float4(fmaxf(a1.x, b1.x), fmaxf(a1.y, b1.y), fmaxf(a1.z, b1.z), 0.0f);
         if (p == 0u) return;
