@@ -218,6 +218,14 @@ private:
     mutable DeviceBuffer d_vertices_, d_tangents_, d_thickness_, d_indices_;
 };
 
+// vkhr::Renderer::Type (include/vkhr/renderer.hh:13-18): the values of
+// vkhr_render_params::renderer.
+struct Renderer {
+    enum Type : int { Rasterizer = 0, Ray_Tracer = 1, Raymarcher = 2, Hybrid_LoD = 3 };
+};
+
+class LinkedList;
+
 // The raymarcher pass (vulkan::Volume::draw + volume.frag) for one style.
 class Raymarcher {
 public:
@@ -226,6 +234,12 @@ public:
     void draw(const HairStyle::Volume& volume, const vkhr_camera& camera, const vkhr_light& light,
               const vkhr_strand_params& strand, const vkhr_render_params& params, uint32_t width, uint32_t height,
               float* rgba, float* depth, vkhr_stream stream = nullptr) const;
+    // Links one node per raymarched fragment into the PPLL (the raymarch
+    // subpass of Rasterizer::draw_color, rasterizer.cc:254-258, with
+    // volume.frag:95-100); depth_buffer (W*H, nullable) is the depth test input.
+    void draw(const HairStyle::Volume& volume, const vkhr_camera& camera, const vkhr_light& light,
+              const vkhr_strand_params& strand, const vkhr_render_params& params, LinkedList& ppll,
+              const float* depth_buffer = nullptr, vkhr_stream stream = nullptr) const;
 };
 
 // vulkan::LinkedList (linked_list.hh:16-45): PPLL storage, clear, resolve.
@@ -253,6 +267,36 @@ public:
     void draw(const HairStyle& style, const HairStyle::Volume& volume, const vkhr_camera& camera,
               const vkhr_light& light, const vkhr_strand_params& strand, const vkhr_render_params& params,
               const float* shadow_map, uint32_t shadow_size, LinkedList& ppll, vkhr_stream stream = nullptr) const;
+};
+
+// The colour pass of vkhr::Rasterizer for one style, with the hybrid LoD.
+// update() takes the level of detail from the camera distance: glm::smoothstep
+// of the two LoD distances in float (rasterizer.cc:162-168).  draw_color()
+// clears the PPLL, draws the strands when the rasterizer is enabled at that
+// level, raymarches into the same list when the raymarcher is, and resolves
+// (rasterizer.cc:232-282).  The enable rules are Interface::rasterizer_enabled
+// and raymarcher_enabled (interface.cc:534-540): the rasterizer draws for
+// Renderer::Rasterizer, or for Hybrid_LoD below level 1; the raymarcher draws
+// for Renderer::Raymarcher, or for Hybrid_LoD above level 0.  Each pass
+// scales its own coverage by lod() (scheme.glsl:7-11): strands by 1 - lod
+// (strand.frag:42), the raymarch by lod (volume.frag:48).
+class HybridRenderer {
+public:
+    void update(const vkhr_camera& camera, const vkhr_render_params& params);
+    bool rasterizer_enabled(const vkhr_render_params& params) const;
+    bool raymarcher_enabled(const vkhr_render_params& params) const;
+    // The colour pass into out_rgba (W*H float4, the PPLL's size); background
+    // as vkhr_ppll_resolve (nullable: opaque white).
+    void draw_color(const HairStyle& style, const HairStyle::Volume& volume, const vkhr_camera& camera,
+                    const vkhr_light& light, const vkhr_strand_params& strand, const vkhr_render_params& params,
+                    const float* shadow_map, uint32_t shadow_size, LinkedList& ppll, float* out_rgba,
+                    const float* background = nullptr, vkhr_stream stream = nullptr) const;
+
+    float level_of_detail = 0.0f;
+
+private:
+    StrandRasterizer strands_;
+    Raymarcher raymarcher_;
 };
 
 }  // namespace vkhr

@@ -3,7 +3,9 @@ Raymarcher, StrandRasterizer, LinkedList) driven end to end by the C++
 program vkhr_amd/host_check, checked against the oracles: AABB and voxel
 counts bit-exact, normalized densities exact, the raymarch within 1e-4 of
 float64, the shadow map texel-exact but boundary ties, the resolve image a
-valid blend over the white background."""
+valid blend over the white background; HybridRenderer's colour pass at three
+levels of detail (both passes, the raymarcher alone, the rasterizer alone)
+against the oracles' fragments composed into one list per pixel."""
 import ctypes as C
 import subprocess
 from pathlib import Path
@@ -19,6 +21,8 @@ from vkhr_amd import params, synth
 pytestmark = pytest.mark.gpu
 ROOT = Path(__file__).resolve().parent.parent
 EXE = ROOT / "vkhr_amd" / "host_check"
+NULL = 0xFFFFFFFF
+RESOLVE_TOL = 2e-3
 
 
 def test_host_surface_end_to_end(device, tmp_path):
@@ -32,6 +36,11 @@ def test_host_surface_end_to_end(device, tmp_path):
     blob = np.array([s.n_strands, s.n_vertices, n, w, h, sm], np.uint32).tobytes()
     blob += s.segments.astype(np.uint16).tobytes() + s.vertices.astype(np.float32).tobytes()
     blob += bytes(cam) + bytes(light) + bytes(sp) + bytes(rp) + light_vp.tobytes()
+    # hybrid LoD distance pairs: between (lod 0.156), camera past both (lod 1),
+    # camera short of both (lod 0)
+    d = float(cam.look_at_distance)
+    hybrid = [(0.75 * d, 1.75 * d), (0.25 * d, 0.5 * d), (2.0 * d, 4.0 * d)]
+    blob += np.array([len(hybrid)], np.uint32).tobytes() + np.asarray(hybrid, np.float32).tobytes()
     (tmp_path / "in.bin").write_bytes(blob)
     r = subprocess.run([str(EXE), str(tmp_path / "in.bin"), str(tmp_path)], capture_output=True, text=True,
                        timeout=120)
@@ -63,3 +72,58 @@ def test_host_surface_end_to_end(device, tmp_path):
     assert np.isfinite(res).all() and res.min() >= 0.0 and res.max() <= 1.0 + 1e-6
     assert (res == 1.0).all(axis=2).mean() > 0.2       # the white background shows around the hair
     assert (res < 0.99).any(axis=2).mean() > 0.05      # and the strands darken it
+    strand_nodes = int(r.stdout.split()[-2])
+
+    # HybridRenderer (Rasterizer::update + draw_color): the full frame against
+    # the oracles' fragments composed into per-pixel lists, on the pixels whose
+    # lists fit the k = 16 buffer (there the resolve is a depth-sorted blend,
+    # independent of insertion order)
+    info = [(tmp_path / f"hybrid{k}.txt").read_text().split() for k in range(3)]
+    lods = [float(i[0]) for i in info]
+    assert abs(lods[0] - 0.15625) < 1e-5 and lods[1] == 1.0 and lods[2] == 0.0
+    assert [i[1:3] for i in info] == [["1", "1"], ["0", "1"], ["1", "0"]]
+    nodes = [int(i[3]) for i in info]
+    thick = np.full(s.n_vertices, 0.042, np.float32)
+    thick[np.cumsum(s.segments.astype(np.int64) + 1) - 1] = 0.0      # generate_thickness(0.042): 0 at the tips
+    for k, (mag, mnf) in enumerate(hybrid[:2]):
+        hp = params.render(raycast_steps=128.0, renderer=3, lod_magnified_distance=mag, lod_minified_distance=mnf)
+        o_r = OR.raymarch(dens, tangents, (n, n, n), cam, light, sp, hp, w, h)
+        rm = o_r["info"][:, 0] == 1
+        frag_pix, frag_col, frag_depth = [np.nonzero(rm)[0]], [o_r["rgba"][rm]], [o_r["depth"][rm]]
+        if k == 0:
+            o_s = ORs.raster(s.vertices, tan, thick, idx, cam, light, sp, hp, density=dens, res=(n, n, n),
+                             shadow_map=smap)
+            kept = o_s["kept"]
+            strand_per_pixel = np.bincount(o_s["pixel"][kept].astype(np.int64), minlength=w * h)
+            frag_pix.insert(0, o_s["pixel"][kept].astype(np.int64))
+            frag_col.insert(0, o_s["rgba"][kept])
+            frag_depth.insert(0, o_s["depth"][kept])
+        pix_all, col_all, dep_all = (np.concatenate(x) for x in (frag_pix, frag_col, frag_depth))
+        per_pixel = np.bincount(pix_all, minlength=w * h)
+        # node counter: at lod 1 the gated-off rasterizer reserves none, so it
+        # holds the raymarch fragments; between, the rasterizer's reservations
+        # (as many as in the rasterizer-only pass) plus those
+        rm_nodes = nodes[1] if k == 1 else nodes[0] - strand_nodes
+        assert abs(rm_nodes - int(rm.sum())) <= max(2, 1e-3 * rm.sum()), (k, nodes, strand_nodes, rm.sum())
+        n_frag = len(pix_all)
+        node_words = np.zeros((n_frag, 3), np.uint32)
+        node_words[:, 0] = ORs.pack_unorm4x8(col_all)
+        node_words[:, 1] = dep_all.astype(np.float32).view(np.uint32)
+        heads = np.full(w * h, NULL, np.uint32)
+        for j in range(n_frag):                                   # link in list order, newest at the head
+            node_words[j, 2] = heads[pix_all[j]]
+            heads[pix_all[j]] = j
+        want = ORs.resolve(heads, node_words.reshape(-1), w, h)
+        got = np.frombuffer((tmp_path / f"hybrid{k}.bin").read_bytes(), np.float32).reshape(h, w, 4)
+        fits = (per_pixel <= 16).reshape(h, w)
+        err = np.abs(got.astype(np.float64) - want)[fits]
+        # a node channel whose float64 value lies within the fragment tolerance
+        # of a packUnorm4x8 half step may round one step the other way
+        assert err.max() <= 1.0 / 255.0 + 1e-6, err.max()
+        assert (err.max(axis=1) > RESOLVE_TOL).mean() < 2e-3
+        assert (per_pixel > 0).sum() > 1000
+    # lod 0: the raymarcher is off; the frame is the rasterizer-only frame
+    assert nodes[2] == strand_nodes
+    h2 = np.frombuffer((tmp_path / "hybrid2.bin").read_bytes(), np.float32).reshape(h, w, 4)
+    fits = (strand_per_pixel <= 16).reshape(h, w)
+    assert np.abs(h2 - res)[fits].max() <= RESOLVE_TOL

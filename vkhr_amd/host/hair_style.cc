@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <cstring>
 
 namespace vkhr {
@@ -237,6 +238,15 @@ void Raymarcher::draw(const HairStyle::Volume& volume, const vkhr_camera& camera
           "vkhr_raymarch");
 }
 
+void Raymarcher::draw(const HairStyle::Volume& volume, const vkhr_camera& camera, const vkhr_light& light,
+                      const vkhr_strand_params& strand, const vkhr_render_params& params, LinkedList& ppll,
+                      const float* depth_buffer, vkhr_stream stream) const {
+    vkhr_volume v = volume.view();
+    const vkhr_ppll& p = ppll.view();
+    check(vkhr_raymarch_ppll(&v, &camera, &light, &strand, &params, p.width, p.height, depth_buffer, &p, stream),
+          "vkhr_raymarch_ppll");
+}
+
 LinkedList::LinkedList(uint32_t width, uint32_t height, std::size_t node_count)
     : heads_(size_t(width) * height * sizeof(uint32_t)),
       nodes_(node_count * NodeSize),
@@ -281,6 +291,36 @@ void StrandRasterizer::draw(const HairStyle& style, const HairStyle::Volume& vol
                               &strand, &params, &v, shadow_map, shadow_size, shadow_size, nullptr, &ppll.view(),
                               nullptr, stream),
           "vkhr_raster_strands");
+}
+
+void HybridRenderer::update(const vkhr_camera& camera, const vkhr_render_params& params) {
+    // glm::smoothstep(edge0, edge1, x) in float
+    const float e0 = params.lod_magnified_distance, e1 = params.lod_minified_distance;
+    float t = (camera.look_at_distance - e0) / (e1 - e0);
+    t = std::min(std::max(t, 0.0f), 1.0f);
+    level_of_detail = t * t * (3.0f - 2.0f * t);
+}
+
+bool HybridRenderer::rasterizer_enabled(const vkhr_render_params& params) const {
+    return params.renderer == Renderer::Rasterizer ||
+           (params.renderer == Renderer::Hybrid_LoD && level_of_detail != 1.0f);
+}
+
+bool HybridRenderer::raymarcher_enabled(const vkhr_render_params& params) const {
+    return params.renderer == Renderer::Raymarcher ||
+           (params.renderer == Renderer::Hybrid_LoD && level_of_detail != 0.0f);
+}
+
+void HybridRenderer::draw_color(const HairStyle& style, const HairStyle::Volume& volume, const vkhr_camera& camera,
+                                const vkhr_light& light, const vkhr_strand_params& strand,
+                                const vkhr_render_params& params, const float* shadow_map, uint32_t shadow_size,
+                                LinkedList& ppll, float* out_rgba, const float* background,
+                                vkhr_stream stream) const {
+    ppll.clear(stream);
+    if (rasterizer_enabled(params))
+        strands_.draw(style, volume, camera, light, strand, params, shadow_map, shadow_size, ppll, stream);
+    if (raymarcher_enabled(params)) raymarcher_.draw(volume, camera, light, strand, params, ppll, nullptr, stream);
+    ppll.resolve(out_rgba, background, stream);
 }
 
 }  // namespace vkhr

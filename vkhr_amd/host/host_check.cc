@@ -10,7 +10,10 @@
 // then the raw vkhr_camera, vkhr_light, vkhr_strand_params and
 // vkhr_render_params blocks (the strand block's bounds are replaced by the
 // style's own AABB), then f32 light_vp[16] (unbiased light view-projection,
-// column-major) for the shadow-map pass.
+// column-major) for the shadow-map pass; optionally u32 n, then n pairs of
+// f32 (lod_magnified_distance, lod_minified_distance), each drawn as one
+// hybrid LoD colour pass into hybrid<k>.bin, with hybrid<k>.txt holding the
+// level of detail, the two enable flags and the PPLL nodes used.
 #include <hip/hip_runtime_api.h>
 
 #include <cstdio>
@@ -106,6 +109,31 @@ int main(int argc, char** argv) {
         rgba.download(img.data(), img.size() * 4);
         write(out + "/resolve.bin", img.data(), img.size() * 4);
         uint32_t used = ppll.node_count_used();
+
+        // hybrid colour passes (Rasterizer::update + draw_color), one per
+        // (magnified, minified) LoD distance pair of the optional tail
+        uint32_t n_hybrid = 0;
+        if (in.read(reinterpret_cast<char*>(&n_hybrid), sizeof(n_hybrid))) {
+            for (uint32_t k = 0; k < n_hybrid; ++k) {
+                float dist[2];
+                read(in, dist, 2);
+                vkhr_render_params hp = rp;
+                hp.renderer = vkhr::Renderer::Hybrid_LoD;
+                hp.lod_magnified_distance = dist[0];
+                hp.lod_minified_distance = dist[1];
+                vkhr::HybridRenderer hybrid;
+                hybrid.update(cam, hp);
+                hybrid.draw_color(style, volume, cam, light, sp, hp, shadow.as<float>(), S, ppll, rgba.as<float>());
+                rgba.download(img.data(), img.size() * 4);
+                const std::string tag = out + "/hybrid" + std::to_string(k);
+                write(tag + ".bin", img.data(), img.size() * 4);
+                const std::string info = std::to_string(hybrid.level_of_detail) + " " +
+                                         std::to_string(int(hybrid.rasterizer_enabled(hp))) + " " +
+                                         std::to_string(int(hybrid.raymarcher_enabled(hp))) + " " +
+                                         std::to_string(ppll.node_count_used()) + "\n";
+                write(tag + ".txt", info.data(), info.size());
+            }
+        }
         std::printf("host_check: %u strands, %u segments, %u nodes\n", style.get_strand_count(),
                     style.get_segment_count(), used);
         return 0;
