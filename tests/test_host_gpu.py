@@ -25,6 +25,20 @@ NULL = 0xFFFFFFFF
 RESOLVE_TOL = 2e-3
 
 
+def _order_free(pix, depth, w, h):
+    """Pixels whose lists fit the k = 16 buffer and hold no two equal float32
+    depths: there the resolve is a depth-sorted blend whatever the insertion
+    order (equal depths, e.g. the shared vertex of two segments of a strand,
+    blend in list order, which the insertion atomics leave open)."""
+    d32 = depth.astype(np.float32)
+    order = np.lexsort((d32, pix))
+    p, d = pix[order], d32[order]
+    tie = (p[1:] == p[:-1]) & (d[1:] == d[:-1])
+    tied = np.zeros(w * h, bool)
+    tied[p[1:][tie]] = True
+    return ((np.bincount(pix, minlength=w * h) <= 16) & ~tied).reshape(h, w)
+
+
 def test_host_surface_end_to_end(device, tmp_path):
     s = synth.hemisphere_style(2000, 16, seed=21)
     n, w, h, sm = 64, 128, 96, 256
@@ -94,12 +108,11 @@ def test_host_surface_end_to_end(device, tmp_path):
             o_s = ORs.raster(s.vertices, tan, thick, idx, cam, light, sp, hp, density=dens, res=(n, n, n),
                              shadow_map=smap)
             kept = o_s["kept"]
-            strand_per_pixel = np.bincount(o_s["pixel"][kept].astype(np.int64), minlength=w * h)
+            strand_fits = _order_free(o_s["pixel"][kept].astype(np.int64), o_s["depth"][kept], w, h)
             frag_pix.insert(0, o_s["pixel"][kept].astype(np.int64))
             frag_col.insert(0, o_s["rgba"][kept])
             frag_depth.insert(0, o_s["depth"][kept])
         pix_all, col_all, dep_all = (np.concatenate(x) for x in (frag_pix, frag_col, frag_depth))
-        per_pixel = np.bincount(pix_all, minlength=w * h)
         # node counter: at lod 1 the gated-off rasterizer reserves none, so it
         # holds the raymarch fragments; between, the rasterizer's reservations
         # (as many as in the rasterizer-only pass) plus those
@@ -115,15 +128,18 @@ def test_host_surface_end_to_end(device, tmp_path):
             heads[pix_all[j]] = j
         want = ORs.resolve(heads, node_words.reshape(-1), w, h)
         got = np.frombuffer((tmp_path / f"hybrid{k}.bin").read_bytes(), np.float32).reshape(h, w, 4)
-        fits = (per_pixel <= 16).reshape(h, w)
+        fits = _order_free(pix_all, dep_all, w, h)
         err = np.abs(got.astype(np.float64) - want)[fits]
         # a node channel whose float64 value lies within the fragment tolerance
         # of a packUnorm4x8 half step may round one step the other way
         assert err.max() <= 1.0 / 255.0 + 1e-6, err.max()
         assert (err.max(axis=1) > RESOLVE_TOL).mean() < 2e-3
-        assert (per_pixel > 0).sum() > 1000
-    # lod 0: the raymarcher is off; the frame is the rasterizer-only frame
+        assert len(np.unique(pix_all)) > 1000
+    # lod 0: the raymarcher is off (no raymarch node reserved), and where the
+    # blend is order free the frame is the rasterizer-only frame bit for bit
+    # (the same nodes: coverage x (1 - 0)); a pixel whose device list differs
+    # from the oracle's by a near-tie fragment may fall outside that rule
     assert nodes[2] == strand_nodes
     h2 = np.frombuffer((tmp_path / "hybrid2.bin").read_bytes(), np.float32).reshape(h, w, 4)
-    fits = (strand_per_pixel <= 16).reshape(h, w)
-    assert np.abs(h2 - res)[fits].max() <= RESOLVE_TOL
+    differs = np.any(h2 != res, axis=2)[strand_fits]
+    assert differs.sum() <= max(2, 1e-3 * strand_fits.sum()), differs.sum()
