@@ -162,6 +162,12 @@ public:
     void generate_tangents();                // hair_style.cc:171-194 (on the device)
     void generate_indices();                 // hair_style.cc:196-213 (on the device)
     void generate_bounding_box();            // hair_style.cc:215-234 (into the header, as the reference)
+    // The index pairs in the Morton order of the segment midpoints in the
+    // bounding box (vkhr_sort_segments; no counterpart in the reference): a
+    // permutation on which every pass gives the same results, with the
+    // strand passes faster for styles stored in random strand order.  After
+    // generate_indices and generate_bounding_box.
+    void sort_indices_spatially();
     AABB get_bounding_box() const;           // hair_style.cc:236-255 (from the header)
 
     Volume voxelize_vertices(std::size_t width, std::size_t height, std::size_t depth) const;

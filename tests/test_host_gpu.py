@@ -87,6 +87,10 @@ def test_host_surface_end_to_end(device, tmp_path):
     assert (res == 1.0).all(axis=2).mean() > 0.2       # the white background shows around the hair
     assert (res < 0.99).any(axis=2).mean() > 0.05      # and the strands darken it
     strand_nodes = int(r.stdout.split()[-2])
+    srt = np.frombuffer((tmp_path / "indices_sorted.bin").read_bytes(), np.uint32).reshape(-1, 2)
+    pairs = idx.reshape(-1, 2)
+    key = lambda p: np.sort(p[:, 0].astype(np.uint64) << np.uint64(32) | p[:, 1])
+    assert np.array_equal(key(srt), key(pairs)) and not np.array_equal(srt, pairs)
 
     # HybridRenderer (Rasterizer::update + draw_color): the full frame against
     # the oracles' fragments composed into per-pixel lists, on the pixels whose

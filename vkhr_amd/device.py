@@ -36,6 +36,16 @@ def generate_indices(offsets: torch.Tensor) -> torch.Tensor:
     return out[:n_idx]
 
 
+def sort_segments(xyz: torch.Tensor, indices: torch.Tensor, bounds: dict) -> torch.Tensor:
+    """The index pairs in the Morton order of the segment midpoints (a
+    permutation of the pairs, made once at load)."""
+    out = torch.empty_like(indices)
+    b = capi.make_aabb(bounds)
+    call("vkhr_sort_segments", ptr(xyz), xyz.shape[0], ptr(indices), indices.shape[0], C.byref(b), ptr(out),
+         stream_of())
+    return out
+
+
 def generate_tangents(xyz: torch.Tensor, offsets: torch.Tensor) -> torch.Tensor:
     out = torch.zeros_like(xyz)
     call("vkhr_generate_tangents", ptr(xyz), ptr(offsets), offsets.shape[0] - 1, ptr(out), stream_of())

@@ -120,6 +120,19 @@ void HairStyle::generate_indices() {
 
 
 
+void HairStyle::sort_indices_spatially() {
+    const AABB box = get_bounding_box();
+    const vkhr_aabb b{{box.origin.x, box.origin.y, box.origin.z}, box.radius, {box.size.x, box.size.y, box.size.z},
+                      box.volume};
+    const size_t n = indices.size();
+    DeviceBuffer out(n ? n * sizeof(uint32_t) : sizeof(uint32_t));
+    check(vkhr_sort_segments(device_vertices().as<float>(), get_vertex_count(), device_indices().as<uint32_t>(),
+                             uint32_t(n), &b, out.as<uint32_t>(), stream),
+          "vkhr_sort_segments");
+    if (n) out.download(indices.data(), n * sizeof(uint32_t));
+    d_indices_ = std::move(out);
+}
+
 const DeviceBuffer& HairStyle::device_vertices() const {
     if (!d_vertices_.get() && !vertices.empty()) d_vertices_ = DeviceBuffer(vertices.data(), vertices.size() * sizeof(vec3));
     return d_vertices_;

@@ -110,6 +110,10 @@ int main(int argc, char** argv) {
         write(out + "/resolve.bin", img.data(), img.size() * 4);
         uint32_t used = ppll.node_count_used();
 
+        // the load-time spatial order of the pairs (a permutation)
+        style.sort_indices_spatially();
+        write(out + "/indices_sorted.bin", style.indices.data(), style.indices.size() * 4);
+
         // hybrid colour passes (Rasterizer::update + draw_color), one per
         // (magnified, minified) LoD distance pair of the optional tail
         uint32_t n_hybrid = 0;
