@@ -227,6 +227,52 @@ __device__ __forceinline__ double pcf_tap(const RasterArgs& a, double u0, double
     return double(expf(float(cnt * a.ln_transmit))) * a.pcf_w[q];
 }
 
+// The reference's 3x3 kernel with the taps' per-axis terms formed once: tap
+// q = 3j + i sits at (u0 + dx[i], v0 + dy[3j]), so there are three texel
+// columns and three rows, each with its floor, weight and clamped texels.
+// Each tap's operations and the summation order are those of pcf_tap, so the
+// sum is the same bit for bit.  Returns false (nothing summed) when a
+// coordinate is beyond +-1e9 texels; the caller then takes pcf_tap's
+// per-tap path, which replaces such taps by texel (0, 0).
+__device__ __forceinline__ bool pcf3x3_separable(const RasterArgs& a, double u0, double v0, double light_depth,
+                                                 double* vis) {
+    double wx[3], wy[3];
+    int x0[3], x1[3];
+    uint32_t r0[3], r1[3];
+    bool sane = true;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double x = (u0 + a.pcf_dx[i]) * a.sw - 0.5, y = (v0 + a.pcf_dy[3 * i]) * a.sh - 0.5;
+        sane = sane && fabs(x) < 1e9 && fabs(y) < 1e9;
+        const double fx = floor(x), fy = floor(y);
+        wx[i] = x - fx;
+        wy[i] = y - fy;
+        const int ix = int(fx), iy = int(fy);
+        x0[i] = min(max(ix, 0), a.sw - 1);
+        x1[i] = min(max(ix + 1, 0), a.sw - 1);
+        r0[i] = uint32_t(min(max(iy, 0), a.sh - 1)) * uint32_t(a.sw);
+        r1[i] = uint32_t(min(max(iy + 1, 0), a.sh - 1)) * uint32_t(a.sw);
+    }
+    if (!sane) return false;
+    double sum = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int q = 3 * j + i;
+            const double t00 = a.shadow_map[r0[j] + x0[i]], t10 = a.shadow_map[r0[j] + x1[i]];
+            const double t01 = a.shadow_map[r1[j] + x0[i]], t11 = a.shadow_map[r1[j] + x1[i]];
+            const double s0 = t00 + wx[i] * (t10 - t00), s1 = t01 + wx[i] * (t11 - t01);
+            const double sd = s0 + wy[j] * (s1 - s0);
+            const double depth = fmax(light_depth - sd, 0.0);
+            double cnt = depth * 15000.0;
+            if (depth > 1e-5) cnt += 1.0;
+            sum += double(expf(float(cnt * a.ln_transmit))) * a.pcf_w[q];
+        }
+    *vis = sum;
+    return true;
+}
+
 // Per-wave LDS of the colour pass: the shading data of the wave's 64
 // segments (structure of arrays, so lanes reading different segments hit
 // different banks) and a ring of queued fragments.
@@ -431,12 +477,9 @@ __device__ void colour_segments(const RasterArgs& a, WaveShared& W, uint32_t s, 
                     double light_depth = ls[2] / ls[3];
                     double u0 = ls[0] / ls[3], v0 = ls[1] / ls[3];
                     double vis = 0.0, tot = 0.0;
-                    if (a.pcf_taps == 9) {  // the reference's 3x3 kernel, unrolled: 36 loads in flight
+                    if (a.pcf_taps == 9 && pcf3x3_separable(a, u0, v0, light_depth, &vis)) {
 #pragma unroll
-                        for (int q = 0; q < 9; ++q) {
-                            vis += pcf_tap(a, u0, v0, light_depth, q);
-                            tot += a.pcf_w[q];
-                        }
+                        for (int q = 0; q < 9; ++q) tot += a.pcf_w[q];
                     } else {
                         for (int q = 0; q < a.pcf_taps; ++q) {
                             vis += pcf_tap(a, u0, v0, light_depth, q);
@@ -449,7 +492,10 @@ __device__ void colour_segments(const RasterArgs& a, WaveShared& W, uint32_t s, 
                     double st[3];
 #pragma unroll
                     for (int c = 0; c < 3; ++c) st[c] = (P[c] - a.vorigin[c]) / a.vsize[c] * a.vres[c] - 0.5;
-                    occlusion *= local_ambient_occlusion_f32pow(a.vol, st, a.ao_scaling, a.ao_clamp, a.ao_exponent);
+                    occlusion *= a.vol.f32 ? local_ambient_occlusion_f32pow(a.vol, st, a.ao_scaling, a.ao_clamp,
+                                                                            a.ao_exponent)
+                                           : local_ambient_occlusion_cells_f32pow(a.vol, st, a.ao_scaling,
+                                                                                  a.ao_clamp, a.ao_exponent);
                 }
                 color = make_float4(float(shading[0] * occlusion), float(shading[1] * occlusion),
                                     float(shading[2] * occlusion), float(coverage));

@@ -212,6 +212,59 @@ __device__ __forceinline__ double local_ambient_occlusion_f32pow(const DensityVo
     return double(powf(float(1.0 - dens / 8.0), float(exponent)));
 }
 
+// local_ambient_occlusion_f32pow on the corner-packed cells with the taps'
+// per-axis terms formed once: the 8 taps sit at s[c] +- scale/2 on each axis,
+// so each axis has two positions, and with them two floors, weights, range
+// tests and cell-index terms.  Every tap then adds three index terms.  The
+// operations on each position, the trilinear blend and the summation order
+// are those of density_at_cells_select in the generic loop, so the value is
+// the same bit for bit; a tap outside [-1, n) on any axis gathers cell 0 and
+// adds 0, as there.
+__device__ __forceinline__ double local_ambient_occlusion_cells_f32pow(const DensityVolume& v, const double s[3],
+                                                                      double scale, double clamp_v,
+                                                                      double exponent) {
+    const uint32_t cx = uint32_t(v.nx) + 1u, cxy = cx * (uint32_t(v.ny) + 1u);
+    const int n[3] = {v.nx, v.ny, v.nz};
+    const uint32_t stride[3] = {1u, cx, cxy};
+    float w[3][2];
+    uint32_t term[3][2];
+    bool in[3][2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const double p = s[c] + (e ? 0.5 : -0.5) * scale;
+            in[c][e] = (p >= -1.0) & (p < double(n[c]));
+            const double f = floor(p);
+            w[c][e] = float(p - f);
+            term[c][e] = uint32_t(int(f) + 1) * stride[c];  // used only when in range
+        }
+    double dens = 0.0;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const int i = t & 1, j = (t >> 1) & 1, k = t >> 2;
+        const bool inside = in[0][i] & in[1][j] & in[2][k];
+        const uint32_t cell = inside ? term[0][i] + term[1][j] + term[2][k] : 0u;
+        const unsigned long long q = v.cells[cell];
+        const uint32_t lo = uint32_t(q), hi = uint32_t(q >> 32);
+        float b[8];
+        b[0] = float(lo & 0xffu); b[1] = float((lo >> 8) & 0xffu);
+        b[2] = float((lo >> 16) & 0xffu); b[3] = float(lo >> 24);
+        b[4] = float(hi & 0xffu); b[5] = float((hi >> 8) & 0xffu);
+        b[6] = float((hi >> 16) & 0xffu); b[7] = float(hi >> 24);
+        const float wx = w[0][i], wy = w[1][j], wz = w[2][k];
+        const float c00 = fmaf(wx, b[1] - b[0], b[0]);
+        const float c10 = fmaf(wx, b[3] - b[2], b[2]);
+        const float c01 = fmaf(wx, b[5] - b[4], b[4]);
+        const float c11 = fmaf(wx, b[7] - b[6], b[6]);
+        const float c0 = fmaf(wy, c10 - c00, c00);
+        const float c1 = fmaf(wy, c11 - c01, c01);
+        const float r = fmaf(wz, c1 - c0, c0);
+        dens += fmin(double(inside ? r * (1.0f / 255.0f) : 0.0f), clamp_v);
+    }
+    return double(powf(float(1.0 - dens / 8.0), float(exponent)));
+}
+
 // Kajiya-Kay (kajiya-kay.glsl:7-24); sqrt of a negative rounding residue is 0.
 // kF32Pow: the specular power in float32 (the strand rasterizer).
 template <bool kF32Pow = false>
