@@ -68,7 +68,7 @@ def time_adjoint(dev, reps=10):
     from vkhr_amd import params, synth
     from vkhr_amd import device as D
     res, w, h = (128, 128, 128), 512, 512
-    s = synth.config0()
+    s = synth.strands_10k()
     b, vol = D.style_volume(s, res, dev, segments=False, want_f32=True)
     fvol = dict(density_f32=vol["density_f32"], tangents=vol["tangents"])
     sc = synth.make_scene(b, w, h)
@@ -324,7 +324,7 @@ def time_adjoint_batch(dev):
     return e[0].elapsed_time(e[1]), e[2].elapsed_time(e[3])
 
 
-TRAFFIC_SUMMARY = Path(__file__).resolve().parent / "profiles" / "raymarch_traffic_config1_1080p.md"
+TRAFFIC_SUMMARY = Path(__file__).resolve().parent / "profiles" / "raymarch_traffic_136k_strands_1080p.md"
 
 
 def measured_traffic():
@@ -471,7 +471,7 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    style = synth.config1()
+    style = synth.strands_136k()
     res = (256, 256, 256)
     W, H, S = 1920, 1080, 512
     xyz = D.to_dev(style.vertices, dev)

@@ -28,7 +28,7 @@ RES = (128, 128, 128)
 
 class Problem:
     def __init__(self, device):
-        s = synth.config0()
+        s = synth.strands_10k()
         b, vol = D.style_volume(s, RES, device, segments=False, want_f32=True)
         self.bounds = b
         self.dens = vol["density_f32"]

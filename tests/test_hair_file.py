@@ -69,7 +69,7 @@ def test_roundtrip_is_byte_identical(tmp_path):
 
 
 def test_ponytail_size_identity(tmp_path):
-    s = synth.config1()
+    s = synth.strands_136k()
     assert s.n_strands == 136_320 and s.n_vertices == 1_772_160
     verts = s.vertices.reshape(-1, 3)
     offs = s.strand_offsets.astype(np.uint32)

@@ -176,7 +176,7 @@ def test_segment_total_is_sum_of_ceil_linf_and_samples_lie_on_segment():
 
 
 def test_vertex_total_and_saturating_view():
-    s = synth.config0()
+    s = synth.strands_10k()
     b = OV.aabb(s.vertices)
     r = OV.voxelize(s.vertices, b, (8, 8, 8))
     assert r["counts"].sum() == s.n_vertices
@@ -198,7 +198,7 @@ def test_golden_fixtures_unchanged():
 
 
 def test_openmp_count_variant_matches():
-    s = synth.config0()
+    s = synth.strands_10k()
     b = OV.aabb(s.vertices)
     idx = s.indices()
     want = OV.voxelize(s.vertices, b, (64, 64, 64), indices=idx)["counts"]

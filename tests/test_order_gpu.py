@@ -41,7 +41,7 @@ def _keys(pairs):
 
 
 def test_sorted_pairs_are_a_morton_ordered_permutation(device):
-    s = synth.config0()
+    s = synth.strands_10k()
     xyz = D.to_dev(s.vertices, device)
     idx = D.generate_indices(D.to_dev(s.strand_offsets, device))
     b = D.aabb(xyz)
@@ -87,7 +87,7 @@ def test_malformed_pairs_sort_last_and_errors(device):
 
 
 def test_passes_are_unchanged_on_the_sorted_order(device):
-    s = synth.config0()
+    s = synth.strands_10k()
     res = (128, 128, 128)
     xyz = D.to_dev(s.vertices, device)
     off = D.to_dev(s.strand_offsets, device)

@@ -112,7 +112,7 @@ def _resolve_parity(ppll, background=None):
 
 @pytest.fixture(scope="module")
 def c0(device):
-    return Scene(synth.config0(), (128, 128, 128), 512, 512, device, azimuth=30.0)
+    return Scene(synth.strands_10k(), (128, 128, 128), 512, 512, device, azimuth=30.0)
 
 
 def test_shadow_map_matches_oracle(c0):
@@ -124,11 +124,11 @@ def test_shadow_map_matches_oracle(c0):
     assert (got < 1).sum() > 10000
 
 
-def test_config0_fragments_and_resolve(c0):
+def test_10k_strands_fragments_and_resolve(c0):
     ppll, cap = c0.raster(1 << 22)
     assert ppll.used() < ppll.node_count
     rep = _fragment_parity(c0, ppll, cap)
-    print("config0 raster", rep)
+    print("10k strands raster", rep)
     assert rep["fragments"] > 100000
     # every node is reachable from exactly one head
     heads = ppll.heads.cpu().numpy().view(np.uint32)
@@ -293,15 +293,15 @@ def test_hybrid_lod_scales_each_pass(c0, device):
     print("hybrid resolve", _resolve_parity(ph))
 
 
-def test_config1_full_scale_sampled(device):
+def test_136k_strands_full_scale_sampled(device):
     """136,320 strands at 1920x1080, 1024^2 shadow map, k=16 and a 32M-node
     pool; the oracle checks 4,000 random segments and the whole resolve."""
-    s = Scene(synth.config1(), (256, 256, 256), 1920, 1080, device)
+    s = Scene(synth.strands_136k(), (256, 256, 256), 1920, 1080, device)
     ppll, cap = s.raster(1 << 25)
     assert ppll.used() < ppll.node_count
     rng = np.random.default_rng(5)
     pairs = rng.choice(s.idx.shape[0] // 2, 4000, replace=False)
     rep = _fragment_parity(s, ppll, cap, pairs=pairs)
-    print("config1 raster", rep, "nodes", ppll.used())
+    print("136k strands raster", rep, "nodes", ppll.used())
     err = _resolve_parity(ppll)
-    print("config1 resolve", err)
+    print("136k strands resolve", err)

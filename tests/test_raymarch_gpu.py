@@ -48,14 +48,14 @@ def test_axis_scene_closed_form(device):
 
 @pytest.fixture(scope="module")
 def c0_volume(device):
-    s = synth.config0()
+    s = synth.strands_10k()
     res = (128, 128, 128)
     b, vol = D.style_volume(s, res, device, segments=False)
     return s, res, b, vol
 
 
 @pytest.mark.parametrize("azimuth", [0.0, 90.0, 215.0])
-def test_config0_512(device, c0_volume, azimuth):
+def test_10k_strands_512(device, c0_volume, azimuth):
     s, res, b, vol = c0_volume
     w = h = 512
     sc = synth.make_scene(b, w, h, azimuth_deg=azimuth)
@@ -63,13 +63,13 @@ def test_config0_512(device, c0_volume, azimuth):
     rp = params.render(raycast_steps=256.0)
     g = _render(vol, res, cam, light, sp, rp, w, h)
     rep = compare_raymarch(g, _np_vol(vol), res, cam, light, sp, rp, w, h)
-    print("config0", azimuth, rep)
+    print("10k strands", azimuth, rep)
     assert rep["fragments"] > 20000
 
 
 @pytest.mark.parametrize("overrides", [dict(shading_model=2), dict(shading_model=3), dict(adsm_on=0),
                                        dict(raycast_steps=100.0), dict(isosurface=0.5), dict(renderer=3)])
-def test_config0_parameter_variants(device, c0_volume, overrides):
+def test_10k_strands_parameter_variants(device, c0_volume, overrides):
     s, res, b, vol = c0_volume
     w, h = 160, 120
     sc = synth.make_scene(b, w, h, azimuth_deg=40.0)
@@ -96,10 +96,10 @@ def test_depth_buffer_breaks_the_march(device, c0_volume):
     print("depth buffer", rep)
 
 
-def test_config1_full_scale_sampled(device):
+def test_136k_strands_full_scale_sampled(device):
     """1920x1080, 512 steps, 256^3 from 1.6M segments: the GPU renders the
     whole frame; the oracle checks 12k random pixels plus 4k covered ones."""
-    s = synth.config1()
+    s = synth.strands_136k()
     res = (256, 256, 256)
     b, vol = D.style_volume(s, res, device, segments=True)
     w, h = 1920, 1080
@@ -111,7 +111,7 @@ def test_config1_full_scale_sampled(device):
     covered = np.nonzero(g["info"].reshape(-1, 4)[:, 0] == 1)[0]
     pix = np.unique(np.concatenate([rng.integers(0, w * h, 12000), rng.choice(covered, 4000, replace=False)]))
     rep = compare_raymarch(g, _np_vol(vol), res, cam, light, sp, rp, w, h, pixels=pix)
-    print("config1", rep, "covered", len(covered))
+    print("136k strands", rep, "covered", len(covered))
     assert len(covered) > 300000
 
 
@@ -143,7 +143,7 @@ def test_occupancy_skip_is_bit_exact(device):
     """Samples in empty bricks skip their gathers: the frame, depth and
     surface decisions are bit-identical to the march without the occupancy
     (1920x1080, 512 steps, 256^3 from 1.6M segments)."""
-    s = synth.config1()
+    s = synth.strands_136k()
     res = (256, 256, 256)
     b, vol = D.style_volume(s, res, device, segments=True)
     assert vol.get("occupancy") is not None

@@ -291,7 +291,7 @@ def test_full_size_frame(device):
     from vkhr_amd import device as D
     from vkhr_amd import params
     from vkhr_amd.trace import Bvh, viewing_plane
-    s = synth.config1()
+    s = synth.strands_136k()
     xyz = D.to_dev(s.vertices, device)
     off = D.to_dev(s.strand_offsets, device)
     idx = D.generate_indices(off)

@@ -31,7 +31,7 @@ def _check_counts(style, res, device, segments=True):
 
 
 def test_prep_bit_exact(device):
-    s = synth.config0()
+    s = synth.strands_10k()
     xyz = D.to_dev(s.vertices, device)
     off = D.to_dev(s.strand_offsets, device)
     b = D.aabb(xyz)
@@ -54,12 +54,12 @@ def test_prep_uneven_strands(device):
 
 
 @pytest.mark.parametrize("segments", [False, True])
-def test_config0_128(device, segments):
-    _check_counts(synth.config0(), (128, 128, 128), device, segments)
+def test_10k_strands_128(device, segments):
+    _check_counts(synth.strands_10k(), (128, 128, 128), device, segments)
 
 
-def test_config1_segments_256(device):
-    _check_counts(synth.config1(), (256, 256, 256), device, True)
+def test_136k_strands_segments_256(device):
+    _check_counts(synth.strands_136k(), (256, 256, 256), device, True)
 
 
 def test_golden_fixtures(device):
@@ -121,7 +121,7 @@ def test_invalid_arguments_are_errors(device):
 
 
 def test_finalize_matches_oracle(device):
-    s = synth.config0()
+    s = synth.strands_10k()
     res = (128, 128, 128)
     ref, counts, sums = _check_counts(s, res, device, True)
     vol = D.finalize(counts, res, sums=sums, normalize=True, want_f32=True, want_cells=True)
@@ -147,7 +147,7 @@ def test_finalize_matches_oracle(device):
 
 
 def test_finalize_without_normalize(device):
-    s = synth.config0()
+    s = synth.strands_10k()
     res = (64, 64, 64)
     ref, counts, _ = _check_counts(s, res, device, True)
     vol = D.finalize(counts, res, normalize=False, want_cells=False)
@@ -159,7 +159,7 @@ def test_packed_counts_path_matches(device):
     packed in the accumulator's second word; finalize unpacks them and gives
     the same volumes as from the unpacked counts (cells also without the
     normalize)."""
-    s = synth.config0()
+    s = synth.strands_10k()
     res = (128, 128, 128)
     ref, counts, sums = _check_counts(s, res, device, True)
     xyz = D.to_dev(s.vertices, device)
@@ -198,7 +198,7 @@ def _occupancy_from_cells(cells: np.ndarray, res) -> np.ndarray:
 
 @pytest.mark.parametrize("res", [(128, 128, 128), (40, 72, 9)])
 def test_occupancy_bits(device, res):
-    s = synth.config0()
+    s = synth.strands_10k()
     _, counts, sums = _check_counts(s, res, device, True)
     for normalize in (True, False):
         vol = D.finalize(counts, res, sums=sums, normalize=normalize, want_cells=True)

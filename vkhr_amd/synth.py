@@ -160,12 +160,12 @@ def hemisphere_style(n_strands: int, verts_per_strand, seed: int,
 
 # --- the named workloads -----------------------------------------------------
 
-def config0(seed: int = 0) -> Style:
+def strands_10k(seed: int = 0) -> Style:
     """10,000 strands x 16 vertices (the reference's CPU-runnable case)."""
     return hemisphere_style(10_000, 16, seed)
 
 
-def config1(seed: int = 1) -> Style:
+def strands_136k(seed: int = 1) -> Style:
     """136,320 strands x 13 vertices, thickness U[0.5, 1.0] (ponytail-sized)."""
     return hemisphere_style(136_320, 13, seed, thickness=(0.5, 1.0))
 
