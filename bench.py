@@ -535,9 +535,16 @@ def main():
     for _ in range(args.steps):
         voxelize()
     ev[5].record()
+    # the finalize alone (unpack, u8 densities, tangents, min/max, normalize, cells)
+    ev_fin = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev_fin[0].record()
+    for _ in range(args.steps):
+        D.finalize(None, res, sums=sums, normalize=True, want_cells=True)
+    ev_fin[1].record()
     torch.cuda.synchronize()
     t_ray = ev[2].elapsed_time(ev[3]) / args.steps * 1e-3
     t_vox = ev[4].elapsed_time(ev[5]) / args.steps * 1e-3
+    t_fin = ev_fin[0].elapsed_time(ev_fin[1]) / args.steps * 1e-3
 
     out = D.raymarch(vol, res, cam, light, sp, rp, W, H, want_depth=True, want_info=True)
     info = out["info"].reshape(-1, 4).cpu().numpy()
@@ -556,6 +563,18 @@ def main():
                         "frac": samples / t_vox / peak["u32_uniform"],
                         "peak_lane_contiguous_atomics_per_s": peak["u32_stream"],
                         "note": "voxelize_ms includes zeroing the 256 MiB accumulator"}
+    # algorithmic bytes of the volume build per segment stream: 12 B per vertex,
+    # 4 B per strand offset, and per voxel the 4-byte count clear, the
+    # finalize's 4-byte count read and its 1-byte density write
+    nvert = int(xyz.shape[0])
+    vox_bytes = 12 * nvert + 4 * (style.n_strands + 1) + 9 * nvox
+    vox_hbm = vox_bytes / (t_vox + t_fin) / 1e9
+    voxelize_roofline = {"algorithmic_bytes": vox_bytes, "seconds": t_vox + t_fin, "achieved_gbs": vox_hbm,
+                         "hbm_frac": vox_hbm / HBM_PEAK_GBS, "atomic_frac": voxelize_atomics["frac"],
+                         "binding_frac": max(vox_hbm / HBM_PEAK_GBS, voxelize_atomics["frac"]),
+                         "finalize_ms": t_fin * 1e3,
+                         "note": "bytes over voxelize + finalize; atomics over voxelize (one line request per "
+                                 "sample) against the measured uniform-random request peak"}
 
     result = {
         "metric": METRIC,
@@ -578,6 +597,7 @@ def main():
         "segments_per_s": n_seg / t_vox,
         "voxelize_ms": t_vox * 1e3,
         "voxelize_atomics": voxelize_atomics,
+        "voxelize_roofline": voxelize_roofline,
         "load": {"sort_segments_ms": t_sort * 1e3,
                  "note": "once per style at load, outside the step: the index pairs in the Morton order of the "
                          "segment midpoints, which every pass here draws"},
@@ -587,9 +607,17 @@ def main():
     }
     if not args.no_extras:
         t_sh, t_ra, t_re, n_frag = time_raster(dev, style, xyz, idx, tan, vol, res, scene, cam, light, bounds)
+        # algorithmic work per fragment: 9 shadow taps x 4 B, 8 AO taps x 8
+        # texels x 1 B, the 12-byte node; per segment the two vertices'
+        # position, tangent and thickness; one head exchange per fragment
+        ra_bytes = (36 + 64 + 12) * n_frag + 2 * (12 + 12 + 4) * n_seg
         result["raster"] = {"workload": "1,635,840 segments, 1920x1080, 1024^2 shadow map, k=16, 2^25-node PPLL",
                             "shadow_ms": t_sh, "raster_ms": t_ra, "resolve_ms": t_re, "fragments": n_frag,
-                            "fragments_per_s": n_frag / (t_ra * 1e-3)}
+                            "fragments_per_s": n_frag / (t_ra * 1e-3),
+                            "algorithmic_bytes": ra_bytes, "achieved_gbs": ra_bytes / (t_ra * 1e-3) / 1e9,
+                            "hbm_frac": ra_bytes / (t_ra * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                            "head_exchanges_per_s": n_frag / (t_ra * 1e-3),
+                            "atomic_frac": n_frag / (t_ra * 1e-3) / peak["u32_uniform"]}
         frames = time_reference_frames(dev, style, xyz, idx, tan, vol, res, bounds)
         result["reference_setup_720p"] = {
             "workload": "136,320 strands x 13 vertices, 1280x720, 1024^2 3x3 PCF shadow map, 256^3, 512 steps",
