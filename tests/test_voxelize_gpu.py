@@ -196,6 +196,36 @@ def _occupancy_from_cells(cells: np.ndarray, res) -> np.ndarray:
     return words
 
 
+def _cells_from_density(dens: np.ndarray, res) -> np.ndarray:
+    """The corner-packed cells from the uint8 density: cell (i, j, k), i in
+    [-1, nx-1], byte b = texel (i + (b & 1), j + (b >> 1 & 1), k + (b >> 2)),
+    0 outside the volume."""
+    nx, ny, nz = res
+    pad = np.zeros((nz + 2, ny + 2, nx + 2), np.uint8)
+    pad[1:nz + 1, 1:ny + 1, 1:nx + 1] = dens.reshape(nz, ny, nx)
+    want = np.zeros((nz + 1, ny + 1, nx + 1), np.uint64)
+    for b in range(8):
+        dx, dy, dz = b & 1, (b >> 1) & 1, b >> 2
+        want |= pad[dz:dz + nz + 1, dy:dy + ny + 1, dx:dx + nx + 1].astype(np.uint64) << np.uint64(8 * b)
+    return want.reshape(-1)
+
+
+@pytest.mark.parametrize("res", [(40, 72, 9), (37, 20, 65), (64, 8, 200)])
+def test_cells_exact_at_uneven_sizes(device, res):
+    """Cells and occupancy at uneven sizes: partial 64-column groups (x 40,
+    37, 64 + the border column), partial brick rows, thin and deep volumes
+    (several cell slabs along z), x sizes that are and are not multiples
+    of 4."""
+    s = synth.strands_10k()
+    _, counts, sums = _check_counts(s, res, device, True)
+    vol = D.finalize(counts, res, sums=sums, normalize=True, want_cells=True)
+    cells = vol["cells"].cpu().numpy().view(np.uint64)
+    want = _cells_from_density(vol["density"].cpu().numpy(), res)
+    assert np.array_equal(cells, want)
+    assert np.array_equal(vol["occupancy"].cpu().numpy().view(np.uint32), _occupancy_from_cells(want, res))
+    assert (want != 0).sum() > 100
+
+
 @pytest.mark.parametrize("res", [(128, 128, 128), (40, 72, 9)])
 def test_occupancy_bits(device, res):
     s = synth.strands_10k()
