@@ -6,8 +6,11 @@
  * default stream), and returns an int status: VKHR_OK (0) or a negative
  * VKHR_ERR_* code, with a message available from vkhr_last_error().  No C++
  * exception crosses this boundary.  Calls on distinct buffers are reentrant;
- * the only hidden state is a small per-device scratch area that the library
- * allocates on first use (reductions, counters).
+ * the only hidden state is a per-device scratch area that the library
+ * allocates on first use (reductions, counters, the adjoint's cell
+ * accumulator).  vkhr_aabb_compute, vkhr_volume_finalize and
+ * vkhr_raymarch_adjoint use it: on one device, issue them on one stream, or
+ * order the streams they run on, since overlapping calls would share it.
  *
  * Layouts:
  *   xyz           float32 x,y,z per vertex, tightly packed (12 B / vertex)
