@@ -52,6 +52,17 @@ def small_style(n_strands=200, seed=7):
     return arrays
 
 
+def test_xorshift64_known_answers():
+    """Marsaglia's xorshift64 with shifts (13, 7, 17), worked by hand: from 1,
+    x ^= x << 13 gives 8193, x ^= x >> 7 gives 8257, x ^= x << 17 gives
+    8257 + 8257 * 2^17 = 1082269761; and a zero state stays zero."""
+    assert HF.xorshift64(1) == 1082269761
+    assert HF.xorshift64(0) == 0
+    # the state is 64 bits wide: from 2^63 both left shifts fall off the top
+    # and the right shift adds 2^56
+    assert HF.xorshift64(1 << 63) == (1 << 63) | (1 << 56)
+
+
 def test_roundtrip_is_byte_identical(tmp_path):
     arrays = small_style()
     data = HF.write(arrays, 200, default_segment_count=0, default_thickness=0.042, default_transparency=0.3,
