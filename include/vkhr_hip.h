@@ -151,7 +151,7 @@ int vkhr_generate_indices(const uint32_t* strand_offsets, uint32_t n_strands,
  * with an index >= n_vertices last.  A permutation of the pairs: voxel
  * counts, fragments and shadow depths are the same on it, while the strand
  * passes gather from a compact region per workgroup (the rasterizer's
- * 1080p pass at 136k strands of random root order: 2.24 -> 1.6 ms).  Both
+ * 1080p pass at 136k strands of random root order: 2.24 -> 1.73 ms).  Both
  * buffers 8-byte aligned and disjoint.  Allocates and frees temporaries;
  * synchronises the stream. */
 int vkhr_sort_segments(const float* xyz, uint32_t n_vertices, const uint32_t* indices, uint32_t n_indices,
@@ -395,7 +395,9 @@ int vkhr_shadow_depth(const float* xyz, uint32_t n_vertices, const uint32_t* ind
  * 136k strands / 1080p).  Nodes beyond ppll->node_count are dropped (the
  * reference accepts index == node_count, one past its pool).  depth_buffer
  * (nullable, W*H) is the depth test input (LESS, no writes); NULL reads
- * 1.0.  capture is nullable (entries of unlinked nodes are not written). */
+ * 1.0.  capture is nullable (entries of unlinked nodes are not written).
+ * Limits: shadow maps up to 16384 x 16384 (VKHR_ERR_INVALID beyond), volumes
+ * with (nx+1)(ny+1)(nz+1) < 2^32 cells (VKHR_ERR_OVERFLOW beyond). */
 int vkhr_raster_strands(const float* xyz, const float* tangents, const float* thickness, uint32_t n_vertices,
                         const uint32_t* indices, uint32_t n_indices, const vkhr_camera* camera,
                         const vkhr_light* light, const vkhr_strand_params* strand,

@@ -16,7 +16,7 @@ import torch
 from oracle import raster as ORs
 from oracle import raymarch as OR
 from tests.parity import compare_raymarch
-from vkhr_amd import params, synth
+from vkhr_amd import capi, params, synth
 from vkhr_amd import device as D
 
 pytestmark = pytest.mark.gpu
@@ -172,6 +172,21 @@ def test_pool_overflow_drops_without_out_of_bounds(c0, device):
     live = heads[heads != NULL]
     assert len(live) and (live < n).all()
     ppll.resolve()
+
+
+def test_size_limits_are_errors(c0, device):
+    """Shadow maps beyond 16384 texels a side and volumes of 2^32 cells or
+    more are refused before any launch (the taps use 32-bit offsets)."""
+    ppll = D.PPLL(c0.w, c0.h, 1 << 16, device)
+    wide = torch.zeros((1, 20000), dtype=torch.float32, device=device)
+    with pytest.raises(capi.VkhrError) as e:
+        D.raster_strands(c0.xyz, c0.tan, c0.th, c0.idx, c0.cam, c0.light, c0.sp, c0.rp, c0.vol, c0.res, wide, ppll)
+    assert e.value.code == -1
+    huge = dict(cells=c0.vol["cells"])
+    with pytest.raises(capi.VkhrError) as e:
+        D.raster_strands(c0.xyz, c0.tan, c0.th, c0.idx, c0.cam, c0.light, c0.sp, c0.rp, huge, (2048, 2048, 1100),
+                         c0.smap, ppll)
+    assert e.value.code == -3
 
 
 def test_hybrid_raymarch_and_strands_share_the_ppll(c0, device):
