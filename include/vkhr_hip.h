@@ -6,11 +6,12 @@
  * default stream), and returns an int status: VKHR_OK (0) or a negative
  * VKHR_ERR_* code, with a message available from vkhr_last_error().  No C++
  * exception crosses this boundary.  Calls on distinct buffers are reentrant;
- * the only hidden state is a per-device scratch area that the library
- * allocates on first use (reductions, counters, the adjoint's cell
- * accumulator).  vkhr_aabb_compute, vkhr_volume_finalize and
- * vkhr_raymarch_adjoint use it: on one device, issue them on one stream, or
- * order the streams they run on, since overlapping calls would share it.
+ * the only hidden state is a scratch area per device and stream that the
+ * library allocates on first use and keeps until the process ends
+ * (reductions, counters, the adjoint's cell accumulator: used by
+ * vkhr_aabb_compute, vkhr_volume_finalize and vkhr_raymarch_adjoint).
+ * Calls on one stream are ordered by it; calls on different streams use
+ * different areas, so they may run concurrently.
  *
  * Layouts:
  *   xyz           float32 x,y,z per vertex, tightly packed (12 B / vertex)

@@ -237,3 +237,28 @@ def test_occupancy_bits(device, res):
         want = _occupancy_from_cells(vol["cells"].cpu().numpy(), res)
         assert np.array_equal(got, want)
         assert 0 < int(np.unpackbits(want.view(np.uint8)).sum()) < 32 * want.size
+
+
+def test_finalize_on_concurrent_streams(device):
+    """Two finalizes (each with its min/max reduction and normalize) in flight
+    on two streams at once give the volumes of serial calls: each stream has
+    its own scratch area."""
+    a_style, b_style = synth.strands_10k(), synth.hemisphere_style(300, 20, seed=17)
+    res = (128, 128, 128)
+    ra, ca, sa = _check_counts(a_style, res, device, True)
+    rb, cb, sb = _check_counts(b_style, res, device, True)
+    # different maxima: a shared min/max would normalize one of them wrongly
+    assert ra["dens_u8"].max() != rb["dens_u8"].max()
+    want_a = D.finalize(ca, res, sums=sa, normalize=True, want_cells=True)
+    want_b = D.finalize(cb, res, sums=sb, normalize=True, want_cells=True)
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    for _ in range(3):
+        with torch.cuda.stream(s1):
+            got_a = D.finalize(ca, res, sums=sa, normalize=True, want_cells=True)
+        with torch.cuda.stream(s2):
+            got_b = D.finalize(cb, res, sums=sb, normalize=True, want_cells=True)
+        torch.cuda.synchronize()
+        for want, got in ((want_a, got_a), (want_b, got_b)):
+            for k in ("density", "tangents", "cells", "occupancy"):
+                assert torch.equal(want[k], got[k]), k

@@ -414,7 +414,7 @@ extern "C" int vkhr_raymarch_adjoint(const vkhr_volume* volume, const vkhr_camer
     finish_adjoint_args(a);
     const size_t n_cells = size_t(volume->nx + 1) * (volume->ny + 1) * (volume->nz + 1);
     void* cg = nullptr;
-    if ((rc = scratch(n_cells * 8 * sizeof(float), &cg))) {
+    if ((rc = scratch(n_cells * 8 * sizeof(float), as_stream(stream), &cg))) {
         delete a;
         return rc;
     }

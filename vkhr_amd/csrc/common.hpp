@@ -43,10 +43,10 @@ void clear_error();
 // Binds per-device state; returns VKHR_OK or an error code.
 int ensure_device();
 
-// Per-device scratch: a zero-initialised-on-allocation device buffer of at
-// least `bytes` bytes owned by the library until process end.  Calls that
-// share it on one device are stream-ordered by the caller.
-int scratch(size_t bytes, void** out);
+// Scratch of one device and stream: a device buffer of at least `bytes`
+// bytes owned by the library until process end.  Calls on one stream are
+// ordered by the stream; calls on different streams get different areas.
+int scratch(size_t bytes, hipStream_t stream, void** out);
 
 inline hipStream_t as_stream(vkhr_stream s) { return reinterpret_cast<hipStream_t>(s); }
 

@@ -1,10 +1,12 @@
-// Library state: thread-local error messages, device checks, per-device
-// scratch, and the generic fill kernel.
+// Library state: thread-local error messages, device checks, the scratch
+// areas (one per device and stream), and the generic fill kernel.
 #include "common.hpp"
 
 #include <cstdarg>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <utility>
 
 namespace vkhr {
 
@@ -14,8 +16,11 @@ thread_local char g_error[512] = {0};
 constexpr int kMaxDevices = 64;
 std::mutex g_mutex;
 bool g_checked[kMaxDevices] = {false};
-void* g_scratch[kMaxDevices] = {nullptr};
-size_t g_scratch_bytes[kMaxDevices] = {0};
+struct Scratch {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;  // (device, stream) -> area
 }  // namespace
 
 void set_error(const char* fmt, ...) {
@@ -41,13 +46,14 @@ int ensure_device() {
     return VKHR_OK;
 }
 
-int scratch(size_t bytes, void** out) {
+int scratch(size_t bytes, hipStream_t stream, void** out) {
     int rc = ensure_device();
     if (rc) return rc;
     int dev = 0;
     VKHR_CHECK_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_mutex);
-    if (g_scratch_bytes[dev] < bytes) {
+    Scratch& area = g_scratch[std::make_pair(dev, stream)];
+    if (area.bytes < bytes) {
         // Grow: the old buffer may still be in use by queued work, so it is
         // kept (not freed) -- growth happens a handful of times per process.
         size_t want = bytes < (1u << 20) ? (1u << 20) : bytes;
@@ -57,10 +63,10 @@ int scratch(size_t bytes, void** out) {
             set_error("scratch hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
             return VKHR_ERR_NOMEM;
         }
-        g_scratch[dev] = p;
-        g_scratch_bytes[dev] = want;
+        area.ptr = p;
+        area.bytes = want;
     }
-    *out = g_scratch[dev];
+    *out = area.ptr;
     return VKHR_OK;
 }
 

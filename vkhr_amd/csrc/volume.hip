@@ -204,7 +204,7 @@ extern "C" int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tange
     uint32_t* mm = nullptr;
     if (normalize) {
         void* scr = nullptr;
-        if ((rc = scratch(kMinMaxBytes, &scr))) return rc;
+        if ((rc = scratch(kMinMaxBytes, s, &scr))) return rc;
         mm = static_cast<uint32_t*>(scr);
         minmax_init_kernel<<<1, 64, 0, s>>>(mm);
         VKHR_CHECK_LAUNCH();
