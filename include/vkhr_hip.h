@@ -453,7 +453,10 @@ typedef struct vkhr_tuning {
     uint32_t tile_w, tile_h;      /* pixels per wave tile, tile_w * tile_h == 64 */
     uint32_t blocks_per_cu;       /* persistent raymarch blocks per compute unit; 0: one block
                                      per (style, tile) work item */
-    uint32_t reserved_;
+    uint32_t groups;              /* styles split into this many groups of consecutive styles
+                                     (about equal voxels + segments), pipelined in the graph: a
+                                     group's raymarch runs beside the next group's volume passes
+                                     on a second stream; 0 or 1: one chain; at most 64 */
 } vkhr_tuning;
 
 typedef struct vkhr_batch vkhr_batch;

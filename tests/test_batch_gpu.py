@@ -36,15 +36,18 @@ def _plain_outputs(b):
     return out
 
 
-@pytest.mark.parametrize("tuning", [None, "table", (16, 4, 2), (32, 2, 8)])
+@pytest.mark.parametrize("tuning", [None, "table", (16, 4, 2, 1), (32, 2, 8, 1), (8, 8, 0, 2), (16, 4, 4, 5),
+                                    (8, 8, 0, 32)])
 def test_batch_matches_per_style_bit_for_bit(batch32, tuning):
+    """Every launch shape and pipeline depth (groups of styles, up to one
+    style per group) gives the per-style outputs bit for bit."""
     if tuning == "table":
         tuning = load_tuning("batch")
         if tuning is None:
             pytest.skip("no tuning table")
     elif tuning is not None:
         t = capi.Tuning()
-        t.tile_w, t.tile_h, t.blocks_per_cu = tuning
+        t.tile_w, t.tile_h, t.blocks_per_cu, t.groups = tuning
         tuning = t
     batch = Batch(batch32, W, H, tuning=tuning)
     batch.launch()

@@ -24,6 +24,7 @@ def load_tuning(key: str = "batch") -> capi.Tuning | None:
         return None
     t = capi.Tuning()
     t.tile_w, t.tile_h, t.blocks_per_cu = (table[key][k] for k in ("tile_w", "tile_h", "blocks_per_cu"))
+    t.groups = int(table[key].get("groups", 1))
     return t
 
 

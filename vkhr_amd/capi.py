@@ -99,7 +99,7 @@ class AdjointRender(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("reserved_", C.c_uint32)]
+                ("groups", C.c_uint32)]
 
 
 class ViewingPlane(C.Structure):

@@ -1,8 +1,9 @@
 // Batched raymarch over (style, tile) work items -- one block per item, or
 // persistent blocks pulling items from an atomic queue -- and the host side
-// of the batched launch: device tables, one hipGraph capturing clear ->
-// voxelize -> finalize -> normalize -> cells -> raymarch, replayed by
-// vkhr_batch_launch.
+// of the batched launch: device tables per group of styles, one hipGraph
+// capturing clear -> voxelize -> finalize -> normalize -> cells -> raymarch
+// of every group (pipelined: a group's raymarch beside the next group's
+// volume passes), replayed by vkhr_batch_launch.
 #include "batch.hpp"
 #include "volume.hpp"
 #include "voxel.hpp"
@@ -79,9 +80,11 @@ void launch_batch_march(const BatchDevice& d, unsigned grid, hipStream_t stream)
 using namespace vkhr;
 
 struct vkhr_batch {
-    BatchDevice d;
+    std::vector<BatchDevice> groups;    // one device table per group of styles
     std::vector<void*> allocations;
     unsigned march_grid = 0;
+    hipStream_t side = nullptr;         // the raymarch branch of the captured pipeline
+    std::vector<hipEvent_t> events;     // fork, one per group, join
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -102,51 +105,43 @@ int upload(vkhr_batch* b, const std::vector<T>& host, T** out) {
 void release(vkhr_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+    b->events.clear();
+    if (b->side) (void)hipStreamDestroy(b->side);
+    b->side = nullptr;
     for (void* p : b->allocations) (void)hipFree(p);
     b->allocations.clear();
 }
 
-}  // namespace
-
-extern "C" int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styles, uint32_t width, uint32_t height,
-                                 const vkhr_tuning* tuning, vkhr_batch** out) {
-    clear_error();
-    int rc = ensure_device();
-    if (rc) return rc;
-    VKHR_REQUIRE(styles && n_styles && out, VKHR_ERR_INVALID, "no styles");
-    VKHR_REQUIRE(width && height, VKHR_ERR_INVALID, "empty image");
-    vkhr_tuning t = {8, 8, 0, 0};
-    if (tuning) t = *tuning;
-    VKHR_REQUIRE(t.tile_w * t.tile_h == 64 && t.tile_w >= 1 && t.tile_h >= 1, VKHR_ERR_INVALID,
-                 "wave tile %ux%u is not 64 pixels", t.tile_w, t.tile_h);
-    VKHR_REQUIRE(t.blocks_per_cu <= 32, VKHR_ERR_INVALID, "blocks_per_cu %u", t.blocks_per_cu);
-    vkhr_batch* b = new vkhr_batch();
-    std::memset(&b->d, 0, sizeof(b->d));
-    std::vector<BatchVolStyle> vs(n_styles);
-    std::vector<RaymarchArgs> args(n_styles);
-    std::vector<uint64_t> pair_prefix(n_styles + 1, 0);
-    std::vector<uint32_t> tile_prefix(n_styles + 1, 0);
+// The device tables of styles[first, last): per style its voxel grid and
+// buffers, raymarch arguments, segment and tile prefixes, finalize and
+// cells chunk lists, and the group's own min/max slots and work counter.
+int build_group(vkhr_batch* b, const vkhr_batch_style* styles, uint32_t first, uint32_t last, uint32_t width,
+                uint32_t height, const vkhr_tuning& t, BatchDevice* out) {
+    const uint32_t n = last - first;
+    std::vector<BatchVolStyle> vs(n);
+    std::vector<RaymarchArgs> args(n);
+    std::vector<uint64_t> pair_prefix(n + 1, 0);
+    std::vector<uint32_t> tile_prefix(n + 1, 0);
     std::vector<BatchChunk> vchunks, cchunks;
     const uint64_t kChunk = 1u << 16;   // voxels per finalize chunk
     const uint32_t bw = 2 * t.tile_w, bh = 2 * t.tile_h;
     const uint32_t tiles_x = (width + bw - 1) / bw, tiles_y = (height + bh - 1) / bh;
-    for (uint32_t s = 0; s < n_styles; ++s) {
-        const vkhr_batch_style& in = styles[s];
+    int rc = 0;
+    for (uint32_t s = 0; s < n; ++s) {
+        const vkhr_batch_style& in = styles[first + s];
         VoxelGrid g;
-        if ((rc = make_grid(&in.bounds, in.volume, in.volume, in.volume, &g))) {
-            delete b;
-            return rc;
-        }
-        if (!((in.counts || in.tangent_acc) && in.density && in.cells && in.rgba && (in.n_indices < 2 || (in.xyz && in.indices)))) {
-            delete b;
-            set_error("style %u: missing buffer", s);
+        if ((rc = make_grid(&in.bounds, in.volume, in.volume, in.volume, &g))) return rc;
+        if (!((in.counts || in.tangent_acc) && in.density && in.cells && in.rgba &&
+              (in.n_indices < 2 || (in.xyz && in.indices)))) {
+            set_error("style %u: missing buffer", first + s);
             return VKHR_ERR_INVALID;
         }
         const auto misaligned = [](const void* p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) % a) != 0; };
         if (misaligned(in.tangent_acc, 16) || misaligned(in.counts, 16) || misaligned(in.density, 4) ||
             misaligned(in.tangent_i8x4, 16)) {
-            delete b;
-            set_error("style %u: tangent_acc, counts, tangent_i8x4 must be 16-byte and density 4-byte aligned", s);
+            set_error("style %u: tangent_acc, counts, tangent_i8x4 must be 16-byte and density 4-byte aligned",
+                      first + s);
             return VKHR_ERR_INVALID;
         }
         BatchVolStyle& v = vs[s];
@@ -164,8 +159,7 @@ extern "C" int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styl
         v.cells = reinterpret_cast<unsigned long long*>(in.cells);
         v.occupancy = in.occupancy;
         if (v.acc && !(v.tangents && v.tangent_q)) {
-            delete b;
-            set_error("style %u: tangent_acc needs tangents and tangent_i8x4", s);
+            set_error("style %u: tangent_acc needs tangents and tangent_i8x4", first + s);
             return VKHR_ERR_INVALID;
         }
         pair_prefix[s + 1] = pair_prefix[s] + v.n_pairs;
@@ -180,34 +174,85 @@ extern "C" int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styl
         vol.occupancy = in.occupancy;
         vol.tangents = in.tangent_i8x4;
         vol.nx = vol.ny = vol.nz = in.volume;
-        if ((rc = make_raymarch_args(&vol, &in.camera, &in.light, &in.strand, &in.params, width, height, &args[s]))) {
-            delete b;
+        if ((rc = make_raymarch_args(&vol, &in.camera, &in.light, &in.strand, &in.params, width, height, &args[s])))
             return rc;
-        }
         args[s].rgba = reinterpret_cast<float4*>(in.rgba);
         tile_prefix[s + 1] = tile_prefix[s] + tiles_x * tiles_y;
     }
-    BatchDevice& d = b->d;
-    std::vector<uint32_t> zeros(2 * n_styles + 1, 0);
+    BatchDevice& d = *out;
+    std::memset(&d, 0, sizeof(d));
+    std::vector<uint32_t> zeros(2 * n + 1, 0);
     uint32_t* mm = nullptr;
     if ((rc = upload(b, vs, &d.styles)) || (rc = upload(b, args, &d.args)) ||
         (rc = upload(b, pair_prefix, &d.pair_prefix)) || (rc = upload(b, tile_prefix, &d.tile_prefix)) ||
         (rc = upload(b, vchunks, &d.voxel_chunks)) || (rc = upload(b, cchunks, &d.cell_chunks)) ||
-        (rc = upload(b, zeros, &mm))) {
-        release(b);
-        delete b;
+        (rc = upload(b, zeros, &mm)))
         return rc;
-    }
     d.minmax = mm;
-    d.work_counter = mm + 2 * n_styles;
-    d.n_styles = n_styles;
+    d.work_counter = mm + 2 * n;
+    d.n_styles = n;
     d.n_voxel_chunks = uint32_t(vchunks.size());
     d.n_cell_chunks = uint32_t(cchunks.size());
-    d.total_tiles = tile_prefix[n_styles];
-    d.total_pairs = pair_prefix[n_styles];
+    d.total_tiles = tile_prefix[n];
+    d.total_pairs = pair_prefix[n];
     d.tile_w = t.tile_w;
     d.tile_h = t.tile_h;
     d.tiles_x = tiles_x;
+    return VKHR_OK;
+}
+
+}  // namespace
+
+extern "C" int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styles, uint32_t width, uint32_t height,
+                                 const vkhr_tuning* tuning, vkhr_batch** out) {
+    clear_error();
+    int rc = ensure_device();
+    if (rc) return rc;
+    VKHR_REQUIRE(styles && n_styles && out, VKHR_ERR_INVALID, "no styles");
+    VKHR_REQUIRE(width && height, VKHR_ERR_INVALID, "empty image");
+    vkhr_tuning t = {8, 8, 0, 0};
+    if (tuning) t = *tuning;
+    VKHR_REQUIRE(t.tile_w * t.tile_h == 64 && t.tile_w >= 1 && t.tile_h >= 1, VKHR_ERR_INVALID,
+                 "wave tile %ux%u is not 64 pixels", t.tile_w, t.tile_h);
+    VKHR_REQUIRE(t.blocks_per_cu <= 32, VKHR_ERR_INVALID, "blocks_per_cu %u", t.blocks_per_cu);
+    VKHR_REQUIRE(t.groups <= 64, VKHR_ERR_INVALID, "groups %u", t.groups);
+    const uint32_t n_groups = std::max(1u, std::min(t.groups, n_styles));
+    // Groups of consecutive styles of about equal work (voxels plus segments).
+    std::vector<double> cost(n_styles + 1, 0.0);
+    for (uint32_t s = 0; s < n_styles; ++s)
+        cost[s + 1] = cost[s] + double(styles[s].volume) * styles[s].volume * styles[s].volume +
+                      8.0 * double(styles[s].n_indices / 2);
+    std::vector<uint32_t> bounds(n_groups + 1, 0);
+    bounds[n_groups] = n_styles;
+    for (uint32_t g = 1; g < n_groups; ++g) {
+        const double target = cost[n_styles] * g / n_groups;
+        uint32_t s = bounds[g - 1] + 1;
+        while (s < n_styles - (n_groups - g) && cost[s] < target) ++s;
+        bounds[g] = s;
+    }
+    vkhr_batch* b = new vkhr_batch();
+    b->groups.resize(n_groups);
+    for (uint32_t g = 0; g < n_groups; ++g) {
+        if ((rc = build_group(b, styles, bounds[g], bounds[g + 1], width, height, t, &b->groups[g]))) {
+            release(b);
+            delete b;
+            return rc;
+        }
+    }
+    if (n_groups > 1) {
+        hipError_t e = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking);
+        for (uint32_t k = 0; e == hipSuccess && k < n_groups + 2; ++k) {
+            hipEvent_t ev = nullptr;
+            e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+            if (e == hipSuccess) b->events.push_back(ev);
+        }
+        if (e != hipSuccess) {
+            release(b);
+            delete b;
+            set_error("batch stream/events: %s", hipGetErrorString(e));
+            return VKHR_ERR_HIP;
+        }
+    }
     int dev = 0, cus = 0;
     VKHR_CHECK_HIP(hipGetDevice(&dev));
     VKHR_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -223,10 +268,34 @@ extern "C" int vkhr_batch_launch(vkhr_batch* b, vkhr_stream stream) {
     hipStream_t s = as_stream(stream);
     if (!b->exec) {
         VKHR_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        launch_batch_volume(b->d, s);
-        launch_batch_march(b->d, b->march_grid, s);
-        hipError_t e = hipStreamEndCapture(s, &b->graph);
-        VKHR_CHECK_HIP(e);
+        hipError_t err = hipSuccess;  // the first failure; the capture is ended either way
+        const auto check = [&](hipError_t x) {
+            if (err == hipSuccess) err = x;
+        };
+        const size_t n_groups = b->groups.size();
+        if (n_groups == 1) {
+            launch_batch_volume(b->groups[0], s);
+            launch_batch_march(b->groups[0], b->march_grid, s);
+        } else {
+            // A software pipeline: the volume passes of group g+1 (atomic and
+            // HBM bound) run beside the raymarch of group g (VALU bound).  The
+            // volume passes stay on the capturing stream, the raymarches go to
+            // the side stream after their group's volume; the side stream
+            // forks from and joins back into the capture.
+            check(hipEventRecord(b->events[0], s));
+            check(hipStreamWaitEvent(b->side, b->events[0], 0));
+            for (size_t g = 0; g < n_groups; ++g) {
+                launch_batch_volume(b->groups[g], s);
+                check(hipEventRecord(b->events[1 + g], s));
+                check(hipStreamWaitEvent(b->side, b->events[1 + g], 0));
+                launch_batch_march(b->groups[g], b->march_grid, b->side);
+            }
+            check(hipEventRecord(b->events[1 + n_groups], b->side));
+            check(hipStreamWaitEvent(s, b->events[1 + n_groups], 0));
+        }
+        check(hipGetLastError());
+        check(hipStreamEndCapture(s, &b->graph));
+        VKHR_REQUIRE(err == hipSuccess, VKHR_ERR_HIP, "batch graph capture failed: %s", hipGetErrorString(err));
         VKHR_CHECK_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
     VKHR_CHECK_HIP(hipGraphLaunch(b->exec, s));

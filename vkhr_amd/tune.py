@@ -24,11 +24,12 @@ from .batch import TUNING_PATH, Batch, StyleBuffers
 
 TILES = [(8, 8), (16, 4), (4, 16), (32, 2)]
 BLOCKS_PER_CU = [0, 2, 4, 8]
+GROUPS = [1, 2, 3, 4, 6, 8]
 
 
-def time_shape(bufs, width, height, tile, bpc, reps):
+def time_shape(bufs, width, height, tile, bpc, reps, groups=1):
     t = capi.Tuning()
-    t.tile_w, t.tile_h, t.blocks_per_cu = tile[0], tile[1], bpc
+    t.tile_w, t.tile_h, t.blocks_per_cu, t.groups = tile[0], tile[1], bpc, groups
     batch = Batch(bufs, width, height, tuning=t)
     batch.launch()  # capture + first replay
     torch.cuda.synchronize()
@@ -63,13 +64,22 @@ def main(argv=None):
             results.append(dict(tile_w=tile[0], tile_h=tile[1], blocks_per_cu=bpc, ms=round(ms, 4)))
             print(f"tile {tile[0]}x{tile[1]} blocks/CU {bpc}: {ms:.3f} ms ({time.time() - t0:.1f} s)", flush=True)
     best = min(results, key=lambda r: r["ms"])
+    # then the pipeline depth for the best launch shape
+    for g in GROUPS:
+        t0 = time.time()
+        ms = time_shape(bufs, args.width, args.height, (best["tile_w"], best["tile_h"]), best["blocks_per_cu"],
+                        args.reps, g)
+        results.append(dict(tile_w=best["tile_w"], tile_h=best["tile_h"], blocks_per_cu=best["blocks_per_cu"],
+                            groups=g, ms=round(ms, 4)))
+        print(f"groups {g}: {ms:.3f} ms ({time.time() - t0:.1f} s)", flush=True)
+    best = min(results, key=lambda r: r["ms"])
     try:
         with open(args.out) as f:
             table = json.load(f)
     except FileNotFoundError:
         table = {}
     table[args.key] = dict(tile_w=best["tile_w"], tile_h=best["tile_h"], blocks_per_cu=best["blocks_per_cu"],
-                           ms=best["ms"], workload=f"{len(bufs)} styles, {args.width}x{args.height}",
+                           groups=best.get("groups", 1), ms=best["ms"], workload=f"{len(bufs)} styles, {args.width}x{args.height}",
                            device=torch.cuda.get_device_name(0), sweep=results)
     with open(args.out, "w") as f:
         json.dump(table, f, indent=1)
