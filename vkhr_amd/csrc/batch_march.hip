@@ -295,6 +295,10 @@ extern "C" int vkhr_batch_launch(vkhr_batch* b, vkhr_stream stream) {
         }
         check(hipGetLastError());
         check(hipStreamEndCapture(s, &b->graph));
+        if (err != hipSuccess && b->graph) {  // the next launch captures afresh
+            (void)hipGraphDestroy(b->graph);
+            b->graph = nullptr;
+        }
         VKHR_REQUIRE(err == hipSuccess, VKHR_ERR_HIP, "batch graph capture failed: %s", hipGetErrorString(err));
         VKHR_CHECK_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
