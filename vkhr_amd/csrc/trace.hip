@@ -102,10 +102,27 @@ __device__ __forceinline__ uint32_t parent_of(const TraceBvh& b, uint32_t x) {
 
 // Closest (kAny = false) or any (kAny = true) hit of the ray in [tnear, tfar];
 // closest ties go to the lowest segment index.
+// A short stack of the deepest pending siblings of one lane, in LDS (entry
+// k of lane x at node[k * stride + x], t[k * stride + x]): the far child
+// and its entry distance, pushed when both children are hit.  The trail
+// bits stay the record of every pending level; the stack caches the deepest
+// ones, most recent on top, so a pop visits exactly the sibling the parent
+// walk would find, without the walk and the re-read of the parent's node.
+// When it is full the oldest (shallowest) entry is dropped, and that level
+// is found by the parent walk once the stack is empty.
+constexpr uint32_t kTraceStack = 16;
+struct TraceStack {
+    uint32_t* node;
+    float* t;
+    uint32_t stride;
+};
+
 template <bool kAny>
-__device__ Hit trace(const TraceBvh& b, float3 o, float3 d, float tnear, float tfar, uint32_t skip = kNone) {
+__device__ Hit trace(const TraceBvh& b, float3 o, float3 d, float tnear, float tfar, TraceStack stk,
+                     uint32_t skip = kNone) {
     Hit best{tfar, 0.0f, kNone, kNone};
     if (b.n == 0u) return best;
+    uint32_t top = 0u, count = 0u;  // ring buffer: top entry index, entries held
     const float dd = dot3(d, d);
     const float3 inv = safe_inverse(d);
     auto test_leaf = [&](uint32_t leaf) -> bool {
@@ -137,6 +154,10 @@ __device__ Hit trace(const TraceBvh& b, float3 o, float3 d, float tnear, float t
             if (hl && hr) {
                 next = tl <= tr ? cl : cr;
                 trail = (trail << 1) | 1ull;
+                top = count ? (top + 1u) % kTraceStack : top;
+                count = count < kTraceStack ? count + 1u : count;
+                stk.node[top * stk.stride] = tl <= tr ? cr : cl;
+                stk.t[top * stk.stride] = tl <= tr ? tr : tl;
             } else if (hl || hr) {
                 next = hl ? cl : cr;
                 trail <<= 1;
@@ -146,6 +167,25 @@ __device__ Hit trace(const TraceBvh& b, float3 o, float3 d, float tnear, float t
                 if (!(next & kLeaf)) continue;  // open it
                 if (test_leaf(next & ~kLeaf) && kAny) return best;
             }
+        }
+        if (trail == 0ull) return best;
+        if (count) {
+            // the deepest pending level is the lowest set trail bit; its sibling is the stack top
+            trail >>= __builtin_ctzll(trail);
+            trail ^= 1ull;
+            const uint32_t sib = stk.node[top * stk.stride];
+            const float ts = stk.t[top * stk.stride];
+            top = (top + kTraceStack - 1u) % kTraceStack;
+            --count;
+            cur = sib;
+            expand = false;
+            if (!(ts <= best.t)) continue;  // culled by the closer hit found meanwhile
+            if (sib & kLeaf) {
+                if (test_leaf(sib & ~kLeaf) && kAny) return best;
+                continue;
+            }
+            expand = true;
+            continue;
         }
         // back up to the deepest level whose sibling is pending
         while ((trail & 1ull) == 0ull) {
@@ -385,11 +425,14 @@ refit_kernel(int n, const float4* __restrict__ leaf_lo, const float4* __restrict
 __global__ void __launch_bounds__(256)
 rays_kernel(TraceBvh b, const float* __restrict__ rays, uint32_t n, int any, float* t_out, float* u_out,
             uint32_t* seg_out) {
+    __shared__ uint32_t stk_node[kTraceStack * 256];
+    __shared__ float stk_t[kTraceStack * 256];
+    const TraceStack stk{stk_node + threadIdx.x, stk_t + threadIdx.x, blockDim.x};
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float* r = rays + 8ull * i;
     const float3 o = make_float3(r[0], r[1], r[2]), d = make_float3(r[4], r[5], r[6]);
-    const Hit h = any ? trace<true>(b, o, d, r[3], r[7]) : trace<false>(b, o, d, r[3], r[7]);
+    const Hit h = any ? trace<true>(b, o, d, r[3], r[7], stk) : trace<false>(b, o, d, r[3], r[7], stk);
     const bool hit = h.seg != kNone;
     if (t_out) t_out[i] = hit ? h.t : 0.0f;
     if (u_out) u_out[i] = hit ? h.u : 0.0f;
@@ -433,6 +476,9 @@ constexpr float kEpsilon = 0.000001f;  // Ray::Epsilon (ray_tracer/ray.hh:15)
 
 // One sample of Raytracer::draw per pixel; 16x16 pixels per block, 8x8 per wave.
 __global__ void __launch_bounds__(256) frame_kernel(FrameArgs a) {
+    __shared__ uint32_t stk_node[kTraceStack * 256];
+    __shared__ float stk_t[kTraceStack * 256];
+    const TraceStack stk{stk_node + threadIdx.x, stk_t + threadIdx.x, blockDim.x};
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t i = blockIdx.x * 16u + (lane & 7u) + 8u * (wave & 1u);
     const uint32_t j = blockIdx.y * 16u + (lane >> 3) + 8u * (wave >> 1);
@@ -446,7 +492,7 @@ __global__ void __launch_bounds__(256) frame_kernel(FrameArgs a) {
     const float fy = float(j) + sample(a.seed, pix, a.frame, 1, 0.0f, 1.0f);
     const float3 d = make_float3((fx * X.x + fy * Y.x) + Z.x, (fx * X.y + fy * Y.y) + Z.y, (fx * X.z + fy * Y.z) + Z.z);
     float color[3] = {1.0f, 1.0f, 1.0f};
-    const Hit h = trace<false>(a.b, eye, d, 0.0f, INFINITY);
+    const Hit h = trace<false>(a.b, eye, d, 0.0f, INFINITY, stk);
     if (h.seg != kNone) {
         const float3 P = make_float3(eye.x + d.x * h.t, eye.y + d.y * h.t, eye.z + d.z * h.t);
         const uint32_t skip = a.ignore_self ? h.seg : kNone;  // secondary rays may skip their own segment
@@ -456,7 +502,7 @@ __global__ void __launch_bounds__(256) frame_kernel(FrameArgs a) {
             const float3 sd = make_float3(a.light[0] + sample(a.seed, pix, a.frame, 2, -16.0f, 16.0f),
                                           a.light[1] + sample(a.seed, pix, a.frame, 3, -16.0f, 16.0f),
                                           a.light[2] + sample(a.seed, pix, a.frame, 4, -16.0f, 16.0f));
-            const bool occluded = trace<true>(a.b, P, sd, kEpsilon, INFINITY, skip).seg != kNone;
+            const bool occluded = trace<true>(a.b, P, sd, kEpsilon, INFINITY, stk, skip).seg != kNone;
             if (!occluded || !a.shadows_on) {
                 if (a.method == 0) {
                     const uint32_t ia = a.indices[2ull * h.seg], ib = a.indices[2ull * h.seg + 1];
@@ -485,7 +531,7 @@ __global__ void __launch_bounds__(256) frame_kernel(FrameArgs a) {
                                           sample(a.seed, pix, a.frame, 7, -1.0f, 1.0f));
             const float len = sqrtf(dot3(rd, rd));
             const float t_max = len > 0.0f ? (a.ao_radius / len) * 1.001f + kEpsilon : INFINITY;
-            const Hit o = trace<false>(a.b, P, rd, kEpsilon, t_max, skip);
+            const Hit o = trace<false>(a.b, P, rd, kEpsilon, t_max, stk, skip);
             float ao = 2.0f;
             if (o.seg != kNone) {
                 const float3 q = make_float3((P.x + rd.x * o.t) - P.x, (P.y + rd.y * o.t) - P.y,
