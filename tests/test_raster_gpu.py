@@ -157,6 +157,19 @@ def test_background_and_variants(c0, device):
     c0.rp = params.render(renderer=0)
 
 
+def test_uneven_volume_resolution(device):
+    """The LAO's per-axis cell strides and range tests at a non-cubic volume:
+    fragments of a sample of segments against the oracle, and the resolve."""
+    s = Scene(synth.strands_10k(), (96, 40, 150), 320, 240, device, azimuth=200.0)
+    ppll, cap = s.raster(1 << 21)
+    rng = np.random.default_rng(5)
+    pairs = rng.choice(s.idx.shape[0] // 2, 4000, replace=False)
+    rep = _fragment_parity(s, ppll, cap, pairs=pairs)
+    print("uneven volume raster", rep)
+    assert rep["fragments"] > 1000
+    _resolve_parity(ppll)
+
+
 def test_pool_overflow_drops_without_out_of_bounds(c0, device):
     n = 5000
     guard = 64
