@@ -230,6 +230,9 @@ extern "C" int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styl
         while (s < n_styles - (n_groups - g) && cost[s] < target) ++s;
         bounds[g] = s;
     }
+    int dev = 0, cus = 0;
+    VKHR_CHECK_HIP(hipGetDevice(&dev));
+    VKHR_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     vkhr_batch* b = new vkhr_batch();
     b->groups.resize(n_groups);
     for (uint32_t g = 0; g < n_groups; ++g) {
@@ -253,9 +256,6 @@ extern "C" int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styl
             return VKHR_ERR_HIP;
         }
     }
-    int dev = 0, cus = 0;
-    VKHR_CHECK_HIP(hipGetDevice(&dev));
-    VKHR_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     b->march_grid = unsigned(cus) * t.blocks_per_cu;
     *out = b;
     return VKHR_OK;

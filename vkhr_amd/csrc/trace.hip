@@ -591,8 +591,10 @@ extern "C" int vkhr_bvh_build(const float* xyz, const float* radius, uint32_t n_
                  "null or implausible geometry pointer");
     VKHR_REQUIRE(n < (1u << 30), VKHR_ERR_OVERFLOW, "too many segments");  // int range search in the tree build
     hipStream_t s = as_stream(stream);
+    int device = 0;
+    VKHR_CHECK_HIP(hipGetDevice(&device));
     vkhr_bvh* b = new vkhr_bvh();
-    VKHR_CHECK_HIP(hipGetDevice(&b->device));
+    b->device = device;
     b->n = n;
     if (n == 0) {
         *out = b;
