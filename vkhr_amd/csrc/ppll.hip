@@ -100,21 +100,43 @@ ppll_resolve_kernel(const uint32_t* __restrict__ heads, const uint32_t* __restri
         resolved = blend(resolved, unpack_unorm4x8(fc));
         head = next;
     }
-    // the k-buffer, back to front
+    // The k-buffer, back to front (resolve.comp:81-99).  Each of the K rounds
+    // there blends the slot of greatest depth above 0 -- the lowest slot
+    // among equal depths, since the scan's running maximum starts at 0 and
+    // takes only strictly greater depths -- and zeroes its depth; once no
+    // depth above 0 is left, a round finds slot 0 and blends slot 0's colour
+    // again.  The same sequence comes from one descending sort of the keys
+    // (depth bits << 4 | K-1-slot) for depths above 0 (positive float bits
+    // order as the floats; NaN is not above 0) and 0 for the others: a
+    // bitonic network over registers, 80 compare-exchanges instead of K
+    // scans of K slots.
+    const uint32_t color0 = color[0];
+    unsigned long long key[kK];
 #pragma unroll
-    for (int r = 0; r < kK; ++r) {
-        float far_depth;
-        int far_idx = furthest(depth, far_depth);
-        uint32_t c = 0u;
+    for (int k = 0; k < kK; ++k)
+        key[k] = depth[k] > 0.0f
+                     ? (static_cast<unsigned long long>(__float_as_uint(depth[k])) << 4) | unsigned(kK - 1 - k)
+                     : 0ull;
 #pragma unroll
-        for (int k = 0; k < kK; ++k) {
-            if (k == far_idx) {
-                depth[k] = 0.0f;
-                c = color[k];
+    for (int size = 2; size <= kK; size <<= 1)
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+            for (int i = 0; i < kK; ++i) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    // runs of `size` alternate direction; the last run (all K) is descending
+                    const bool out_of_order = (i & size) == 0 ? key[i] < key[j] : key[i] > key[j];
+                    const unsigned long long ki = key[i], kj = key[j];
+                    const uint32_t ci = color[i], cj = color[j];
+                    key[i] = out_of_order ? kj : ki;
+                    key[j] = out_of_order ? ki : kj;
+                    color[i] = out_of_order ? cj : ci;
+                    color[j] = out_of_order ? ci : cj;
+                }
             }
-        }
-        resolved = blend(resolved, unpack_unorm4x8(c));
-    }
+#pragma unroll
+    for (int r = 0; r < kK; ++r) resolved = blend(resolved, unpack_unorm4x8(key[r] ? color[r] : color0));
     out[pix] = resolved;
 }
 
