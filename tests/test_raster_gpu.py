@@ -170,6 +170,38 @@ def test_uneven_volume_resolution(device):
     _resolve_parity(ppll)
 
 
+def test_resolve_of_crafted_lists(device):
+    """Node lists written on the host, 0-56 nodes per pixel (tails of up to
+    40 past the 16 slots), with equal depths (the lowest slot blends first),
+    depths of 0, -0, negative and NaN (never taken from the slots: slot 0's
+    colour blends instead), and strong, distinct colours so that any change
+    of blend order is far outside the tolerance."""
+    rng = np.random.default_rng(11)
+    w, h = 64, 32
+    special = np.array([0.5, 0.5, 0.25, 0.0, -0.0, -0.3, np.nan, 0.75, 0.75, 1.0], np.float32)
+    heads = np.full(w * h, NULL, np.uint32)
+    nodes = []
+    for p in range(w * h):
+        for _ in range(int(rng.integers(0, 57))):
+            if rng.random() < 0.6:
+                d = special[rng.integers(len(special))]
+            else:
+                d = np.float32(rng.integers(1, 8) / 8)
+            c = rng.integers(0, 256, 4)
+            c[3] = rng.integers(96, 256)
+            colour = int(c[0]) | int(c[1]) << 8 | int(c[2]) << 16 | int(c[3]) << 24
+            nodes.append((colour, int(np.float32(d).view(np.uint32)), int(heads[p])))
+            heads[p] = len(nodes) - 1
+    nodes = np.asarray(nodes, np.uint32).reshape(-1)
+    ppll = D.PPLL(w, h, len(nodes) // 3, device)
+    ppll.heads.copy_(torch.from_numpy(heads.view(np.int32)))
+    ppll.nodes.copy_(torch.from_numpy(nodes.view(np.int32)))
+    out = ppll.resolve().cpu().numpy().astype(np.float64)
+    want = ORs.resolve(heads, nodes, w, h)
+    err = float(np.abs(out - want).max())
+    assert err <= RESOLVE_TOL, err
+
+
 def test_pool_overflow_drops_without_out_of_bounds(c0, device):
     n = 5000
     guard = 64
