@@ -7,11 +7,14 @@
  * VKHR_ERR_* code, with a message available from vkhr_last_error().  No C++
  * exception crosses this boundary.  Calls on distinct buffers are reentrant;
  * the only hidden state is a scratch area per device and stream that the
- * library allocates on first use and keeps until the process ends
- * (reductions, counters, the adjoint's cell accumulator: used by
- * vkhr_aabb_compute, vkhr_volume_finalize and vkhr_raymarch_adjoint).
- * Calls on one stream are ordered by it; calls on different streams use
- * different areas, so they may run concurrently.
+ * library allocates on first use and keeps until vkhr_release_scratch on
+ * that stream (or the process ends): reductions, counters, the adjoint's
+ * cell accumulator, used by vkhr_aabb_compute, vkhr_volume_finalize and
+ * vkhr_raymarch_adjoint.  Calls on one stream are ordered by it; calls on
+ * different streams use different areas, so they may run concurrently.
+ * hipStreamPerThread gets one area per calling thread.  An area that must
+ * grow synchronizes its stream and frees the old buffer (during graph
+ * capture the old buffer is kept until the release instead).
  *
  * Layouts:
  *   xyz           float32 x,y,z per vertex, tightly packed (12 B / vertex)
@@ -126,6 +129,16 @@ const char* vkhr_last_error(void);
 /* Binds the current device, checks it is gfx950 and sets up the per-device
  * scratch.  Optional: every entry point calls it on first use. */
 int vkhr_init(void);
+/* Synchronizes the stream, then frees the scratch area the library holds
+ * for it on the current device (and on this thread, for
+ * hipStreamPerThread).  A later call on the stream allocates a new area.
+ * Call it before destroying a stream the library has used, so that a host
+ * that creates streams per request does not accumulate areas.  No
+ * counterpart in the reference (which keeps no device scratch). */
+int vkhr_release_scratch(vkhr_stream stream);
+/* Bytes of the scratch area held for this stream on the current device (0
+ * when none). */
+size_t vkhr_scratch_bytes(vkhr_stream stream);
 
 /* ------------------------------------------------------ style preparation */
 

@@ -180,27 +180,3 @@ def test_autograd_function_and_descent(prob):
             th = th - lr * torch.sign(t.grad)
     print("descent", losses, th.tolist())
     assert losses[-1] < 0.5 * losses[0]
-
-
-def test_adjoint_to_forward_time_ratio(prob):
-    rgba, rec = prob.forward(prob.dens)
-    d_rgba = 2.0 * (rgba - prob.target)
-    gd = torch.zeros_like(prob.dens)
-    gp = torch.zeros(3, device=gd.device)
-    for _ in range(3):
-        prob.forward(prob.dens)
-        D.raymarch_adjoint(prob.vol(prob.dens), RES, prob.cam, prob.light, prob.sp, prob.rp, prob.diff, W, H, rec,
-                           d_rgba, gd, gp)
-    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    e[0].record()
-    for _ in range(10):
-        prob.forward(prob.dens, prob.diff)
-    e[1].record()
-    e[2].record()
-    for _ in range(10):
-        D.raymarch_adjoint(prob.vol(prob.dens), RES, prob.cam, prob.light, prob.sp, prob.rp, prob.diff, W, H, rec,
-                           d_rgba, gd, gp)
-    e[3].record()
-    torch.cuda.synchronize()
-    tf, tb = e[0].elapsed_time(e[1]) / 10, e[2].elapsed_time(e[3]) / 10
-    print(f"forward {tf:.3f} ms, adjoint {tb:.3f} ms, ratio {tb / tf:.2f}")

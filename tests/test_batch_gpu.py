@@ -68,29 +68,6 @@ def test_batch_matches_per_style_bit_for_bit(batch32, tuning):
     assert sum(int((b.rgba[..., 3] > 0).sum()) for b in batch32) > 1_000_000
 
 
-def test_batch_speedup_over_per_style_launches(batch32):
-    batch = Batch(batch32, W, H, tuning=load_tuning("batch"))
-    for _ in range(2):
-        batch.launch()
-        for b in batch32:
-            b.run_plain()
-    torch.cuda.synchronize()
-    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    e[0].record()
-    for _ in range(3):
-        for b in batch32:
-            b.run_plain()
-    e[1].record()
-    torch.cuda.synchronize()
-    e[2].record(batch.stream)
-    for _ in range(3):
-        batch.launch(wait=False)
-    e[3].record(batch.stream)
-    torch.cuda.synchronize()
-    plain, batched = e[0].elapsed_time(e[1]) / 3, e[2].elapsed_time(e[3]) / 3
-    print(f"per-style launches {plain:.3f} ms, batched graph {batched:.3f} ms, speed-up {plain / batched:.2f}")
-
-
 def test_invalid_batch_arguments(batch32):
     bad = capi.Tuning()
     bad.tile_w, bad.tile_h, bad.blocks_per_cu = 8, 4, 4    # 32 pixels per wave

@@ -120,6 +120,7 @@ I32 = C.c_int
 SIGNATURES = {
     "vkhr_abi_version": [],
     "vkhr_init": [],
+    "vkhr_release_scratch": [P],
     "vkhr_aabb_compute": [P, U32, C.POINTER(AABB), P],
     "vkhr_generate_indices": [P, U32, P, P],
     "vkhr_sort_segments": [P, U32, P, U32, C.POINTER(AABB), P, P],
@@ -158,7 +159,8 @@ SIGNATURES = {
     "vkhr_volume_filter": [P, U32, U32, U32, U32, P, P],
 }
 # entry points that do not return a status
-SIZE_FUNCTIONS = {"vkhr_march_record_bytes": [U32, U32], "vkhr_occupancy_words": [U32, U32, U32]}
+SIZE_FUNCTIONS = {"vkhr_march_record_bytes": [U32, U32], "vkhr_occupancy_words": [U32, U32, U32],
+                  "vkhr_scratch_bytes": [P]}
 
 
 def declared_symbols() -> list[str]:

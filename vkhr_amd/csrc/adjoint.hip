@@ -412,9 +412,9 @@ extern "C" int vkhr_raymarch_adjoint(const vkhr_volume* volume, const vkhr_camer
     a->d_rgba = reinterpret_cast<const float4*>(d_rgba);
     a->d_params = d_params;
     finish_adjoint_args(a);
-    const size_t n_cells = size_t(volume->nx + 1) * (volume->ny + 1) * (volume->nz + 1);
+    const size_t n_cells = (size_t(volume->nx) + 1) * (size_t(volume->ny) + 1) * (size_t(volume->nz) + 1);
     void* cg = nullptr;
-    if ((rc = scratch(n_cells * 8 * sizeof(float), as_stream(stream), &cg))) {
+    if ((rc = stream_scratch(n_cells * 8 * sizeof(float), as_stream(stream), &cg))) {
         delete a;
         return rc;
     }
@@ -474,9 +474,8 @@ int build_adjoint_batch(vkhr_adjoint_batch* b, const vkhr_adjoint_style* styles,
         const vkhr_adjoint_style& in = styles[s];
         VKHR_REQUIRE(in.volume.density_f32 && in.volume.tangents && in.d_density && in.d_params, VKHR_ERR_INVALID,
                      "style %u: the float density, tangents, d_density and d_params are required", s);
-        VKHR_REQUIRE(in.volume.nx && in.volume.ny && in.volume.nz, VKHR_ERR_INVALID, "style %u: empty volume", s);
-        const uint64_t cells = uint64_t(in.volume.nx + 1) * (in.volume.ny + 1) * (in.volume.nz + 1);
-        VKHR_REQUIRE(cells < (uint64_t(1) << 32), VKHR_ERR_OVERFLOW, "style %u: volume too large", s);
+        if ((rc = check_volume(in.volume.nx, in.volume.ny, in.volume.nz, true, "adjoint style volume"))) return rc;
+        const uint64_t cells = (uint64_t(in.volume.nx) + 1) * (uint64_t(in.volume.ny) + 1) * (uint64_t(in.volume.nz) + 1);
         void* cg = nullptr;
         VKHR_CHECK_HIP(hipMalloc(&cg, cells * 8 * sizeof(float)));
         b->allocations.push_back(cg);

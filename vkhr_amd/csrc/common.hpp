@@ -43,10 +43,32 @@ void clear_error();
 // Binds per-device state; returns VKHR_OK or an error code.
 int ensure_device();
 
+// Largest volume side any entry point accepts.  Every entry point that takes
+// a volume validates its sides with check_volume before any size arithmetic,
+// so no product of sides can wrap.
+constexpr uint32_t kMaxVolumeSide = 1u << 16;
+
+// Volume dimension check shared by every entry point that takes a volume:
+// each side in [1, kMaxVolumeSide] (VKHR_ERR_INVALID for a zero side,
+// VKHR_ERR_OVERFLOW above), at most 2^31 voxels, and with cells the cell
+// lattice (nx+1)(ny+1)(nz+1) below 2^32 with (ny+1)(nz+1) below 2^24 (the
+// kernels index cells with 24-bit multiplies and 32-bit sums).  All
+// arithmetic is 64-bit.  `what` names the volume in the error message.
+int check_volume(uint32_t nx, uint32_t ny, uint32_t nz, bool cells, const char* what);
+
 // Scratch of one device and stream: a device buffer of at least `bytes`
-// bytes owned by the library until process end.  Calls on one stream are
-// ordered by the stream; calls on different streams get different areas.
-int scratch(size_t bytes, hipStream_t stream, void** out);
+// bytes owned by the library until vkhr_release_scratch on that stream (or
+// process end).  Calls on one stream are ordered by the stream; calls on
+// different streams (and hipStreamPerThread on different threads) get
+// different areas.  Growing an area synchronizes the stream before freeing
+// the old buffer (outside graph capture).
+int stream_scratch(size_t bytes, hipStream_t stream, void** out);
+
+// Synchronizes the stream and frees its area (vkhr_release_scratch).
+int release_stream_scratch(hipStream_t stream);
+
+// Bytes held by the area of the calling thread's device and this stream.
+size_t stream_scratch_bytes(hipStream_t stream);
 
 inline hipStream_t as_stream(vkhr_stream s) { return reinterpret_cast<hipStream_t>(s); }
 

@@ -122,7 +122,7 @@ extern "C" int vkhr_aabb_compute(const float* xyz, uint32_t n_vertices, vkhr_aab
     float mm[6] = {0, 0, 0, 0, 0, 0};
     if (n_vertices) {
         void* scr = nullptr;
-        if ((rc = scratch(sizeof(float) * (6 * kAabbGrid + 8), as_stream(stream), &scr))) return rc;
+        if ((rc = stream_scratch(sizeof(float) * (6 * kAabbGrid + 8), as_stream(stream), &scr))) return rc;
         float* partial = static_cast<float*>(scr);
         float* result = partial + 6 * kAabbGrid;
         hipStream_t s = as_stream(stream);

@@ -595,11 +595,10 @@ extern "C" int vkhr_raster_strands(const float* xyz, const float* tangents, cons
     const bool adsm = rp->adsm_on == 1 && rp->shading_model != 3;
     const bool lao = rp->shading_model != 2;
     VKHR_REQUIRE(!adsm || (shadow_map && shadow_w && shadow_h), VKHR_ERR_INVALID, "ADSM needs a shadow map");
-    VKHR_REQUIRE(!lao || (vol && (vol->cells || vol->density_f32) && vol->nx && vol->ny && vol->nz),
-                 VKHR_ERR_INVALID, "LAO needs the density volume");
+    VKHR_REQUIRE(!lao || (vol && (vol->cells || vol->density_f32)), VKHR_ERR_INVALID,
+                 "LAO needs the density volume");
     // the LAO taps index cells with 32-bit sums, the PCF taps texels with 32-bit row offsets
-    VKHR_REQUIRE(!lao || uint64_t(vol->nx + 1) * (vol->ny + 1) * (vol->nz + 1) < (uint64_t(1) << 32),
-                 VKHR_ERR_OVERFLOW, "volume %ux%ux%u too large", vol->nx, vol->ny, vol->nz);
+    if (lao && (rc = check_volume(vol->nx, vol->ny, vol->nz, true, "LAO volume"))) return rc;
     VKHR_REQUIRE(!adsm || (shadow_w <= 16384 && shadow_h <= 16384), VKHR_ERR_INVALID, "shadow map %ux%u",
                  shadow_w, shadow_h);
     VKHR_REQUIRE(sp->strand_radius > 0.0f && sp->strand_radius < 64.0f, VKHR_ERR_INVALID, "strand_radius %g",

@@ -95,12 +95,10 @@ int make_raymarch_args(const vkhr_volume* vol, const vkhr_camera* cam, const vkh
     VKHR_REQUIRE(vol && cam && light && sp && rp, VKHR_ERR_INVALID, "null parameter struct");
     VKHR_REQUIRE(width && height && width <= 16384 && height <= 16384, VKHR_ERR_INVALID, "bad image size %ux%u",
                  width, height);
-    VKHR_REQUIRE(vol->nx && vol->ny && vol->nz, VKHR_ERR_INVALID, "zero volume resolution");
     VKHR_REQUIRE(vol->cells || vol->density_f32, VKHR_ERR_INVALID, "volume needs cells or density_f32");
     // the march indexes cells with 24-bit multiplies (and 32-bit sums)
-    VKHR_REQUIRE(uint64_t(vol->nx + 1) * (vol->ny + 1) * (vol->nz + 1) < (uint64_t(1) << 32) &&
-                     uint64_t(vol->ny + 1) * (vol->nz + 1) < (uint64_t(1) << 24) && vol->nx + 1 < (1u << 24),
-                 VKHR_ERR_OVERFLOW, "volume %ux%ux%u too large", vol->nx, vol->ny, vol->nz);
+    int rc = check_volume(vol->nx, vol->ny, vol->nz, true, "raymarch volume");
+    if (rc) return rc;
     VKHR_REQUIRE(vol->tangents || rp->shading_model != 0, VKHR_ERR_INVALID, "Kajiya-Kay needs tangents");
     VKHR_REQUIRE(rp->raycast_steps >= 1.0f && rp->raycast_steps <= 65536.0f, VKHR_ERR_INVALID,
                  "raycast_steps %g out of range", rp->raycast_steps);

@@ -4,9 +4,12 @@
 
 #include <cstdarg>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
-#include <utility>
+#include <thread>
+#include <tuple>
+#include <vector>
 
 namespace vkhr {
 
@@ -19,8 +22,18 @@ bool g_checked[kMaxDevices] = {false};
 struct Scratch {
     void* ptr = nullptr;
     size_t bytes = 0;
+    std::vector<void*> retired;  // outgrown buffers still owned by work captured into a graph
 };
-std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;  // (device, stream) -> area
+// (device, stream, thread) -> area.  The thread is part of the key only for
+// hipStreamPerThread, one handle value that names a different stream on
+// every thread.
+using ScratchKey = std::tuple<int, hipStream_t, size_t>;
+std::map<ScratchKey, Scratch> g_scratch;
+
+ScratchKey scratch_key(int dev, hipStream_t stream) {
+    const size_t thread = stream == hipStreamPerThread ? std::hash<std::thread::id>()(std::this_thread::get_id()) : 0;
+    return ScratchKey(dev, stream, thread);
+}
 }  // namespace
 
 void set_error(const char* fmt, ...) {
@@ -46,16 +59,44 @@ int ensure_device() {
     return VKHR_OK;
 }
 
-int scratch(size_t bytes, hipStream_t stream, void** out) {
+int check_volume(uint32_t nx, uint32_t ny, uint32_t nz, bool cells, const char* what) {
+    VKHR_REQUIRE(nx && ny && nz, VKHR_ERR_INVALID, "%s: zero volume resolution %ux%ux%u", what, nx, ny, nz);
+    VKHR_REQUIRE(nx <= kMaxVolumeSide && ny <= kMaxVolumeSide && nz <= kMaxVolumeSide, VKHR_ERR_OVERFLOW,
+                 "%s: volume %ux%ux%u has a side above %u", what, nx, ny, nz, kMaxVolumeSide);
+    const uint64_t x = nx, y = ny, z = nz;  // each <= 2^16: no product below wraps
+    VKHR_REQUIRE(x * y * z <= (uint64_t(1) << 31), VKHR_ERR_OVERFLOW, "%s: volume %ux%ux%u exceeds 2^31 voxels",
+                 what, nx, ny, nz);
+    if (cells)
+        VKHR_REQUIRE((x + 1) * (y + 1) * (z + 1) < (uint64_t(1) << 32) && (y + 1) * (z + 1) < (uint64_t(1) << 24),
+                     VKHR_ERR_OVERFLOW, "%s: volume %ux%ux%u has too many cells for 32-bit cell indices", what, nx,
+                     ny, nz);
+    return VKHR_OK;
+}
+
+int stream_scratch(size_t bytes, hipStream_t stream, void** out) {
     int rc = ensure_device();
     if (rc) return rc;
     int dev = 0;
     VKHR_CHECK_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_mutex);
-    Scratch& area = g_scratch[std::make_pair(dev, stream)];
+    Scratch& area = g_scratch[scratch_key(dev, stream)];
     if (area.bytes < bytes) {
-        // Grow: the old buffer may still be in use by queued work, so it is
-        // kept (not freed) -- growth happens a handful of times per process.
+        // Grow.  Work already queued on this stream may still read the old
+        // buffer: it is freed once the stream has drained, or, while the
+        // stream is being captured into a graph (which may replay it later),
+        // kept until vkhr_release_scratch.
+        if (area.ptr) {
+            hipStreamCaptureStatus status = hipStreamCaptureStatusNone;
+            VKHR_CHECK_HIP(hipStreamIsCapturing(stream, &status));
+            if (status == hipStreamCaptureStatusNone) {
+                VKHR_CHECK_HIP(hipStreamSynchronize(stream));
+                VKHR_CHECK_HIP(hipFree(area.ptr));
+            } else {
+                area.retired.push_back(area.ptr);
+            }
+            area.ptr = nullptr;
+            area.bytes = 0;
+        }
         size_t want = bytes < (1u << 20) ? (1u << 20) : bytes;
         void* p = nullptr;
         hipError_t e = hipMalloc(&p, want);
@@ -68,6 +109,33 @@ int scratch(size_t bytes, hipStream_t stream, void** out) {
     }
     *out = area.ptr;
     return VKHR_OK;
+}
+
+int release_stream_scratch(hipStream_t stream) {
+    int rc = ensure_device();
+    if (rc) return rc;
+    int dev = 0;
+    VKHR_CHECK_HIP(hipGetDevice(&dev));
+    Scratch area;
+    {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        auto it = g_scratch.find(scratch_key(dev, stream));
+        if (it == g_scratch.end()) return VKHR_OK;
+        area = it->second;
+        g_scratch.erase(it);
+    }
+    VKHR_CHECK_HIP(hipStreamSynchronize(stream));
+    if (area.ptr) VKHR_CHECK_HIP(hipFree(area.ptr));
+    for (void* p : area.retired) VKHR_CHECK_HIP(hipFree(p));
+    return VKHR_OK;
+}
+
+size_t stream_scratch_bytes(hipStream_t stream) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    auto it = g_scratch.find(scratch_key(dev, stream));
+    return it == g_scratch.end() ? 0 : it->second.bytes;
 }
 
 __global__ void fill_u32_kernel(uint32_t* __restrict__ dst, size_t n, uint32_t value) {
@@ -94,5 +162,12 @@ int vkhr_init(void) {
     vkhr::clear_error();
     return vkhr::ensure_device();
 }
+
+int vkhr_release_scratch(vkhr_stream stream) {
+    vkhr::clear_error();
+    return vkhr::release_stream_scratch(vkhr::as_stream(stream));
+}
+
+size_t vkhr_scratch_bytes(vkhr_stream stream) { return vkhr::stream_scratch_bytes(vkhr::as_stream(stream)); }
 
 }  // extern "C"

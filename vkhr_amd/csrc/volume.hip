@@ -152,6 +152,7 @@ extern "C" int vkhr_volume_downsample(const uint8_t* in, uint32_t nx, uint32_t n
     int rc = ensure_device();
     if (rc) return rc;
     VKHR_REQUIRE(in && out, VKHR_ERR_INVALID, "in/out is null");
+    if ((rc = check_volume(nx, ny, nz, false, "downsample input"))) return rc;
     VKHR_REQUIRE(nx >= 2 && ny >= 2 && nz >= 2 && nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0, VKHR_ERR_INVALID,
                  "downsample needs even sizes, got %ux%ux%u", nx, ny, nz);
     VKHR_REQUIRE(filter >= 0 && filter <= 2, VKHR_ERR_INVALID, "unknown filter %d", filter);
@@ -168,7 +169,7 @@ extern "C" int vkhr_volume_filter(const float* in, uint32_t nx, uint32_t ny, uin
     int rc = ensure_device();
     if (rc) return rc;
     VKHR_REQUIRE(in && out && in != out, VKHR_ERR_INVALID, "in/out is null or the same");
-    VKHR_REQUIRE(nx && ny && nz && uint64_t(nx) * ny * nz <= (uint64_t(1) << 31), VKHR_ERR_INVALID, "bad volume");
+    if ((rc = check_volume(nx, ny, nz, false, "filter input"))) return rc;
     VKHR_REQUIRE(kernel_width % 2 == 1 && kernel_width <= 15, VKHR_ERR_INVALID,
                  "kernel_width %u: odd and at most 15", kernel_width);
     const float sigma = (float(kernel_width) / 2.0f) / 2.4f;
@@ -179,13 +180,14 @@ extern "C" int vkhr_volume_filter(const float* in, uint32_t nx, uint32_t ny, uin
     return VKHR_OK;
 }
 
-extern "C" int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tangent_acc, uint32_t nx, uint32_t ny,
-                                    uint32_t nz, int normalize, uint8_t* density_u8, int8_t* tangent_i8x4,
-                                    float* density_f32, uint64_t* cells, uint32_t* occupancy,
-                                    vkhr_stream stream) {
-    clear_error();
-    int rc = ensure_device();
-    if (rc) return rc;
+namespace vkhr {
+namespace {
+
+// vkhr_volume_finalize with the min/max slots either from the per-stream
+// scratch (minmax_static == nullptr) or from a buffer the library owns.
+int finalize(const uint32_t* counts, const int64_t* tangent_acc, uint32_t nx, uint32_t ny, uint32_t nz,
+             int normalize, uint8_t* density_u8, int8_t* tangent_i8x4, float* density_f32, uint64_t* cells,
+             uint32_t* occupancy, hipStream_t s, uint32_t* minmax_static) {
     VKHR_REQUIRE(density_u8, VKHR_ERR_INVALID, "density_u8 is null");
     VKHR_REQUIRE(reinterpret_cast<uintptr_t>(density_u8) % 4 == 0 && reinterpret_cast<uintptr_t>(density_f32) % 16 == 0 &&
                      reinterpret_cast<uintptr_t>(counts) % 16 == 0 && reinterpret_cast<uintptr_t>(tangent_i8x4) % 16 == 0 &&
@@ -194,19 +196,20 @@ extern "C" int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tange
                  "density_u8 / counts, tangent_acc, density_f32, tangent_i8x4 must be 4 / 16-byte aligned");
     VKHR_REQUIRE(counts || tangent_acc, VKHR_ERR_INVALID, "neither counts nor tangent_acc");
     VKHR_REQUIRE(!occupancy || cells, VKHR_ERR_INVALID, "occupancy needs cells");
-    VKHR_REQUIRE(nx && ny && nz, VKHR_ERR_INVALID, "zero volume resolution");
+    int rc = check_volume(nx, ny, nz, cells != nullptr, "finalize volume");
+    if (rc) return rc;
     VKHR_REQUIRE(!tangent_i8x4 || tangent_acc, VKHR_ERR_INVALID, "tangent_i8x4 needs tangent_acc");
     size_t n = size_t(nx) * ny * nz;
-    VKHR_REQUIRE(n <= (size_t(1) << 31), VKHR_ERR_OVERFLOW, "volume too large");
-    VKHR_REQUIRE(!cells || size_t(nx + 1) * (ny + 1) * (nz + 1) < (size_t(1) << 32), VKHR_ERR_OVERFLOW,
-                 "too many cells");
-    hipStream_t s = as_stream(stream);
     uint32_t* mm = nullptr;
     if (normalize) {
-        void* scr = nullptr;
-        if ((rc = scratch(kMinMaxBytes, s, &scr))) return rc;
-        mm = static_cast<uint32_t*>(scr);
-        minmax_init_kernel<<<1, 64, 0, s>>>(mm);
+        if (minmax_static) {
+            mm = minmax_static;
+        } else {
+            void* scr = nullptr;
+            if ((rc = stream_scratch(kMinMaxBytes, s, &scr))) return rc;
+            mm = static_cast<uint32_t*>(scr);
+        }
+        minmax_init_kernel<<<1, 64, 0, s>>>(mm);  // every call starts from empty slots
         VKHR_CHECK_LAUNCH();
     }
     // One block per 1024 voxels (capped): one min/max atomic pair per block,
@@ -226,6 +229,20 @@ extern "C" int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tange
         VKHR_CHECK_LAUNCH();
     }
     return VKHR_OK;
+}
+
+}  // namespace
+}  // namespace vkhr
+
+extern "C" int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tangent_acc, uint32_t nx, uint32_t ny,
+                                    uint32_t nz, int normalize, uint8_t* density_u8, int8_t* tangent_i8x4,
+                                    float* density_f32, uint64_t* cells, uint32_t* occupancy,
+                                    vkhr_stream stream) {
+    clear_error();
+    int rc = ensure_device();
+    if (rc) return rc;
+    return vkhr::finalize(counts, tangent_acc, nx, ny, nz, normalize, density_u8, tangent_i8x4, density_f32, cells,
+                          occupancy, as_stream(stream), nullptr);
 }
 
 extern "C" size_t vkhr_occupancy_words(uint32_t nx, uint32_t ny, uint32_t nz) { return occupancy_words(nx, ny, nz); }

@@ -151,7 +151,8 @@ __device__ __forceinline__ void voxelize_segment(const VoxelGrid& g, const float
 // Voxel grid of a style (hair_style.cc:258-268, 297-307); validates the box.
 inline int make_grid(const vkhr_aabb* bounds, uint32_t nx, uint32_t ny, uint32_t nz, VoxelGrid* g) {
     VKHR_REQUIRE(bounds, VKHR_ERR_INVALID, "bounds is null");
-    VKHR_REQUIRE(nx && ny && nz, VKHR_ERR_INVALID, "zero volume resolution");
+    int rc = check_volume(nx, ny, nz, false, "voxel grid");
+    if (rc) return rc;
     VKHR_REQUIRE(uint64_t(nx) * ny * nz <= (1ull << 24), VKHR_ERR_OVERFLOW,
                  "volume %ux%ux%u exceeds 2^24 voxels (float linear index no longer exact)", nx, ny, nz);
     for (int c = 0; c < 3; ++c)
