@@ -18,6 +18,7 @@ tangent tap, 16 bytes of output per covered ray.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import re
 import os
@@ -44,6 +45,9 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 256 MiB copy kernel three times (the reference for the rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE traffic counters) and exit")
+    ap.add_argument("--summarize-profile", metavar="ROOT",
+                    help="write the raymarch counter summary from rocprofv3 CSV runs under ROOT (no GPU) and exit")
+    ap.add_argument("--out", default=str(PROFILE_SUMMARY), help="summary file of --summarize-profile")
     return ap.parse_args()
 
 
@@ -324,16 +328,174 @@ def time_adjoint_batch(dev):
     return e[0].elapsed_time(e[1]), e[2].elapsed_time(e[3])
 
 
-TRAFFIC_SUMMARY = Path(__file__).resolve().parent / "profiles" / "raymarch_traffic_136k_strands_1080p.md"
+PROFILE_SUMMARY = ROOT / "profiles" / "raymarch_counters_136k_strands_1080p.md"
+# What compiles into the raymarch kernel: its translation unit, the headers
+# it includes and the build flags.  A profile summary holds the hash of these
+# files as they were when it was taken, and its counters are used only for
+# the same sources.
+RAYMARCH_SOURCES = ("vkhr_amd/csrc/raymarch.hip", "vkhr_amd/csrc/march.hpp", "vkhr_amd/csrc/render.hpp",
+                    "vkhr_amd/csrc/common.hpp", "include/vkhr_hip.h", "vkhr_amd/Makefile")
+RAYMARCH_KERNEL = "raymarch_kernel<false>"
+CLOCK_HZ = 2.4e9          # MI355X peak engine clock
+SIMDS = 1024              # 256 CUs x 4 SIMDs
+VALU_CYCLES = 4           # a wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles
 
 
-def measured_traffic():
-    """HBM bytes per raymarch launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
-    if not TRAFFIC_SUMMARY.exists():
-        return None
-    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", TRAFFIC_SUMMARY.read_text())
-    return float(m.group(1)) if m else None
+def source_hash(files=RAYMARCH_SOURCES) -> str:
+    h = hashlib.sha256()
+    for f in files:
+        h.update(f.encode())
+        h.update((ROOT / f).read_bytes())
+    return h.hexdigest()[:16]
+
+
+def profile_counters():
+    """(counters, None) from the committed rocprofv3 summary of the raymarch
+    kernel when it was taken of this tree's kernel sources, else (None,
+    reason)."""
+    if not PROFILE_SUMMARY.exists():
+        return None, f"{PROFILE_SUMMARY.name} missing"
+    m = re.search(r"```json\n(.*?)\n```", PROFILE_SUMMARY.read_text(), re.S)
+    if not m:
+        return None, f"{PROFILE_SUMMARY.name} holds no counter block"
+    data = json.loads(m.group(1))
+    here = source_hash()
+    if data.get("source_hash") != here:
+        return None, f"the profile was taken of other kernel sources ({data.get('source_hash')} != {here})"
+    return data, None
+
+
+def raymarch_roofline(t_ray: float, ray_bytes: float) -> dict:
+    """The raymarch kernel against the resource that binds it, VALU issue:
+    its VALU wave-instructions per launch (from the profile summary of the
+    same kernel sources) x 4 cycles over 1024 SIMDs x 2.4 GHz x the launch
+    time measured in this run.  Beside it the measured DRAM traffic per
+    launch against 8 TB/s, and the gathered-byte figure of the per-ray
+    formula (every sample of every covered ray, skipped ones included),
+    which is not a DRAM figure."""
+    prof, why = profile_counters()
+    peak = SIMDS * CLOCK_HZ / VALU_CYCLES / 1e9   # G wave-instructions per second
+    out = {"bound": "valu", "kernel": RAYMARCH_KERNEL, "unit": "G VALU wave-instructions/s", "peak": peak,
+           "achieved": None, "frac": None, "traffic": None, "seconds_per_launch": t_ray,
+           "profile": f"profiles/{PROFILE_SUMMARY.name}", "source_hash": source_hash()}
+    if prof is None:
+        out["profile_note"] = why
+    else:
+        out["achieved"] = prof["valu_wave_instructions"] / t_ray / 1e9
+        out["frac"] = out["achieved"] / peak
+        out["traffic"] = prof["dram_bytes"]
+        out["dram"] = {"bytes_per_launch": prof["dram_bytes"], "achieved_gbs": prof["dram_bytes"] / t_ray / 1e9,
+                       "frac": prof["dram_bytes"] / t_ray / 1e9 / HBM_PEAK_GBS}
+        out["profile_avg_duration_us"] = prof["avg_duration_us"]
+        out["profile_valu_issue_frac"] = prof["valu_issue_frac"]
+    out["gathered_bytes_effective"] = {
+        "bytes_per_launch": ray_bytes, "gbs": ray_bytes / t_ray / 1e9, "frac_of_hbm": ray_bytes / t_ray / 1e9 / HBM_PEAK_GBS,
+        "note": "the per-ray formula 8 B x (S primary + S + 8 ADSM/LAO taps) + 48 B over every covered ray, samples "
+                "in empty brick regions included although the kernel skips them; not a DRAM figure"}
+    return out
+
+
+def _csv_rows(root: Path, suffix: str):
+    import csv
+    rows = []
+    for f in sorted(root.rglob(f"*{suffix}")):
+        with open(f, newline="") as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def _counter_means(root: Path) -> dict:
+    """{kernel name: {counter: mean per dispatch}} from rocprofv3 --pmc CSV output."""
+    per = {}
+    for r in _csv_rows(root, "counter_collection.csv"):
+        key = (r["Kernel_Name"], r.get("Dispatch_Id", r.get("Correlation_Id")))
+        d = per.setdefault(key, {})
+        d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    out = {}
+    for (name, _), d in per.items():
+        k = out.setdefault(name, {})
+        for c, v in d.items():
+            k.setdefault(c, []).append(v)
+    return {n: {c: sum(v) / len(v) for c, v in d.items()} for n, d in out.items()}
+
+
+def _short(name: str) -> str:
+    name = re.sub(r"\(.*$", "", name.replace("(anonymous namespace)::", ""))
+    return name.replace("vkhr::", "").replace("void ", "")
+
+
+def summarize_profile(root: Path, out: Path):
+    """Writes the profile summary of the raymarch kernel (and the step's
+    other kernels) from rocprofv3 runs of bench.py under root: stats/
+    (--kernel-trace --stats), fetch/ and write/ (--pmc FETCH_SIZE,
+    --pmc WRITE_SIZE), sq/ (--pmc SQ counters), copy_fetch/ and copy_write/
+    (the same two counters over bench.py --copy-calibration)."""
+    stats = {r["Name"]: r for r in _csv_rows(root / "stats", "kernel_stats.csv")}
+    fetch, write = _counter_means(root / "fetch"), _counter_means(root / "write")
+    sq = _counter_means(root / "sq")
+    cf, cw = _counter_means(root / "copy_fetch"), _counter_means(root / "copy_write")
+    copy_name = next(n for n in cf if "copy_kernel" in n or "mb_copy" in n)
+    copy_bytes = 268435456.0
+    f_scale = copy_bytes / (cf[copy_name]["FETCH_SIZE"] * 1024.0)
+    w_scale = copy_bytes / (cw[copy_name]["WRITE_SIZE"] * 1024.0)
+    names = sorted(set(fetch) | set(sq), key=lambda n: -float(stats.get(n, {}).get("TotalDurationNs", 0)))
+    rm = next(n for n in names if RAYMARCH_KERNEL in n)
+    try:
+        commit = subprocess.run(["git", "-C", str(ROOT), "rev-parse", "--short=12", "HEAD"], capture_output=True,
+                                text=True).stdout.strip()
+        dirty = subprocess.run(["git", "-C", str(ROOT), "status", "--porcelain", "--", *RAYMARCH_SOURCES],
+                               capture_output=True, text=True).stdout.strip()
+    except OSError:
+        commit, dirty = "unknown", ""
+    avg_ns = float(stats[rm]["AverageNs"]) if rm in stats else None
+    valu = sq[rm].get("SQ_INSTS_VALU")
+    traffic = fetch[rm]["FETCH_SIZE"] * 1024.0 * f_scale + write[rm]["WRITE_SIZE"] * 1024.0 * w_scale
+    block = {
+        "source_hash": source_hash(), "tree": commit + ("+uncommitted kernel sources" if dirty else ""),
+        "kernel": rm, "avg_duration_us": avg_ns / 1e3 if avg_ns else None,
+        "valu_wave_instructions": valu, "salu_wave_instructions": sq[rm].get("SQ_INSTS_SALU"),
+        "vmem_read_wave_instructions": sq[rm].get("SQ_INSTS_VMEM_RD"), "lds_wave_instructions": sq[rm].get("SQ_INSTS_LDS"),
+        "waves": sq[rm].get("SQ_WAVES"),
+        "fetch_kib": fetch[rm]["FETCH_SIZE"], "write_kib": write[rm]["WRITE_SIZE"],
+        "fetch_scale": f_scale, "write_scale": w_scale, "dram_bytes": traffic,
+        "valu_issue_frac": (valu * VALU_CYCLES / (SIMDS * CLOCK_HZ * avg_ns * 1e-9)) if (valu and avg_ns) else None,
+        "dram_frac": traffic / (avg_ns * 1e-9) / (HBM_PEAK_GBS * 1e9) if avg_ns else None,
+    }
+    lines = [
+        "# Raymarch kernel counters: 136,320-strand step (256^3, 1920x1080, 512 steps)", "",
+        "Commands (one MI355X, repository root, `cd /tmp && export TMPDIR=/tmp` first; each `--pmc` pass a run of its own):",
+        "", "```",
+        "rocprofv3 --kernel-trace --stats -d <root>/stats -o run --output-format csv -- python3 bench.py --steps 10 --warmup 2 --no-cpu-baseline --no-extras",
+        "rocprofv3 --pmc FETCH_SIZE -d <root>/fetch -o run --output-format csv -- python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-extras",
+        "rocprofv3 --pmc WRITE_SIZE -d <root>/write -o run --output-format csv -- python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-extras",
+        "rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_VMEM_RD SQ_INSTS_LDS SQ_BUSY_CYCLES -d <root>/sq -o run --output-format csv -- python3 bench.py --steps 2 --warmup 1 --no-cpu-baseline --no-extras",
+        "rocprofv3 --pmc FETCH_SIZE -d <root>/copy_fetch -o run --output-format csv -- python3 bench.py --copy-calibration",
+        "rocprofv3 --pmc WRITE_SIZE -d <root>/copy_write -o run --output-format csv -- python3 bench.py --copy-calibration",
+        "python3 bench.py --summarize-profile <root> --out profiles/raymarch_counters_136k_strands_1080p.md",
+        "```", "",
+        f"Copy calibration: the 256 MiB copy kernel reads {cf[copy_name]['FETCH_SIZE']:.0f} KiB (FETCH_SIZE) and writes "
+        f"{cw[copy_name]['WRITE_SIZE']:.0f} KiB (WRITE_SIZE) per dispatch for 262144 KiB each way: scale factors "
+        f"{f_scale:.3f} (fetch) and {w_scale:.3f} (write).", "",
+        f"VALU issue fraction = VALU wave-instructions x {VALU_CYCLES} cycles / ({SIMDS} SIMDs x {CLOCK_HZ / 1e9:.1f} GHz x "
+        "average duration).  `bench.py` reads the block below only while the hash of the raymarch kernel's sources "
+        f"({', '.join(RAYMARCH_SOURCES)}) equals its `source_hash`.", "",
+        "| kernel | avg us | VALU | SALU | VMEM rd | LDS | waves | FETCH KiB | WRITE KiB | HBM MB (scaled) |",
+        "|---|---|---|---|---|---|---|---|---|---|"]
+    for n in names:
+        st, q = stats.get(n, {}), sq.get(n, {})
+        fk, wk = fetch.get(n, {}).get("FETCH_SIZE"), write.get(n, {}).get("WRITE_SIZE")
+        hbm = (fk * 1024 * f_scale + wk * 1024 * w_scale) / 1e6 if fk is not None and wk is not None else None
+
+        def g(v, fmt="{:.3g}"):
+            return fmt.format(v) if v is not None else "-"
+        lines.append(f"| `{_short(n)[:90]}` | {g(float(st['AverageNs']) / 1e3 if st else None, '{:.1f}')} | "
+                     f"{g(q.get('SQ_INSTS_VALU'))} | {g(q.get('SQ_INSTS_SALU'))} | {g(q.get('SQ_INSTS_VMEM_RD'))} | "
+                     f"{g(q.get('SQ_INSTS_LDS'))} | {g(q.get('SQ_WAVES'), '{:.0f}')} | {g(fk, '{:.0f}')} | "
+                     f"{g(wk, '{:.0f}')} | {g(hbm, '{:.1f}')} |")
+    lines += ["", "Counter block of the raymarch kernel (per dispatch):", "", "```json", json.dumps(block, indent=1),
+              "```", ""]
+    out.write_text("\n".join(lines))
+    print(json.dumps(block))
 
 
 def time_batch(dev, reps=3):
@@ -401,28 +563,47 @@ def host_cores() -> int:
 
 
 def cpu_baselines(style, bounds, res, vol_np, scene_args, covered_pixels, budget):
-    """Oracle legs on the host cores, each for about `budget` seconds: the
-    float64 raymarch transcription on random covered pixels with all cores
-    (OpenMP over pixels, the reported value) and with one; the strict-IEEE
-    voxelizer with one thread (the reference's serial DDA loop) and its
-    OpenMP count-only variant."""
+    """Host-core legs, each for about `budget` seconds.
+
+    The reported value is the reference's own load-time volume build,
+    restated in the oracle: HairStyle::voxelize_segments
+    (src/vkhr/scene_graph/hair_style.cc:296-342) -- the serial DDA over the
+    segments with the saturating uint8 densities and the first-255 tangent
+    sums, then the tangent quantization over an OpenMP team (dynamic
+    schedule, as the reference) -- followed by Volume::normalize (:344-357),
+    in segments per second.  Beside it, not the reference's: the OpenMP
+    count-only DDA and the float64 raymarch transcription (this build's
+    checker; the reference has no CPU raymarcher)."""
     import numpy as np
     from oracle import raymarch as OR
     from oracle import voxel as OV
     cores = host_cores()
     idx = style.indices()
-    OV.set_threads(1)
-    t0, reps = time.perf_counter(), 0
+    tan = OV.generate_tangents(style.vertices, style.strand_offsets)
+    n_seg = len(idx) // 2
+    t_dda = t_quant = t_norm = 0.0
+    reps = 0
+    t0 = time.perf_counter()
     while time.perf_counter() - t0 < budget or reps == 0:
-        OV.voxelize(style.vertices, bounds, res, indices=idx)
+        OV.set_threads(1)
+        a = time.perf_counter()
+        v = OV.voxelize(style.vertices, bounds, res, indices=idx, tangents=tan, want_sums=True, want_sums_all=False)
+        b = time.perf_counter()
+        OV.set_threads(cores)
+        OV.quantize_tangents(v["sums_ref"], v["dens_u8"])
+        c = time.perf_counter()
+        OV.set_threads(1)
+        OV.normalize(v["dens_u8"])
+        d = time.perf_counter()
+        t_dda, t_quant, t_norm = t_dda + (b - a), t_quant + (c - b), t_norm + (d - c)
         reps += 1
-    seg_rate = reps * (len(idx) // 2) / (time.perf_counter() - t0)
+    build_rate = reps * n_seg / (t_dda + t_quant + t_norm)
     OV.set_threads(cores)
     t0, reps_omp = time.perf_counter(), 0
     while time.perf_counter() - t0 < budget / 2 or reps_omp == 0:
         OV.voxelize_counts_omp(style.vertices, bounds, res, idx)
         reps_omp += 1
-    seg_rate_omp = reps_omp * (len(idx) // 2) / (time.perf_counter() - t0)
+    seg_rate_omp = reps_omp * n_seg / (time.perf_counter() - t0)
     rng = np.random.default_rng(0)
     order = rng.permutation(covered_pixels)
     cam, light, sp, rp, w, h = scene_args
@@ -443,19 +624,30 @@ def cpu_baselines(style, bounds, res, vol_np, scene_args, covered_pixels, budget
     done_all, ray_rate = rates[cores]
     done_one, ray_rate_one = rates[1]
     return dict(
-        value=ray_rate / 1e6, unit="Mrays/s", cores=cores, kind="port",
-        sample=(f"float64 raymarch transcription (OpenMP over pixels, {cores} threads) on {done_all} random "
-                f"covered pixels of the 1920x1080 frame (512 steps, 256^3); 1 thread: {ray_rate_one / 1e6:.4g} "
-                f"Mrays/s on {done_one} pixels; strict-IEEE voxelize_segments restatement: {reps} passes over "
-                f"{len(idx) // 2} segments, 1 thread (the reference's serial loop); host {cpu_model()}, "
-                f"{cores} cores available; Embree {'present (not timed)' if embree_present() else 'unavailable'}"),
-        raymarch_mrays_per_s_1_thread=ray_rate_one / 1e6,
-        segments_per_s=seg_rate,
-        segments_per_s_openmp_count_only=seg_rate_omp)
+        value=build_rate, unit="segments/s", cores=cores, kind="port",
+        sample=(f"HairStyle::voxelize_segments + Volume::normalize (src/vkhr/scene_graph/hair_style.cc:296-357) "
+                f"restated in the oracle, {reps} passes over the {n_seg} segments into 256^3: the DDA with the "
+                f"first-255 tangent sums on 1 thread (the reference's serial loop, {t_dda / reps * 1e3:.0f} ms per "
+                f"pass), the tangent quantization on {cores} threads (OpenMP, dynamic schedule, as the reference; "
+                f"{t_quant / reps * 1e3:.0f} ms), normalize on 1 thread ({t_norm / reps * 1e3:.0f} ms); host "
+                f"{cpu_model()}, {cores} cores available"),
+        reference_function="HairStyle::voxelize_segments + HairStyle::Volume::normalize "
+                           "(src/vkhr/scene_graph/hair_style.cc:296-357)",
+        dda_segments_per_s_1_thread=reps * n_seg / t_dda,
+        segments_per_s_openmp_count_only=seg_rate_omp,
+        raymarch_port={"mrays_per_s": ray_rate / 1e6, "threads": cores, "pixels": done_all,
+                       "mrays_per_s_1_thread": ray_rate_one / 1e6, "pixels_1_thread": done_one,
+                       "note": "this build's float64 transcription of volume.frag (its checker), on random covered "
+                               "pixels of the 1920x1080 frame (512 steps, 256^3); the reference has no CPU "
+                               "raymarcher.  Embree " + ("present (not timed)" if embree_present() else
+                                                         "unavailable")})
 
 
 def main():
     args = parse()
+    if args.summarize_profile:
+        summarize_profile(Path(args.summarize_profile), Path(args.out))
+        return
     if args.copy_calibration:
         copy_calibration()
         return
@@ -584,7 +776,6 @@ def main():
     # every covered ray marches S steps; fragments also march S ADSM steps,
     # 8 AO taps and one tangent tap
     ray_bytes = 8 * S * covered + 8 * (S + 8) * frags + 32 * frags + 16 * covered
-    achieved = ray_bytes / t_ray / 1e9
     samples = int((sums.view(-1, 2)[:, 1] & 0xFFFFFFFF).sum().item())  # the packed counts
     # every voxelizer sample is one line request (two 8-byte adds, paired lanes)
     peak = atomic_peak(dev)
@@ -635,9 +826,7 @@ def main():
         "load": {"sort_segments_ms": t_sort * 1e3,
                  "note": "once per style at load, outside the step: the index pairs in the Morton order of the "
                          "segment midpoints, which every pass here draws"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "raymarch_kernel", "algorithmic_bytes": ray_bytes},
+        "roofline": raymarch_roofline(t_ray, ray_bytes),
     }
     if not args.no_extras:
         t_sh, t_ra, t_re, n_frag = time_raster(dev, style, xyz, idx, tan, vol, res, scene, cam, light, bounds)

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define VKHR_ABI_VERSION 2
+#define VKHR_ABI_VERSION 3
 
 enum vkhr_status {
     VKHR_OK = 0,
@@ -239,20 +239,30 @@ int vkhr_voxelize_segments(const float* xyz, uint32_t n_vertices,
  *                  (nx+1)*(ny+1)*(nz+1) entries, index (i+1) + (j+1)*(nx+1)
  *                  + (k+1)*(nx+1)*(ny+1).  The raymarcher reads this layout:
  *                  one 8-byte gather per trilinear tap;
- *   occupancy    = one bit per brick of 8x8x8 cells, set when a cell of
- *                  the brick holds a nonzero texel (nullable; needs cells;
- *                  vkhr_occupancy_words(nx, ny, nz) uint32 words, brick
- *                  (bx,by,bz) = cell (i+1,j+1,k+1) >> 3 at bit bx + BX*(by +
- *                  BY*bz) with BX = (nx+8)/8, BY = (ny+8)/8).  The raymarcher
- *                  skips the gathers of samples in empty bricks (their taps
- *                  read 0: the result is unchanged). */
+ *   occupancy    = the brick occupancy of the cells (nullable; needs cells;
+ *                  vkhr_occupancy_words(nx, ny, nz) uint32 words).  Bricks
+ *                  are 8x8x8 cells, brick (bx,by,bz) = cell (i+1,j+1,k+1) >> 3
+ *                  with index b = bx + BX*(by + BY*bz), BX = (nx+8)/8, BY =
+ *                  (ny+8)/8, BZ = (nz+8)/8, B = BX*BY*BZ bricks.  Words
+ *                  [0, ceil(B/32)): bit b set when a cell of brick b holds a
+ *                  nonzero texel; then ceil(B/8) words of 4-bit distances:
+ *                  brick b at bits 4*(b%8) of word b/8, 0 for an occupied
+ *                  brick, else the chessboard distance in bricks to the
+ *                  nearest occupied brick, capped at 15 (every brick within
+ *                  distance - 1 is empty; all 0 when a side exceeds 2040, and
+ *                  then nothing is skipped); then ceil(B/4) words the finalize
+ *                  works in.  The raymarcher skips the gathers of samples in
+ *                  empty bricks and, on the power-of-two sample grid, jumps
+ *                  over empty brick regions (their taps read 0: the result is
+ *                  unchanged). */
 int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tangent_acc,
                          uint32_t nx, uint32_t ny, uint32_t nz, int normalize,
                          uint8_t* density_u8, int8_t* tangent_i8x4,
                          float* density_f32, uint64_t* cells, uint32_t* occupancy,
                          vkhr_stream stream);
 
-/* Words of the brick occupancy bitmask of an nx x ny x nz volume. */
+/* Words of the brick occupancy buffer of an nx x ny x nz volume (bitmask,
+ * distance nibbles and working space, see vkhr_volume_finalize). */
 size_t vkhr_occupancy_words(uint32_t nx, uint32_t ny, uint32_t nz);
 
 /* HairStyle::Volume::downsample (include/vkhr/scene_graph/hair_style.hh:
@@ -284,7 +294,7 @@ typedef struct vkhr_volume {
     const float* density_f32;    /* nullable: float densities, used instead of cells when set */
     uint32_t nx, ny, nz;
     uint32_t reserved_;
-    const uint32_t* occupancy;   /* nullable: brick occupancy of the cells (vkhr_volume_finalize) */
+    const uint32_t* occupancy;   /* nullable: brick occupancy and distances of the cells (vkhr_volume_finalize) */
 } vkhr_volume;
 
 /* Per-pixel volume raymarch of one hair style: the raymarcher pass

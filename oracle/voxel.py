@@ -52,17 +52,18 @@ def generate_tangents(xyz: np.ndarray, offsets: np.ndarray) -> np.ndarray:
     return out
 
 
-def voxelize(xyz, bounds: dict, res, indices=None, tangents=None, want_sums=False):
+def voxelize(xyz, bounds: dict, res, indices=None, tangents=None, want_sums=False, want_sums_all=True):
     """Returns dict(counts=u32, dens_u8=u8 (reference saturating view),
     sums_ref (reference's first-255 tangent sums), sums_all) for vertex mode
-    (indices None) or segment mode."""
+    (indices None) or segment mode.  want_sums_all=False leaves out the sums
+    over every contribution (the reference keeps only sums_ref)."""
     xyz = np.ascontiguousarray(xyz, np.float32)
     nx, ny, nz = res
     n = nx * ny * nz
     counts = np.zeros(n, np.uint32)
     dens = np.zeros(n, np.uint8)
     sums_ref = np.zeros(3 * n, np.float32) if want_sums else None
-    sums_all = np.zeros(3 * n, np.float32) if want_sums else None
+    sums_all = np.zeros(3 * n, np.float32) if want_sums and want_sums_all else None
     tan = None if tangents is None else np.ascontiguousarray(tangents, np.float32)
     b = aabb_array(bounds)
     if indices is None:

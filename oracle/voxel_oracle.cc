@@ -228,9 +228,13 @@ void oracle_voxelize_segments_counts_omp(const float* xyz, const uint32_t* indic
     }
 }
 
-// i8vec3(sum / float(denominator) * 127), w = 0.
+// i8vec3(sum / float(denominator) * 127), w = 0.  Over the OpenMP team with
+// dynamic scheduling, as the reference's quantization loop
+// (hair_style.cc:333-339); each element is independent, so the result does
+// not depend on the team.
 void oracle_quantize_tangents(const float* sums, const uint32_t* denominators, uint64_t n, int8_t* out) {
-    for (uint64_t i = 0; i < n; ++i) {
+#pragma omp parallel for schedule(dynamic)
+    for (int64_t i = 0; i < int64_t(n); ++i) {
         float den = static_cast<float>(denominators[i]);
         out[4 * i + 0] = to_i8(sums[3 * i + 0] / den * 127.0f);
         out[4 * i + 1] = to_i8(sums[3 * i + 1] / den * 127.0f);

@@ -22,7 +22,7 @@ def test_every_declared_symbol_has_a_binding():
 
 
 def test_abi_version_and_error_string():
-    assert capi.lib.vkhr_abi_version() == 2
+    assert capi.lib.vkhr_abi_version() == 3
     assert capi.lib.vkhr_march_record_bytes(16, 4) == 64 * 64
     assert isinstance(capi.lib.vkhr_last_error(), bytes)
 

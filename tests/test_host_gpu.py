@@ -147,3 +147,71 @@ def test_host_surface_end_to_end(device, tmp_path):
     h2 = np.frombuffer((tmp_path / "hybrid2.bin").read_bytes(), np.float32).reshape(h, w, 4)
     differs = np.any(h2 != res, axis=2)[strand_fits]
     assert differs.sum() <= max(2, 1e-3 * strand_fits.sum()), differs.sum()
+
+
+def test_hair_file_through_add_style_on_the_device(device, tmp_path):
+    """A .hair asset through the reference's load path: a synthetic style of
+    uneven strands (4-24 vertices) saved with HairStyle::save, loaded with
+    HairStyle(path), prepared in SceneGraph::add_style's order (shuffle with
+    a fixed seed, then tangents, thickness, indices, bounding box, as the
+    file holds none of them) and voxelized (segments, 96^3) and raymarched
+    on the device.  The oracle reads the same file, shuffles it with the
+    restated draw loop and voxelizes and raymarches it: the shuffled style
+    and the counts bit-exact, the normalized densities exact, the raymarch
+    within 1e-4."""
+    from oracle import hair_format as HF
+    s = synth.hemisphere_style(2500, (4, 24), seed=33)
+    n, w, h, sm = 96, 128, 128, 256
+    blob = np.array([s.n_strands, s.n_vertices, n, w, h, sm], np.uint32).tobytes()
+    blob += s.segments.astype(np.uint16).tobytes() + s.vertices.astype(np.float32).tobytes()
+    (tmp_path / "style.bin").write_bytes(blob)
+    hair = tmp_path / "style.hair"
+    r = subprocess.run([str(EXE), "--save", str(tmp_path / "style.bin"), str(hair)], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stderr
+    data = hair.read_bytes()
+    assert len(data) == 128 + 2 * s.n_strands + 12 * s.n_vertices   # header, segments, vertices
+    arrays = HF.read(data)
+    assert np.array_equal(arrays["vertices"].reshape(-1, 3), s.vertices.reshape(-1, 3))
+
+    # the oracle's add_style: shuffle, then the generators
+    seed = 0xD1B54A32D192ED03
+    shuffled, _ = HF.reduce(arrays, s.n_strands, 0, 1.0, seed)
+    segs = shuffled["segments"].astype(np.int64)
+    verts = shuffled["vertices"].reshape(-1, 3).astype(np.float32)
+    offs = np.concatenate([[0], np.cumsum(segs + 1)]).astype(np.uint32)
+    assert not np.array_equal(verts, s.vertices.reshape(-1, 3))          # the order changed
+    tan = OV.generate_tangents(verts, offs)
+    idx = OV.generate_indices(offs)
+    b = OV.aabb(verts)
+
+    sc = synth.make_scene(b, w, h, azimuth_deg=140.0)
+    cam, light, sp = params.camera(sc), params.light(sc), params.strand(b, (n, n, n))
+    rp = params.render(raycast_steps=256.0)
+    light_vp = np.asarray(sc.light_vp, np.float64).T.reshape(-1).astype(np.float32)
+    pblob = np.array([n, w, h, sm], np.uint32).tobytes() + bytes(cam) + bytes(light) + bytes(sp) + bytes(rp)
+    pblob += light_vp.tobytes()
+    (tmp_path / "params.bin").write_bytes(pblob)
+    out = tmp_path / "out"
+    out.mkdir()
+    r = subprocess.run([str(EXE), "--hair", str(hair), hex(seed), str(tmp_path / "params.bin"), str(out)],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+
+    got_v = np.frombuffer((out / "vertices.bin").read_bytes(), np.float32).reshape(-1, 3)
+    assert np.array_equal(got_v, verts)
+    assert np.array_equal(np.frombuffer((out / "style_tangents.bin").read_bytes(), np.float32),
+                          tan.reshape(-1))
+    assert np.array_equal(np.frombuffer((out / "indices.bin").read_bytes(), np.uint32), idx)
+    assert np.array_equal(np.frombuffer((out / "aabb.bin").read_bytes(), np.float32), OV.aabb_array(b))
+    ref = OV.voxelize(verts, b, (n, n, n), indices=idx, tangents=tan, want_sums=True)
+    counts = np.frombuffer((out / "counts.bin").read_bytes(), np.uint32)
+    assert np.array_equal(counts, ref["counts"]) and counts.sum() > 10000
+    dens = np.frombuffer((out / "density.bin").read_bytes(), np.uint8)
+    assert np.array_equal(dens, OV.normalize(ref["dens_u8"]))
+    tangents = np.frombuffer((out / "tangents.bin").read_bytes(), np.int8)
+    img = np.frombuffer((out / "raymarch.bin").read_bytes(), np.float32).reshape(-1, 4)
+    o = OR.raymarch(dens, tangents, (n, n, n), cam, light, sp, rp, w, h)
+    err = np.abs(img.astype(np.float64) - o["rgba"]).max()
+    assert err <= 1e-4 and (o["info"][:, 0] == 1).sum() > 1000, err

@@ -182,18 +182,32 @@ def test_packed_counts_path_matches(device):
 
 
 def _occupancy_from_cells(cells: np.ndarray, res) -> np.ndarray:
-    """One bit per brick of 8x8x8 cells, set when a cell of the brick is
-    nonzero (the layout of vkhr_volume_finalize's occupancy)."""
+    """The bitmask and distance nibbles of vkhr_volume_finalize's occupancy
+    buffer (its working bytes after them are not part of the result): one
+    bit per brick of 8x8x8 cells, set when a cell of the brick is nonzero,
+    then 4 bits per brick, the chessboard distance in bricks to the nearest
+    occupied brick capped at 15 (0 for an occupied one).  The distances come
+    from scipy's chamfer transform, not the kernels' separable passes."""
+    from scipy import ndimage
     nx, ny, nz = res
     c = cells.view(np.uint64).reshape(nz + 1, ny + 1, nx + 1) != 0
     bx, by, bz = (nx + 8) // 8, (ny + 8) // 8, (nz + 8) // 8
     pad = np.zeros((bz * 8, by * 8, bx * 8), bool)
     pad[:nz + 1, :ny + 1, :nx + 1] = c
-    bricks = pad.reshape(bz, 8, by, 8, bx, 8).any(axis=(1, 3, 5)).reshape(-1)
+    occupied = pad.reshape(bz, 8, by, 8, bx, 8).any(axis=(1, 3, 5))
+    bricks = occupied.reshape(-1)
     words = np.zeros((bricks.size + 31) // 32, np.uint32)
     idx = np.nonzero(bricks)[0]
     np.bitwise_or.at(words, idx >> 5, (np.uint32(1) << (idx & 31).astype(np.uint32)))
-    return words
+    if occupied.any():
+        dist = ndimage.distance_transform_cdt(~occupied, metric="chessboard").reshape(-1)
+    else:
+        dist = np.full(bricks.size, 15)
+    dist = np.minimum(dist, 15).astype(np.uint32)
+    nib = np.zeros((bricks.size + 7) // 8, np.uint32)
+    b = np.arange(bricks.size)
+    np.bitwise_or.at(nib, b >> 3, dist << (4 * (b & 7)).astype(np.uint32))
+    return np.concatenate([words, nib])
 
 
 def _cells_from_density(dens: np.ndarray, res) -> np.ndarray:
@@ -222,7 +236,8 @@ def test_cells_exact_at_uneven_sizes(device, res):
     cells = vol["cells"].cpu().numpy().view(np.uint64)
     want = _cells_from_density(vol["density"].cpu().numpy(), res)
     assert np.array_equal(cells, want)
-    assert np.array_equal(vol["occupancy"].cpu().numpy().view(np.uint32), _occupancy_from_cells(want, res))
+    occ = _occupancy_from_cells(want, res)
+    assert np.array_equal(vol["occupancy"].cpu().numpy().view(np.uint32)[:occ.size], occ)
     assert (want != 0).sum() > 100
 
 
@@ -235,8 +250,14 @@ def test_occupancy_bits(device, res):
         got = vol["occupancy"].cpu().numpy().view(np.uint32)
         assert got.size == D.occupancy_words(res)
         want = _occupancy_from_cells(vol["cells"].cpu().numpy(), res)
-        assert np.array_equal(got, want)
-        assert 0 < int(np.unpackbits(want.view(np.uint8)).sum()) < 32 * want.size
+        assert np.array_equal(got[:want.size], want)
+        n_bricks = ((res[0] + 8) // 8) * ((res[1] + 8) // 8) * ((res[2] + 8) // 8)
+        bits = want[:(n_bricks + 31) // 32]
+        assert 0 < int(np.unpackbits(bits.view(np.uint8)).sum()) < n_bricks
+        # empty bricks at several distances (the 17^3-brick volume reaches 4)
+        nib = want[(n_bricks + 31) // 32:]
+        d = (nib[np.arange(n_bricks) >> 3] >> (4 * (np.arange(n_bricks) & 7)).astype(np.uint32)) & 15
+        assert d.max() >= (4 if res[0] == 128 else 2), np.unique(d)
 
 
 def test_finalize_on_concurrent_streams(device):

@@ -67,25 +67,41 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
+// The weights c of the 8 lanes of this lane's group of 8, which its rounds
+// of scatter_wave take their samples from (a ray's weight is the same for
+// all its samples, so this is formed once per ray family).  Every lane of
+// the wave must call it (shuffles).
+struct GroupWeights {
+    float c[8];
+};
+__device__ __forceinline__ GroupWeights group_weights(float c) {
+    GroupWeights g;
+    const int group = __lane_id() & 56;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) g.c[r] = __shfl(c, group | r, kWave);
+    return g;
+}
+
 // Adds c * (trilinear corner weights) of every lane's sample to the cell
 // accumulator; lanes without a sample pass cell == kNoCell.  In round r the
 // 8 lanes of each group of 8 add the 8 corners of the sample of lane r of
-// the group.  Every lane of the wave must call it (shuffles).
+// the group, whose weight c is gw.c[r].  Every lane of the wave must call it
+// (shuffles).
 __device__ __forceinline__ void scatter_wave(float* __restrict__ cg, uint32_t cell, float wx, float wy, float wz,
-                                             float c) {
+                                             const GroupWeights& gw) {
     const unsigned long long live = __ballot(cell != kNoCell);
     if (!live) return;
     const int lane = __lane_id();
     const int k = lane & 7, group = lane & 56;
+#pragma unroll
     for (int r = 0; r < 8; ++r) {
         if (!((live >> r) & 0x0101010101010101ull)) continue;
         const int src = group | r;
         const uint32_t sc = __shfl(cell, src, kWave);
         const float sx = __shfl(wx, src, kWave), sy = __shfl(wy, src, kWave), sz = __shfl(wz, src, kWave);
-        const float s_c = __shfl(c, src, kWave);
         if (sc != kNoCell) {
             const float wt = ((k & 1) ? sx : 1.0f - sx) * ((k & 2) ? sy : 1.0f - sy) * ((k & 4) ? sz : 1.0f - sz);
-            unsafeAtomicAdd(&cg[size_t(sc) * 8u + size_t(k)], s_c * wt);
+            unsafeAtomicAdd(&cg[size_t(sc) * 8u + size_t(k)], gw.c[r] * wt);
         }
     }
 }
@@ -100,6 +116,7 @@ __device__ void scatter_rays(const AdjointArgs& a, bool active, const double x0[
     const float step = f.step_size;
     const int n_lane = active ? n : 0;
     const int n_wave = wave_max(n_lane);
+    const GroupWeights gw = group_weights(c);
     float t = 0.0f;  // the march state carried across chunks
     for (int i0 = 0; i0 < n_wave; i0 += kChunk) {
         const int i1 = min(i0 + kChunk, n_wave);
@@ -110,7 +127,7 @@ __device__ void scatter_rays(const AdjointArgs& a, bool active, const double x0[
             float w[3] = {0.0f, 0.0f, 0.0f};
             uint32_t cell = kNoCell;
             if (i < n_lane && in_range(f, x)) cell = cell_of(f, x, w);
-            scatter_wave(a.cell_grad, cell, w[0], w[1], w[2], c);
+            scatter_wave(a.cell_grad, cell, w[0], w[1], w[2], gw);
         }
         // advance the carried state past this chunk
         for (int i = 0; i < kChunk; ++i) t += step;
@@ -240,12 +257,13 @@ __device__ __forceinline__ void adjoint_tile(const AdjointArgs& a, int x0, int y
         D += fmin(v[n], f.ao_clamp);
     }
     const float cl = lao ? float(G * (-f.ao_exponent) / (8.0 - D)) : 0.0f;
+    const GroupWeights gl = group_weights(cl);
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
         float w[3] = {0.0f, 0.0f, 0.0f};
         uint32_t cell = kNoCell;
         if (lao && v[n] < f.ao_clamp && in_range(f, tap[n])) cell = cell_of(f, tap[n], w);
-        scatter_wave(a.cell_grad, cell, w[0], w[1], w[2], cl);
+        scatter_wave(a.cell_grad, cell, w[0], w[1], w[2], gl);
     }
 }
 

@@ -39,6 +39,9 @@ struct BatchDevice {
     uint32_t* tile_prefix;          // n_styles + 1
     BatchChunk* voxel_chunks;
     BatchChunk* cell_chunks;
+    BatchChunk* line_chunks[3];     // four lines of bricks per chunk, per distance pass
+    uint32_t n_line_chunks[3];
+    uint32_t march_lds_bytes;       // the largest distance field the raymarch stages
     uint32_t* minmax;               // 2 per style
     uint32_t* work_counter;
     uint32_t n_styles;

@@ -26,7 +26,8 @@ __device__ __forceinline__ void batch_item(const BatchDevice& d, const RaymarchA
     // arguments become scalar loads instead of filling vector registers
     const RaymarchArgs& a = args[s];
     if (s != *staged) {  // block-uniform
-        *use_occ = stage_occupancy(a.vol, occ_lds);
+        if (*staged != 0xffffffffu) __syncthreads();   // every wave is done with the previous style's field
+        *use_occ = stage_distance(a.vol, occ_lds);
         *staged = s;
     }
     const uint32_t local = it - d.tile_prefix[s];
@@ -40,7 +41,7 @@ __device__ __forceinline__ void batch_item(const BatchDevice& d, const RaymarchA
 // styles.
 __global__ void __launch_bounds__(256)
 batch_raymarch_grid_kernel(BatchDevice d, const RaymarchArgs* __restrict__ args) {
-    __shared__ uint32_t occ_lds[kOccupancyLdsWords];
+    extern __shared__ uint32_t occ_lds[];   // d.march_lds_bytes
     uint32_t staged = 0xffffffffu;
     bool use_occ = false;
     batch_item(d, args, uint32_t(__builtin_amdgcn_readfirstlane(int(blockIdx.x))), occ_lds, &staged, &use_occ);
@@ -52,8 +53,8 @@ batch_raymarch_grid_kernel(BatchDevice d, const RaymarchArgs* __restrict__ args)
 __global__ void __launch_bounds__(256)
 batch_raymarch_kernel(BatchDevice d, const RaymarchArgs* __restrict__ args) {
     __shared__ uint32_t item;
-    __shared__ uint32_t occ_lds[kOccupancyLdsWords];
-    uint32_t staged = 0xffffffffu;   // the style whose occupancy is in LDS
+    extern __shared__ uint32_t occ_lds[];   // d.march_lds_bytes
+    uint32_t staged = 0xffffffffu;   // the style whose distance field is in LDS
     bool use_occ = false;
     for (;;) {
         if (threadIdx.x == 0) item = atomicAdd(d.work_counter, 1u);
@@ -70,9 +71,9 @@ batch_raymarch_kernel(BatchDevice d, const RaymarchArgs* __restrict__ args) {
 // grid == 0: one block per work item; else that many persistent blocks.
 void launch_batch_march(const BatchDevice& d, unsigned grid, hipStream_t stream) {
     if (grid == 0)
-        batch_raymarch_grid_kernel<<<d.total_tiles, 256, 0, stream>>>(d, d.args);
+        batch_raymarch_grid_kernel<<<d.total_tiles, 256, d.march_lds_bytes, stream>>>(d, d.args);
     else
-        batch_raymarch_kernel<<<grid, 256, 0, stream>>>(d, d.args);
+        batch_raymarch_kernel<<<grid, 256, d.march_lds_bytes, stream>>>(d, d.args);
 }
 
 }  // namespace vkhr
@@ -123,7 +124,8 @@ int build_group(vkhr_batch* b, const vkhr_batch_style* styles, uint32_t first, u
     std::vector<RaymarchArgs> args(n);
     std::vector<uint64_t> pair_prefix(n + 1, 0);
     std::vector<uint32_t> tile_prefix(n + 1, 0);
-    std::vector<BatchChunk> vchunks, cchunks;
+    std::vector<BatchChunk> vchunks, cchunks, lchunks[3];
+    size_t lds_bytes = 0;
     const uint64_t kChunk = 1u << 16;   // voxels per finalize chunk
     const uint32_t bw = 2 * t.tile_w, bh = 2 * t.tile_h;
     const uint32_t tiles_x = (width + bw - 1) / bw, tiles_y = (height + bh - 1) / bh;
@@ -167,6 +169,12 @@ int build_group(vkhr_batch* b, const vkhr_batch_style* styles, uint32_t first, u
         for (uint64_t a = 0; a < nvox; a += kChunk) vchunks.push_back({s, 0, a, std::min(nvox, a + kChunk)});
         const uint64_t cb = cell_blocks(in.volume, in.volume, in.volume);
         for (uint64_t k = 0; k < cb; ++k) cchunks.push_back({s, 0, k, k + 1});
+        if (in.occupancy && distance_field_fits(in.volume, in.volume, in.volume))
+            for (int axis = 0; axis < 3; ++axis) {
+                const uint32_t lines = distance_lines(in.volume, in.volume, in.volume, axis);
+                for (uint32_t l = 0; l < lines; l += 256 / kWave)
+                    lchunks[axis].push_back({s, 0, l, std::min<uint64_t>(lines, l + 256 / kWave)});
+            }
         vkhr_volume vol;
         std::memset(&vol, 0, sizeof(vol));
         vol.density = in.density;
@@ -177,6 +185,7 @@ int build_group(vkhr_batch* b, const vkhr_batch_style* styles, uint32_t first, u
         if ((rc = make_raymarch_args(&vol, &in.camera, &in.light, &in.strand, &in.params, width, height, &args[s])))
             return rc;
         args[s].rgba = reinterpret_cast<float4*>(in.rgba);
+        lds_bytes = std::max(lds_bytes, distance_lds_bytes(args[s].vol));
         tile_prefix[s + 1] = tile_prefix[s] + tiles_x * tiles_y;
     }
     BatchDevice& d = *out;
@@ -186,6 +195,8 @@ int build_group(vkhr_batch* b, const vkhr_batch_style* styles, uint32_t first, u
     if ((rc = upload(b, vs, &d.styles)) || (rc = upload(b, args, &d.args)) ||
         (rc = upload(b, pair_prefix, &d.pair_prefix)) || (rc = upload(b, tile_prefix, &d.tile_prefix)) ||
         (rc = upload(b, vchunks, &d.voxel_chunks)) || (rc = upload(b, cchunks, &d.cell_chunks)) ||
+        (rc = upload(b, lchunks[0], &d.line_chunks[0])) || (rc = upload(b, lchunks[1], &d.line_chunks[1])) ||
+        (rc = upload(b, lchunks[2], &d.line_chunks[2])) ||
         (rc = upload(b, zeros, &mm)))
         return rc;
     d.minmax = mm;
@@ -193,6 +204,8 @@ int build_group(vkhr_batch* b, const vkhr_batch_style* styles, uint32_t first, u
     d.n_styles = n;
     d.n_voxel_chunks = uint32_t(vchunks.size());
     d.n_cell_chunks = uint32_t(cchunks.size());
+    for (int axis = 0; axis < 3; ++axis) d.n_line_chunks[axis] = uint32_t(lchunks[axis].size());
+    d.march_lds_bytes = uint32_t(lds_bytes);
     d.total_tiles = tile_prefix[n];
     d.total_pairs = pair_prefix[n];
     d.tile_w = t.tile_w;

@@ -119,6 +119,21 @@ batch_cells_kernel(BatchDevice d) {
     }
 }
 
+// One pass of every style's brick distance field: a line chunk is four
+// lines (one per wave) of one style.
+__global__ void __launch_bounds__(256)
+batch_distance_kernel(BatchDevice d, int axis) {
+    __shared__ uint32_t lds[256 / kWave][kDistanceLineMax];
+    const uint32_t wave = threadIdx.x / kWave;
+    for (uint32_t c = blockIdx.x; c < d.n_line_chunks[axis]; c += gridDim.x) {
+        const BatchChunk ch = d.line_chunks[axis][c];
+        const BatchVolStyle& st = d.styles[ch.style];
+        const uint32_t line = uint32_t(ch.begin) + wave;
+        distance_line(st.occupancy, st.nx, st.ny, st.nz, axis, line, line < uint32_t(ch.end), lds[wave]);
+        __syncthreads();   // the LDS rows are reused by the next chunk
+    }
+}
+
 __global__ void __launch_bounds__(256)
 batch_normalize_kernel(BatchDevice d) {
     for (uint32_t c = blockIdx.x; c < d.n_voxel_chunks; c += gridDim.x) {
@@ -148,6 +163,8 @@ void launch_batch_volume(const BatchDevice& d, hipStream_t stream) {
     batch_finalize_kernel<<<4096, 256, 0, stream>>>(d);
     batch_normalize_kernel<<<4096, 256, 0, stream>>>(d);
     batch_cells_kernel<<<8192, 256, 0, stream>>>(d);
+    for (int axis = 0; axis < 3; ++axis)
+        if (d.n_line_chunks[axis]) batch_distance_kernel<<<grid_for(d.n_line_chunks[axis], 1, 4096), 256, 0, stream>>>(d, axis);
 }
 
 }  // namespace vkhr

@@ -161,28 +161,68 @@ __device__ __forceinline__ void support_interval(const RaymarchArgs& a, const do
     *hi = h;
 }
 
-// Parameter t where the ray x(t) = s + t / inv leaves brick b of cell c1
-// (cells 8b .. 8b+7 cover x in [8b-1, 8b+7)) along one axis, with the plane
-// taken 1e-6 texels inside the brick (float64 rounding of a sample's
-// position cannot cross it); +inf when the ray is parallel to the planes.
-__device__ __forceinline__ double brick_exit(uint32_t c1, double s, double inv) {
-    if (inv == 0.0) return INFINITY;
-    const double b = double(c1 >> kBrickShift);
-    const double plane = inv > 0.0 ? 8.0 * b + 7.0 - 1e-6 : 8.0 * b - 1.0 + 1e-6;
-    return (plane - s) * inv;
+// A march over the power-of-two sample grid t_i = i / steps (exact in float
+// and double): sample i lies at texel coordinate + 1  x1s + i * dxs, the
+// same point as x = xs + t_i * dx (dxs = dx / steps exactly).  ea / eb give
+// the exits of empty brick regions (region_exit); pos bit c is set when the
+// march moves up along axis c.
+struct GridRay {
+    double x1s[3], dxs[3];
+    double ea[3], eb[3];
+    uint32_t pos;
+};
+
+__device__ __forceinline__ GridRay grid_ray(const double xs[3], const double dx[3], double step, bool skip) {
+    GridRay g;
+    g.pos = 0u;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        g.x1s[c] = xs[c] + 1.0;
+        g.dxs[c] = dx[c] * step;
+        g.ea[c] = 0.0;
+        g.eb[c] = INFINITY;   // parallel to the planes: never leaves a region along c
+        if (skip && g.dxs[c] != 0.0) {
+            const double inv = 1.0 / g.dxs[c];
+            g.ea[c] = 8.0 * inv;
+            // the plane 1e-6 texels inside the region (float64 rounding of a
+            // sample's position cannot cross it)
+            g.eb[c] = (g.dxs[c] > 0.0 ? -1e-6 - g.x1s[c] : 1e-6 - g.x1s[c]) * inv;
+            if (g.dxs[c] > 0.0) g.pos |= 1u << c;
+        }
+    }
+    return g;
 }
 
-// The index to continue at after sample i read an empty brick (cell c):
-// the last index whose t does not pass the brick's exit (nor t_cap); every
-// sample in between lies at least one step inside the brick and reads 0.
-// steps is raycast_steps (a power of two: t_j = j / steps exactly); s is
-// the ray's start and inv the reciprocals of its direction (texel units).
-__device__ __forceinline__ int skip_empty_brick(int i, uint3 c, double s0, double s1, double s2, double inv0,
-                                                double inv1, double inv2, double t_cap, double steps) {
-    double te = fmin(fmin(brick_exit(c.x, s0, inv0), brick_exit(c.y, s1, inv1)), brick_exit(c.z, s2, inv2));
-    te = fmin(te, t_cap);
-    const double j = floor(te * steps);
-    return j > double(i + 1) ? int(j) : i + 1;
+// Sample index (real-valued) where the march leaves the region of bricks
+// within chessboard radius r of brick bk: along axis c it spans x1 in
+// [8 (b - r), 8 (b + r + 1)), left at the far plane in the direction of
+// travel, K = b + r + 1 moving up, K = b - r moving down, at index
+// (8 K -+ 1e-6 - x1s) / dxs = K * ea + eb.  Every sample whose index does
+// not pass it lies inside the region.
+__device__ __forceinline__ double region_exit(const GridRay& g, uint3 bk, int r) {
+    const int b[3] = {int(bk.x), int(bk.y), int(bk.z)};
+    double te = INFINITY;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int k = ((g.pos >> c) & 1u) ? b[c] + r + 1 : b[c] - r;
+        te = fmin(te, fma(double(k), g.ea[c], g.eb[c]));
+    }
+    return te;
+}
+
+// First grid index i >= 0 with i / steps >= t (steps when there is none
+// below steps).
+__device__ __forceinline__ int grid_first(double t, double steps) {
+    if (!(t > 0.0)) return 0;
+    const double q = t * steps;
+    return q >= steps ? int(steps) : int(ceil(q));
+}
+
+// Number of grid indices i < steps with i / steps <= t.
+__device__ __forceinline__ int grid_count(double t, double steps) {
+    const double q = t * steps;
+    if (!(q >= 0.0)) return 0;
+    return q >= steps ? int(steps) : int(floor(q)) + 1;
 }
 
 // One pixel of the raymarch pass (volume.frag:31-102).  Every lane of a
@@ -236,42 +276,71 @@ __device__ __forceinline__ void raymarch_pixel(const RaymarchArgs& a, int px, in
         const DensityVolume vol = a.vol;
         const double iso = a.iso;
         const float step = a.step_size;
-        // Empty bricks are stepped over when the loop parameter is exactly
-        // i * step: their samples read 0 and change neither the sum nor the
-        // surface choices; the jump stops short of t_hi and of t_depth (where
-        // the depth test may end the march), so the march ends at the same
-        // sample as without it.
-        const bool jump = occ.use && a.steps_per_unit > 0.0f;
-        const double steps = a.steps_per_unit;
-        const double inv0 = jump && dx[0] != 0.0 ? 1.0 / dx[0] : 0.0;
-        const double inv1 = jump && dx[1] != 0.0 ? 1.0 / dx[1] : 0.0;
-        const double inv2 = jump && dx[2] != 0.0 ? 1.0 / dx[2] : 0.0;
-        const double t_cap = fmin(t_hi, t_depth);
         int i = 0;
-        for (float t = 0.0f; t < 1.0f;) {
-            const double td = double(t);
-            // depth test: db < z/w  <=>  db*w < z  (w > 0 in front of the eye);
-            // evaluated as a predicate (no branch around the products)
-            const bool behind = (td >= t_depth) & (db * fma(td, dw, ws) < fma(td, dz, zs));
-            if (behind | (td > t_hi)) break;
-            bool empty = false;
-            uint3 cell;
-            const float d = march_tap<kF32>(vol, fma(td, dx[0], xs[0]), fma(td, dx[1], xs[1]), fma(td, dx[2], xs[2]),
-                                            occ, &empty, &cell);
-            int next = i + 1;
-            if (jump && empty) next = skip_empty_brick(i, cell, xs[0], xs[1], xs[2], inv0, inv1, inv2, t_cap, steps);
-            acc += double(d);
-            // surface / entry bookkeeping as selects
-            const bool nonzero = d != 0.0f;
-            const bool below = nonzero & (acc <= iso);
-            surf = below ? i : surf;
-            t_surf = below ? t : t_surf;
-            found = found | (nonzero & (acc >= iso));
-            const bool first = nonzero & (entry < 0);
-            entry = first ? i : entry;
-            t_entry = first ? t : t_entry;
-            i = next;
-            t = jump ? float(i) * step : t + step;
+        if (a.steps_per_unit > 0.0f) {
+            // The loop parameter is exactly i * step (a power-of-two step
+            // count): the march runs over integer sample indices, bounded by
+            // the support interval, with the depth test from the first index
+            // at or past t_depth.  A sample in an empty brick region (every
+            // sample in it reads 0 and changes neither the sum nor the
+            // surface choices) jumps to the first sample past the region, but
+            // not past the first index of the depth test, so the march ends
+            // at the same sample as without the jumps.
+            const double steps = a.steps_per_unit, st = double(step);
+            const GridRay g = grid_ray(xs, dx, st, occ.use);
+            const double dzs = dz * st, dws = dw * st;
+            const int i_end = grid_count(t_hi, steps);
+            const int i_depth = grid_first(t_depth, steps);
+            i = min(grid_first(t_lo, steps), i_depth);
+            while (i < i_end) {
+                const double di = double(i);
+                if (i >= i_depth) {
+                    // depth test: db < z/w  <=>  db*w < z  (w > 0 in front of the eye)
+                    if (db * fma(di, dws, ws) < fma(di, dzs, zs)) break;
+                }
+                int region;
+                uint3 bk;
+                const float d = grid_tap<kF32>(vol, fma(di, g.dxs[0], g.x1s[0]), fma(di, g.dxs[1], g.x1s[1]),
+                                               fma(di, g.dxs[2], g.x1s[2]), occ, &region, &bk);
+                int next = i + 1;
+                if (region >= 0 && next < i_depth) {
+                    const int j = int(floor(fmin(region_exit(g, bk, region), double(i_end))));
+                    next = max(next, min(j + 1, i_depth));
+                }
+                acc += double(d);
+                // surface / entry bookkeeping as selects
+                const bool nonzero = d != 0.0f;
+                const bool below = nonzero & (acc <= iso);
+                surf = below ? i : surf;
+                found = found | (nonzero & (acc >= iso));
+                entry = (nonzero & (entry < 0)) ? i : entry;
+                i = next;
+            }
+            i = min(i, i_end);
+            t_surf = float(surf) * step;
+            t_entry = float(entry) * step;
+        } else {
+            // The shader's float loop counter t += step (raycast_steps not a
+            // power of two), sample by sample.
+            for (float t = 0.0f; t < 1.0f; t += step) {
+                const double td = double(t);
+                // depth test: db < z/w  <=>  db*w < z  (w > 0 in front of the eye);
+                // evaluated as a predicate (no branch around the products)
+                const bool behind = (td >= t_depth) & (db * fma(td, dw, ws) < fma(td, dz, zs));
+                if (behind | (td > t_hi)) break;
+                const float d = march_tap<kF32>(vol, fma(td, dx[0], xs[0]), fma(td, dx[1], xs[1]),
+                                                fma(td, dx[2], xs[2]), occ);
+                acc += double(d);
+                const bool nonzero = d != 0.0f;
+                const bool below = nonzero & (acc <= iso);
+                surf = below ? i : surf;
+                t_surf = below ? t : t_surf;
+                found = found | (nonzero & (acc >= iso));
+                const bool first = nonzero & (entry < 0);
+                entry = first ? i : entry;
+                t_entry = first ? t : t_entry;
+                ++i;
+            }
         }
         info = make_int4(2, surf, entry, found ? 1 : 0);  // marched; 1 once a fragment is written
         if (a.record) {
@@ -314,26 +383,31 @@ __device__ __forceinline__ void raymarch_pixel(const RaymarchArgs& a, int px, in
                 support_interval(a, sx, xl, &t_lo, &t_hi);
                 const DensityVolume vol = a.vol;
                 const float step = a.step_size;
-                const bool jump = occ.use && a.steps_per_unit > 0.0f;
-                const double steps = a.steps_per_unit;
-                const double inv0 = jump && xl[0] != 0.0 ? 1.0 / xl[0] : 0.0;
-                const double inv1 = jump && xl[1] != 0.0 ? 1.0 / xl[1] : 0.0;
-                const double inv2 = jump && xl[2] != 0.0 ? 1.0 / xl[2] : 0.0;
-                int j = 0;
-                for (float t = 0.0f; t < 1.0f;) {
-                    const double td = double(t);
-                    if (td > t_hi) break;
-                    int next = j + 1;
-                    if (td >= t_lo) {
-                        bool empty = false;
-                        uint3 cell;
-                        strands += double(march_tap<kF32>(vol, fma(td, xl[0], sx[0]), fma(td, xl[1], sx[1]),
-                                                          fma(td, xl[2], sx[2]), occ, &empty, &cell));
-                        if (jump && empty)
-                            next = skip_empty_brick(j, cell, sx[0], sx[1], sx[2], inv0, inv1, inv2, t_hi, steps);
+                if (a.steps_per_unit > 0.0f) {
+                    // over the sample grid from the support's start, jumping
+                    // over empty brick regions (their samples read 0)
+                    const double steps = a.steps_per_unit;
+                    const GridRay g = grid_ray(sx, xl, double(step), occ.use);
+                    const int j_end = grid_count(t_hi, steps);
+                    for (int j = grid_first(t_lo, steps); j < j_end;) {
+                        const double dj = double(j);
+                        int region;
+                        uint3 bk;
+                        strands += double(grid_tap<kF32>(vol, fma(dj, g.dxs[0], g.x1s[0]), fma(dj, g.dxs[1], g.x1s[1]),
+                                                         fma(dj, g.dxs[2], g.x1s[2]), occ, &region, &bk));
+                        int next = j + 1;
+                        if (region >= 0)
+                            next = max(next, int(floor(fmin(region_exit(g, bk, region), double(j_end)))) + 1);
+                        j = next;
                     }
-                    j = next;
-                    t = jump ? float(j) * step : t + step;
+                } else {
+                    for (float t = 0.0f; t < 1.0f; t += step) {
+                        const double td = double(t);
+                        if (td > t_hi) break;
+                        if (td >= t_lo)
+                            strands += double(march_tap<kF32>(vol, fma(td, xl[0], sx[0]), fma(td, xl[1], sx[1]),
+                                                              fma(td, xl[2], sx[2]), occ));
+                    }
                 }
                 adsm = pow(1.0 - a.alpha, strands * a.adsm_k);
                 occlusion *= adsm;

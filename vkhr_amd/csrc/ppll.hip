@@ -75,15 +75,17 @@ ppll_resolve_kernel(const uint32_t* __restrict__ heads, const uint32_t* __restri
             head = kPpllNull;
         }
     }
-    // tail: swap closer nodes into the k-buffer, blend the evicted ones
+    // tail: swap closer nodes into the k-buffer, blend the evicted ones.  The
+    // furthest slot (resolve.comp:48-57) changes only when a node is swapped
+    // in, so it is rescanned only then.
+    float far_depth;
+    int far_idx = furthest(depth, far_depth);
     for (int f = 0; f < kMaxFragments; ++f) {
         if (head == kPpllNull || head >= node_count) break;
-        float far_depth;
-        int far_idx = furthest(depth, far_depth);
         const uint32_t* n = nodes + 3ull * head;
         uint32_t fc = n[0];
         float fd = __uint_as_float(n[1]);
-        uint32_t next = n[2];
+        const uint32_t next = n[2];
         if (fd < far_depth) {
 #pragma unroll
             for (int k = 0; k < kK; ++k) {
@@ -96,6 +98,7 @@ ppll_resolve_kernel(const uint32_t* __restrict__ heads, const uint32_t* __restri
                     fd = td;
                 }
             }
+            far_idx = furthest(depth, far_depth);
         }
         resolved = blend(resolved, unpack_unorm4x8(fc));
         head = next;

@@ -28,8 +28,8 @@ namespace {
 template <bool kF32>
 __global__ void __launch_bounds__(256)
 raymarch_kernel(RaymarchArgs a) {
-    __shared__ uint32_t occ_lds[kOccupancyLdsWords];
-    const OccupancyView occ = {occ_lds, stage_occupancy(a.vol, occ_lds)};
+    extern __shared__ uint32_t dist_lds[];   // distance_lds_bytes(a.vol)
+    const OccupancyView occ = {dist_lds, stage_distance(a.vol, dist_lds)};
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const int px = blockIdx.x * 16 + (wave & 1) * 8 + (lane & 7);
     const int py = blockIdx.y * 16 + (wave >> 1) * 8 + (lane >> 3);
@@ -82,10 +82,11 @@ double smoothstep(double e0, double e1, double x) {
 
 void launch_raymarch(const RaymarchArgs& a, hipStream_t stream) {
     dim3 grid((a.width + 15) / 16, (a.height + 15) / 16);
+    const size_t lds = distance_lds_bytes(a.vol);
     if (a.vol.f32)
-        raymarch_kernel<true><<<grid, 256, 0, stream>>>(a);
+        raymarch_kernel<true><<<grid, 256, lds, stream>>>(a);
     else
-        raymarch_kernel<false><<<grid, 256, 0, stream>>>(a);
+        raymarch_kernel<false><<<grid, 256, lds, stream>>>(a);
 }
 
 // Shared with the batched and adjoint paths: fills the per-launch arguments.
@@ -153,10 +154,13 @@ int make_raymarch_args(const vkhr_volume* vol, const vkhr_camera* cam, const vkh
     a->vol.cells = reinterpret_cast<const unsigned long long*>(vol->cells);
     a->vol.f32 = vol->density_f32;
     a->vol.nx = a->nx; a->vol.ny = a->ny; a->vol.nz = a->nz;
-    a->vol.occupancy = vol->occupancy;
+    // the distance field of the occupancy buffer, when its passes ran
+    a->vol.distance = vol->occupancy && distance_field_fits(vol->nx, vol->ny, vol->nz)
+                          ? vol->occupancy + occupancy_bit_words(vol->nx, vol->ny, vol->nz)
+                          : nullptr;
     a->vol.bx = bricks_along(vol->nx);
     a->vol.by = bricks_along(vol->ny);
-    a->vol.occ_words = uint32_t(occupancy_words(vol->nx, vol->ny, vol->nz));
+    a->vol.dist_words = uint32_t(distance_words(vol->nx, vol->ny, vol->nz));
     a->tangents = reinterpret_cast<const char4*>(vol->tangents);
     return VKHR_OK;
 }

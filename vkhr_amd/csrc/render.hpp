@@ -11,58 +11,91 @@ constexpr uint32_t kPpllNull = 0xffffffffu;
 
 // The density volume as the kernels read it: corner-packed uint8 cells
 // (one 8-byte gather per trilinear tap) or a float volume, and the brick
-// occupancy of the cells (one bit per 8x8x8 cells, nullable).
+// distance field of the cells (4 bits per 8x8x8 cells, nullable; see
+// occupancy_words).
 struct DensityVolume {
     const unsigned long long* cells;
     const float* f32;
-    const uint32_t* occupancy;
+    const uint32_t* distance;
     int nx, ny, nz;
     uint32_t bx, by;         // bricks along x and y
-    uint32_t occ_words;      // words of the occupancy bitmask
+    uint32_t dist_words;     // words of the distance nibbles
 };
 
 constexpr uint32_t kBrickShift = 3;              // 8x8x8 cells per brick
-constexpr uint32_t kOccupancyLdsWords = 2048;    // largest bitmask staged in LDS (8 KiB)
+constexpr uint32_t kDistanceCap = 15;            // largest brick distance a nibble holds
+constexpr uint32_t kDistanceLdsWords = 8192;     // largest distance field staged in LDS (32 KiB, 65536 bricks)
 
 __host__ __device__ __forceinline__ uint32_t bricks_along(uint32_t n) { return (n + 8u) >> kBrickShift; }
 
-__host__ __device__ __forceinline__ size_t occupancy_words(uint32_t nx, uint32_t ny, uint32_t nz) {
-    return (size_t(bricks_along(nx)) * bricks_along(ny) * bricks_along(nz) + 31) / 32;
+__host__ __device__ __forceinline__ size_t brick_count(uint32_t nx, uint32_t ny, uint32_t nz) {
+    return size_t(bricks_along(nx)) * bricks_along(ny) * bricks_along(nz);
 }
 
-// Copies the occupancy bitmask into LDS when it fits; returns whether it did
+// The occupancy buffer of a volume, in uint32 words, holds three arrays:
+//   [0, bits)                  one bit per brick, set when a cell of the brick
+//                              holds a nonzero texel (brick (bx,by,bz) = cell
+//                              (i+1,j+1,k+1) >> 3 at index bx + BX*(by + BY*bz));
+//   [bits, bits + nibbles)     4 bits per brick (brick b at bits 4(b%8) of word
+//                              b/8): 0 for an occupied brick, else the
+//                              chessboard distance in bricks to the nearest
+//                              occupied one, capped at kDistanceCap -- every
+//                              brick within distance - 1 of it is empty;
+//   [bits + nibbles, end)      one byte per brick of working space for the
+//                              distance passes.
+__host__ __device__ __forceinline__ size_t occupancy_bit_words(uint32_t nx, uint32_t ny, uint32_t nz) {
+    return (brick_count(nx, ny, nz) + 31) / 32;
+}
+__host__ __device__ __forceinline__ size_t distance_words(uint32_t nx, uint32_t ny, uint32_t nz) {
+    return (brick_count(nx, ny, nz) + 7) / 8;
+}
+__host__ __device__ __forceinline__ size_t occupancy_words(uint32_t nx, uint32_t ny, uint32_t nz) {
+    return occupancy_bit_words(nx, ny, nz) + distance_words(nx, ny, nz) + (brick_count(nx, ny, nz) + 3) / 4;
+}
+
+// Longest line of bricks the distance passes transform (volume.hpp): a
+// volume with a longer one keeps its nibbles at 0, and nothing is skipped.
+constexpr uint32_t kDistanceLineMax = 256;
+
+__host__ __device__ __forceinline__ bool distance_field_fits(uint32_t nx, uint32_t ny, uint32_t nz) {
+    return bricks_along(nx) <= kDistanceLineMax && bricks_along(ny) <= kDistanceLineMax &&
+           bricks_along(nz) <= kDistanceLineMax;
+}
+
+// Copies the distance field into LDS when it fits; returns whether it did
 // (otherwise no sample is skipped).  All threads of the block call it.
-__device__ __forceinline__ bool stage_occupancy(const DensityVolume& v, uint32_t* lds) {
-    const bool use = v.occupancy && v.cells && !v.f32 && v.occ_words <= kOccupancyLdsWords;
+__device__ __forceinline__ bool stage_distance(const DensityVolume& v, uint32_t* lds) {
+    const bool use = v.distance && v.cells && !v.f32 && v.dist_words <= kDistanceLdsWords;
     if (!use) return false;
-    for (uint32_t i = threadIdx.x; i < v.occ_words; i += blockDim.x) lds[i] = v.occupancy[i];
+    for (uint32_t i = threadIdx.x; i < v.dist_words; i += blockDim.x) lds[i] = v.distance[i];
     __syncthreads();
     return true;
 }
 
-// The LDS copy of the occupancy a march reads (use = false: none).
+// LDS bytes stage_distance needs for a volume (0: it stages nothing).
+__host__ __forceinline__ size_t distance_lds_bytes(const DensityVolume& v) {
+    return v.distance && v.cells && !v.f32 && v.dist_words <= kDistanceLdsWords ? size_t(v.dist_words) * 4 : 0;
+}
+
+// The LDS copy of the distance field a march reads (use = false: none).
 struct OccupancyView {
     const uint32_t* lds;
     bool use;
 };
 
+// The distance nibble of brick b.
+__device__ __forceinline__ uint32_t brick_distance(const OccupancyView& occ, uint32_t b) {
+    return (occ.lds[b >> 3] >> ((b & 7u) << 2)) & 0xfu;
+}
+
 // texture(density, uvw).r at texel coordinate (x, y, z) = uvw * n - 0.5:
 // exact weights (the fraction of a float64 coordinate), zero border.
-// With the occupancy bitmask (staged in LDS), a sample whose cell lies in
-// an empty brick returns 0 without the gather: every texel of its footprint
-// is 0, so the result is the same.
-__device__ __forceinline__ float density_at(const DensityVolume& v, double x, double y, double z,
-                                            OccupancyView occ = {nullptr, false}) {
+__device__ __forceinline__ float density_at(const DensityVolume& v, double x, double y, double z) {
     if (!(x >= -1.0 && y >= -1.0 && z >= -1.0 && x < double(v.nx) && y < double(v.ny) && z < double(v.nz)))
         return 0.0f;
     double fx = floor(x), fy = floor(y), fz = floor(z);
     float wx = float(x - fx), wy = float(y - fy), wz = float(z - fz);
     int ix = int(fx), iy = int(fy), iz = int(fz);
-    if (occ.use) {
-        const uint32_t b = (uint32_t(ix + 1) >> kBrickShift) +
-                           v.bx * ((uint32_t(iy + 1) >> kBrickShift) + v.by * (uint32_t(iz + 1) >> kBrickShift));
-        if (!((occ.lds[b >> 5] >> (b & 31u)) & 1u)) return 0.0f;
-    }
     float b[8];
     if (v.f32) {
 #pragma unroll
@@ -98,29 +131,79 @@ __device__ __forceinline__ float density_at(const DensityVolume& v, double x, do
 // use 24-bit multiplies: the callers bound n + 1 and (ny + 1)(nz + 1)
 // below 2^24 (and the cell count below 2^32).  kF32 selects the float
 // volume.
-// With empty/cell: *empty is set when the cell is inside the volume but its
-// brick is empty (the sample reads 0), and cell[] receives the cell (i+1,
-// j+1, k+1).
+// With the distance field staged (occ.use), a sample whose brick is empty
+// (distance nibble above 0) reads 0 without its gather: every texel of its
+// footprint is 0.
 template <bool kF32>
-__device__ __forceinline__ float march_tap(const DensityVolume& v, double x, double y, double z, OccupancyView occ,
-                                           bool* empty = nullptr, uint3* cell = nullptr) {
+__device__ __forceinline__ float march_tap(const DensityVolume& v, double x, double y, double z, OccupancyView occ) {
     const double fx = floor(x), fy = floor(y), fz = floor(z);
     const int ix = int(fx), iy = int(fy), iz = int(fz);
     const uint32_t cx1 = uint32_t(ix + 1), cy1 = uint32_t(iy + 1), cz1 = uint32_t(iz + 1);
     bool live = (cx1 <= uint32_t(v.nx)) & (cy1 <= uint32_t(v.ny)) & (cz1 <= uint32_t(v.nz));
-    if (cell) *cell = make_uint3(cx1, cy1, cz1);
     if (occ.use) {
         const uint32_t b = (cx1 >> kBrickShift) +
                            __umul24(v.bx, (cy1 >> kBrickShift) + __umul24(v.by, cz1 >> kBrickShift));
-        const uint32_t word = occ.lds[live ? (b >> 5) : 0u];
-        const bool occupied = ((word >> (b & 31u)) & 1u) != 0u;
-        if (empty) *empty = live & !occupied;
-        live = live & occupied;
+        live = live & (brick_distance(occ, live ? b : 0u) == 0u);  // b is in range only for live cells
     }
     if (!live) return 0.0f;
     const float wx = float(x - fx), wy = float(y - fy), wz = float(z - fz);
     float b[8];
     if constexpr (kF32) {
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            const int tx = ix + (n & 1), ty = iy + ((n >> 1) & 1), tz = iz + (n >> 2);
+            const bool in = (tx >= 0) & (ty >= 0) & (tz >= 0) & (tx < v.nx) & (ty < v.ny) & (tz < v.nz);
+            b[n] = in ? v.f32[uint32_t(tx) + __umul24(uint32_t(v.nx), uint32_t(ty) + __umul24(uint32_t(v.ny),
+                                                                                             uint32_t(tz)))]
+                      : 0.0f;
+        }
+    } else {
+        const uint32_t c = cx1 + __umul24(uint32_t(v.nx) + 1u, cy1 + __umul24(uint32_t(v.ny) + 1u, cz1));
+        const unsigned long long q = v.cells[c];
+        const uint32_t lo = uint32_t(q), hi = uint32_t(q >> 32);
+        b[0] = float(lo & 0xffu); b[1] = float((lo >> 8) & 0xffu);
+        b[2] = float((lo >> 16) & 0xffu); b[3] = float(lo >> 24);
+        b[4] = float(hi & 0xffu); b[5] = float((hi >> 8) & 0xffu);
+        b[6] = float((hi >> 16) & 0xffu); b[7] = float(hi >> 24);
+    }
+    const float c00 = fmaf(wx, b[1] - b[0], b[0]);
+    const float c10 = fmaf(wx, b[3] - b[2], b[2]);
+    const float c01 = fmaf(wx, b[5] - b[4], b[4]);
+    const float c11 = fmaf(wx, b[7] - b[6], b[6]);
+    const float c0 = fmaf(wy, c10 - c00, c00);
+    const float c1 = fmaf(wy, c11 - c01, c01);
+    const float r = fmaf(wz, c1 - c0, c0);
+    return kF32 ? r : r * (1.0f / 255.0f);
+}
+
+// march_tap for the marches over the power-of-two sample grid, at texel
+// coordinate + 1 (x1 = x + 1, so floor(x1) is the cell index i + 1 itself).
+// Same value as march_tap at x = x1 - 1.  *region receives the radius of
+// the empty region around the sample's brick (its distance nibble - 1:
+// every brick within that chessboard radius is empty) when the sample's
+// cell is inside the volume and its brick is empty, else -1; *brick its
+// brick coordinates.
+template <bool kF32>
+__device__ __forceinline__ float grid_tap(const DensityVolume& v, double x1, double y1, double z1,
+                                          OccupancyView occ, int* region, uint3* brick) {
+    const double fx = floor(x1), fy = floor(y1), fz = floor(z1);
+    const uint32_t cx1 = uint32_t(int(fx)), cy1 = uint32_t(int(fy)), cz1 = uint32_t(int(fz));
+    bool live = (cx1 <= uint32_t(v.nx)) & (cy1 <= uint32_t(v.ny)) & (cz1 <= uint32_t(v.nz));
+    const uint3 bk = make_uint3(cx1 >> kBrickShift, cy1 >> kBrickShift, cz1 >> kBrickShift);
+    int reg = -1;
+    if (occ.use) {
+        const uint32_t b = bk.x + __umul24(v.bx, bk.y + __umul24(v.by, bk.z));
+        const uint32_t d = brick_distance(occ, live ? b : 0u);  // b is in range only for live cells
+        reg = (live & (d != 0u)) ? int(d) - 1 : -1;
+        live = live & (d == 0u);
+    }
+    *region = reg;
+    *brick = bk;
+    if (!live) return 0.0f;
+    const float wx = float(x1 - fx), wy = float(y1 - fy), wz = float(z1 - fz);
+    float b[8];
+    if constexpr (kF32) {
+        const int ix = int(cx1) - 1, iy = int(cy1) - 1, iz = int(cz1) - 1;
 #pragma unroll
         for (int n = 0; n < 8; ++n) {
             const int tx = ix + (n & 1), ty = iy + ((n >> 1) & 1), tz = iz + (n >> 2);

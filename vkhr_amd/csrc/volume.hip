@@ -10,7 +10,8 @@
 //                     view.
 // Pass 3 (per cell; blocks of 64 x 8 cells walking 32 cell levels up):
 //                     corner packing, 8 texels -> one uint64 per cell, and
-//                     the brick occupancy.
+//                     the brick occupancy bits.
+// Pass 4 (three launches over lines of bricks): the brick distance field.
 // Compiled with -ffp-contract=off (quantization and normalize are float
 // expressions whose rounding must match the reference's).
 #include "volume.hpp"
@@ -68,6 +69,15 @@ cells_kernel(const uint8_t* __restrict__ density, uint32_t nx, uint32_t ny, uint
              unsigned long long* __restrict__ cells, uint32_t* __restrict__ occupancy) {
     const uint32_t n_blocks = cell_blocks(nx, ny, nz);
     for (uint32_t b = blockIdx.x; b < n_blocks; b += gridDim.x) cells_block(density, nx, ny, nz, b, cells, occupancy);
+}
+
+// One pass of the brick distance field: four lines per block (one per wave).
+__global__ void __launch_bounds__(256)
+distance_kernel(uint32_t* __restrict__ occupancy, uint32_t nx, uint32_t ny, uint32_t nz, int axis) {
+    __shared__ uint32_t lds[256 / kWave][kDistanceLineMax];
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t line = blockIdx.x * (256 / kWave) + wave;
+    distance_line(occupancy, nx, ny, nz, axis, line, line < distance_lines(nx, ny, nz, axis), lds[wave]);
 }
 
 // Four voxels per thread (uchar4), the tail of n % 4 by the first threads.
@@ -227,6 +237,13 @@ int finalize(const uint32_t* counts, const int64_t* tangent_acc, uint32_t nx, ui
         cells_kernel<<<grid_for(cell_blocks(nx, ny, nz), 1, 1u << 20), 256, 0, s>>>(
             density_u8, nx, ny, nz, reinterpret_cast<unsigned long long*>(cells), occupancy);
         VKHR_CHECK_LAUNCH();
+        if (occupancy && distance_field_fits(nx, ny, nz)) {
+            for (int axis = 0; axis < 3; ++axis) {
+                distance_kernel<<<grid_for(distance_lines(nx, ny, nz, axis), 256 / kWave), 256, 0, s>>>(
+                    occupancy, nx, ny, nz, axis);
+                VKHR_CHECK_LAUNCH();
+            }
+        }
     }
     return VKHR_OK;
 }

@@ -263,4 +263,70 @@ __device__ __forceinline__ void cells_block(const uint8_t* __restrict__ density,
     }
 }
 
+// The brick distance field of the occupancy buffer (occupancy_words): a
+// chessboard distance transform of the occupied bricks, capped at
+// kDistanceCap, as three passes of a one-dimensional transform (along x,
+// then y, then z; for the L-infinity distance the passes compose).  A pass
+// takes one line of bricks per wave: the wave loads the line into its LDS
+// row, then each brick takes min over offsets |o| <= cap - 1 of max(|o|,
+// value at o).  Pass 0 reads the bitmask (0 occupied, cap empty) and
+// writes the line bytes; pass 1 updates the bytes in place (a line is read
+// whole before it is written); pass 2 reads the bytes and or-s the nibbles
+// (zeroed before the cells pass).  Lines longer than kDistanceLineMax
+// bricks are not transformed: the nibbles stay 0 ("occupied"), so nothing
+// is skipped.
+// Lines of pass `axis`: the bricks of the other two axes.
+__host__ __device__ __forceinline__ uint32_t distance_lines(uint32_t nx, uint32_t ny, uint32_t nz, int axis) {
+    const uint32_t bx = bricks_along(nx), by = bricks_along(ny), bz = bricks_along(nz);
+    return axis == 0 ? by * bz : axis == 1 ? bx * bz : bx * by;
+}
+
+// One line of pass `axis` per wave (inactive waves still reach the
+// barrier); lds: this wave's kDistanceLineMax words.  All threads of the
+// block call it.
+__device__ __forceinline__ void distance_line(uint32_t* occupancy, uint32_t nx, uint32_t ny, uint32_t nz, int axis,
+                                              uint32_t line, bool active, uint32_t* lds) {
+    const uint32_t lane = threadIdx.x % kWave;
+    const uint32_t bx = bricks_along(nx), by = bricks_along(ny), bz = bricks_along(nz);
+    const uint32_t len = axis == 0 ? bx : axis == 1 ? by : bz;
+    uint32_t base = 0u, stride = 1u;
+    if (axis == 0) {
+        base = bx * line;                                   // line = y + by * z
+    } else if (axis == 1) {
+        base = (line % bx) + bx * by * (line / bx);         // line = x + bx * z
+        stride = bx;
+    } else {
+        base = line;                                        // line = x + bx * y
+        stride = bx * by;
+    }
+    const uint32_t* bits = occupancy;
+    uint32_t* nibbles = occupancy + occupancy_bit_words(nx, ny, nz);
+    uint8_t* bytes = reinterpret_cast<uint8_t*>(nibbles + distance_words(nx, ny, nz));
+    constexpr uint32_t kPer = kDistanceLineMax / kWave;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const uint32_t e = lane + k * kWave;
+        if (active && e < len) {
+            const uint32_t b = base + e * stride;
+            lds[e] = axis == 0 ? (((bits[b >> 5] >> (b & 31u)) & 1u) ? 0u : kDistanceCap) : uint32_t(bytes[b]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const uint32_t e = lane + k * kWave;
+        if (!(active && e < len)) continue;
+        uint32_t m = lds[e];
+        for (uint32_t o = 1; o < kDistanceCap && m > o; ++o) {
+            if (e >= o) m = min(m, max(o, lds[e - o]));
+            if (e + o < len) m = min(m, max(o, lds[e + o]));
+        }
+        const uint32_t b = base + e * stride;
+        if (axis < 2)
+            bytes[b] = uint8_t(m);
+        else if (m)
+            atomicOr(&nibbles[b >> 3], m << ((b & 7u) << 2));
+    }
+}
+
 }  // namespace vkhr

@@ -4,6 +4,8 @@
 // and writes the results for comparison with the oracle.
 //
 //   host_check <input.bin> <output-dir>
+//   host_check --save <input.bin> <out.hair>
+//   host_check --hair <style.hair> <seed> <params.bin> <output-dir>
 //
 // input.bin (little endian): u32 n_strands, n_vertices, volume_n, width,
 // height, shadow_size; u16 segments[n_strands]; f32 xyz[3*n_vertices];
@@ -14,9 +16,19 @@
 // f32 (lod_magnified_distance, lod_minified_distance), each drawn as one
 // hybrid LoD colour pass into hybrid<k>.bin, with hybrid<k>.txt holding the
 // level of detail, the two enable flags and the PPLL nodes used.
+//
+// --save writes the style of input.bin (segments and vertices) as a .hair
+// file with HairStyle::save.  --hair loads a .hair file with
+// HairStyle(path) and prepares it as SceneGraph::add_style does
+// (src/vkhr/scene_graph.cc:222-245: shuffle with the given seed, then the
+// tangents, thickness, indices and bounding box the file does not hold),
+// then runs the pipeline above; params.bin holds u32 volume_n, width,
+// height, shadow_size and the blocks that follow the vertices in input.bin.
+// It also writes the prepared style's vertices, tangents and indices.
 #include <hip/hip_runtime_api.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <string>
 #include <vector>
@@ -38,20 +50,60 @@ void write(const std::string& path, const void* p, size_t bytes) {
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc != 3) {
-        std::fprintf(stderr, "usage: %s <input.bin> <output-dir>\n", argv[0]);
+    const std::string mode = argc > 1 ? argv[1] : "";
+    const bool save = mode == "--save", hair = mode == "--hair";
+    if (!((argc == 3 && !save && !hair) || (save && argc == 4) || (hair && argc == 6))) {
+        std::fprintf(stderr,
+                     "usage: %s <input.bin> <output-dir> | --save <input.bin> <out.hair> | "
+                     "--hair <style.hair> <seed> <params.bin> <output-dir>\n",
+                     argv[0]);
         return 2;
     }
     try {
-        std::ifstream in(argv[1], std::ios::binary);
-        uint32_t hdr[6];
-        read(in, hdr, 6);
-        const uint32_t n_strands = hdr[0], n_vertices = hdr[1], n = hdr[2], W = hdr[3], H = hdr[4], S = hdr[5];
         vkhr::HairStyle style;
-        style.segments.resize(n_strands);
-        style.vertices.resize(n_vertices);
-        read(in, style.segments.data(), n_strands);
-        read(in, style.vertices.data(), n_vertices);
+        uint32_t n = 0, W = 0, H = 0, S = 0;
+        std::ifstream in;
+        std::string out;
+        if (hair) {
+            style = vkhr::HairStyle(argv[2]);
+            if (!style) throw std::runtime_error(std::string("cannot load ") + argv[2]);
+            // SceneGraph::add_style preprocessing (src/vkhr/scene_graph.cc:232-242)
+            style.seed = std::strtoull(argv[3], nullptr, 0);
+            style.shuffle();
+            if (!style.has_tangents()) style.generate_tangents();
+            if (!style.has_thickness()) style.generate_thickness(0.042f);
+            if (!style.has_indices()) style.generate_indices();
+            if (!style.has_bounding_box()) style.generate_bounding_box();
+            in.open(argv[4], std::ios::binary);
+            uint32_t hdr[4];
+            read(in, hdr, 4);
+            n = hdr[0], W = hdr[1], H = hdr[2], S = hdr[3];
+            out = argv[5];
+            write(out + "/vertices.bin", style.vertices.data(), style.vertices.size() * 12);
+            write(out + "/style_tangents.bin", style.tangents.data(), style.tangents.size() * 12);
+        } else {
+            in.open(argv[save ? 2 : 1], std::ios::binary);
+            uint32_t hdr[6];
+            read(in, hdr, 6);
+            const uint32_t n_strands = hdr[0], n_vertices = hdr[1];
+            n = hdr[2], W = hdr[3], H = hdr[4], S = hdr[5];
+            style.segments.resize(n_strands);
+            style.vertices.resize(n_vertices);
+            read(in, style.segments.data(), n_strands);
+            read(in, style.vertices.data(), n_vertices);
+            if (save) {
+                if (!style.save(argv[3])) throw std::runtime_error(std::string("cannot save ") + argv[3]);
+                std::printf("host_check: saved %u strands, %u vertices\n", style.get_strand_count(),
+                            style.get_vertex_count());
+                return 0;
+            }
+            out = argv[2];
+            // SceneGraph::add_style preprocessing order (src/vkhr/scene_graph.cc:235-242)
+            style.generate_tangents();
+            style.generate_thickness(0.042f);
+            style.generate_indices();
+            style.generate_bounding_box();
+        }
         vkhr_camera cam;
         vkhr_light light;
         vkhr_strand_params sp;
@@ -60,12 +112,6 @@ int main(int argc, char** argv) {
         read(in, &light, 1);
         read(in, &sp, 1);
         read(in, &rp, 1);
-
-        // SceneGraph::add_style preprocessing order (src/vkhr/scene_graph.cc:235-242)
-        style.generate_tangents();
-        style.generate_thickness(0.042f);
-        style.generate_indices();
-        style.generate_bounding_box();
 
         // vulkan::HairStyle::load (src/vkhr/rasterizer/hair_style.cc:75-77)
         vkhr::HairStyle::Volume volume = style.voxelize_segments(n, n, n);
@@ -77,7 +123,6 @@ int main(int argc, char** argv) {
         vkhr::AABB b = style.get_bounding_box();
         sp.volume_bounds = vkhr_aabb{{b.origin.x, b.origin.y, b.origin.z}, b.radius, {b.size.x, b.size.y, b.size.z},
                                      b.volume};
-        const std::string out = argv[2];
         write(out + "/aabb.bin", &sp.volume_bounds, sizeof(vkhr_aabb));
         write(out + "/counts.bin", counts.data(), counts.size() * 4);
         write(out + "/density.bin", volume.densities.data(), volume.densities.size());
