@@ -14,7 +14,9 @@
  * different streams use different areas, so they may run concurrently.
  * hipStreamPerThread gets one area per calling thread.  An area that must
  * grow synchronizes its stream and frees the old buffer (during graph
- * capture the old buffer is kept until the release instead).
+ * capture the old buffer is kept until the release instead).  The one
+ * allocation-free entry point (vkhr_volume_finalize_noalloc) uses a static
+ * 4 KiB array per device instead, shared by all its calls on that device.
  *
  * Layouts:
  *   xyz           float32 x,y,z per vertex, tightly packed (12 B / vertex)
@@ -260,6 +262,25 @@ int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tangent_acc,
                          uint8_t* density_u8, int8_t* tangent_i8x4,
                          float* density_f32, uint64_t* cells, uint32_t* occupancy,
                          vkhr_stream stream);
+
+/* vkhr_volume_finalize that allocates nothing: same arguments, same
+ * results, but the min/max slots of the normalize (2 KiB) live in a static
+ * per-device array of VKHR_STATIC_SCRATCH_BYTES (4 KiB), resident from
+ * library load until the process ends, instead of the per-stream scratch.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight per-frame serving
+ * loops, harnesses that forbid hipMalloc.  The call makes no hipMalloc or
+ * hipFree and keeps no per-stream area (vkhr_scratch_bytes stays 0).  Every
+ * call on a device shares the one array (its slots are re-initialised by a
+ * kernel at the start of each call), so two calls that overlap on one
+ * device -- concurrent streams or threads -- race; order them.  Returns
+ * VKHR_ERR_OVERFLOW if the scratch needed exceeds the array. */
+#define VKHR_STATIC_SCRATCH_BYTES 4096
+int vkhr_volume_finalize_noalloc(const uint32_t* counts, const int64_t* tangent_acc,
+                                 uint32_t nx, uint32_t ny, uint32_t nz, int normalize,
+                                 uint8_t* density_u8, int8_t* tangent_i8x4,
+                                 float* density_f32, uint64_t* cells, uint32_t* occupancy,
+                                 vkhr_stream stream);
 
 /* Words of the brick occupancy buffer of an nx x ny x nz volume (bitmask,
  * distance nibbles and working space, see vkhr_volume_finalize). */

@@ -64,6 +64,16 @@ int check_volume(uint32_t nx, uint32_t ny, uint32_t nz, bool cells, const char* 
 // the old buffer (outside graph capture).
 int stream_scratch(size_t bytes, hipStream_t stream, void** out);
 
+// Bytes of the static per-device scratch of the allocation-free entry
+// points (a file-scope __device__ array, no allocation at run time).
+constexpr size_t kStaticScratchBytes = VKHR_STATIC_SCRATCH_BYTES;
+
+// The current device's static scratch, after checking that `bytes` fit
+// (VKHR_ERR_OVERFLOW otherwise).  Every call on a device shares it and it
+// keeps the last call's contents: users clear what must start at zero, and
+// calls that overlap on one device race.
+int static_scratch(size_t bytes, void** out);
+
 // Synchronizes the stream and frees its area (vkhr_release_scratch).
 int release_stream_scratch(hipStream_t stream);
 

@@ -30,6 +30,14 @@ struct Scratch {
 using ScratchKey = std::tuple<int, hipStream_t, size_t>;
 std::map<ScratchKey, Scratch> g_scratch;
 
+// The fixed scratch of the allocation-free entry points: one copy per
+// device, resident from module load until the process ends.  Host code
+// never passes `scratch` itself (its host-side name is not a device
+// address); ensure_device looks up each device's copy once, with that
+// device current, and keeps the address in g_static_scratch.
+static __device__ __align__(256) uint32_t scratch[kStaticScratchBytes / sizeof(uint32_t)];
+void* g_static_scratch[kMaxDevices] = {nullptr};
+
 ScratchKey scratch_key(int dev, hipStream_t stream) {
     const size_t thread = stream == hipStreamPerThread ? std::hash<std::thread::id>()(std::this_thread::get_id()) : 0;
     return ScratchKey(dev, stream, thread);
@@ -55,7 +63,23 @@ int ensure_device() {
     VKHR_CHECK_HIP(hipGetDeviceProperties(&prop, dev));
     VKHR_REQUIRE(std::strncmp(prop.gcnArchName, "gfx950", 6) == 0, VKHR_ERR_NO_DEVICE,
                  "device %d is %s; this library is built for gfx950 only", dev, prop.gcnArchName);
+    void* fixed = nullptr;
+    VKHR_CHECK_HIP(hipGetSymbolAddress(&fixed, HIP_SYMBOL(scratch)));
+    VKHR_REQUIRE(fixed, VKHR_ERR_HIP, "device %d: no address for the static scratch", dev);
+    g_static_scratch[dev] = fixed;
     g_checked[dev] = true;
+    return VKHR_OK;
+}
+
+int static_scratch(size_t bytes, void** out) {
+    int rc = ensure_device();
+    if (rc) return rc;
+    VKHR_REQUIRE(bytes <= kStaticScratchBytes, VKHR_ERR_OVERFLOW,
+                 "needs %zu bytes of static scratch; the allocation-free entry points have %zu", bytes,
+                 kStaticScratchBytes);
+    int dev = 0;
+    VKHR_CHECK_HIP(hipGetDevice(&dev));
+    *out = g_static_scratch[dev];  // set under g_mutex before g_checked[dev]
     return VKHR_OK;
 }
 

@@ -194,7 +194,8 @@ namespace vkhr {
 namespace {
 
 // vkhr_volume_finalize with the min/max slots either from the per-stream
-// scratch (minmax_static == nullptr) or from a buffer the library owns.
+// scratch (minmax_static == nullptr) or from the static per-device scratch
+// (vkhr_volume_finalize_noalloc), which it then neither allocates nor frees.
 int finalize(const uint32_t* counts, const int64_t* tangent_acc, uint32_t nx, uint32_t ny, uint32_t nz,
              int normalize, uint8_t* density_u8, int8_t* tangent_i8x4, float* density_f32, uint64_t* cells,
              uint32_t* occupancy, hipStream_t s, uint32_t* minmax_static) {
@@ -227,6 +228,15 @@ int finalize(const uint32_t* counts, const int64_t* tangent_acc, uint32_t nx, ui
     finalize_counts_kernel<<<grid_for(n / 4 + 1, 256, 2048), 256, 0, s>>>(
         counts, reinterpret_cast<const long long*>(tangent_acc), n, density_u8, tangent_i8x4, mm);
     VKHR_CHECK_LAUNCH();
+    return finish_volume(density_u8, nx, ny, nz, normalize, density_f32, cells, occupancy, mm, s);
+}
+
+}  // namespace
+
+int finish_volume(uint8_t* density_u8, uint32_t nx, uint32_t ny, uint32_t nz, int normalize, float* density_f32,
+                  uint64_t* cells, uint32_t* occupancy, const uint32_t* mm, hipStream_t s) {
+    int rc = VKHR_OK;
+    const size_t n = size_t(nx) * ny * nz;
     if (normalize || density_f32) {
         // capped: each wave first reduces the min/max slots
         normalize_kernel<<<grid_for(n / 4 + 1, 256, 2048), 256, 0, s>>>(density_u8, n, mm, normalize, density_f32);
@@ -248,7 +258,6 @@ int finalize(const uint32_t* counts, const int64_t* tangent_acc, uint32_t nx, ui
     return VKHR_OK;
 }
 
-}  // namespace
 }  // namespace vkhr
 
 extern "C" int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tangent_acc, uint32_t nx, uint32_t ny,
@@ -260,6 +269,22 @@ extern "C" int vkhr_volume_finalize(const uint32_t* counts, const int64_t* tange
     if (rc) return rc;
     return vkhr::finalize(counts, tangent_acc, nx, ny, nz, normalize, density_u8, tangent_i8x4, density_f32, cells,
                           occupancy, as_stream(stream), nullptr);
+}
+
+static_assert(kMinMaxBytes <= kStaticScratchBytes, "the min/max slots must fit the static scratch");
+
+extern "C" int vkhr_volume_finalize_noalloc(const uint32_t* counts, const int64_t* tangent_acc, uint32_t nx,
+                                            uint32_t ny, uint32_t nz, int normalize, uint8_t* density_u8,
+                                            int8_t* tangent_i8x4, float* density_f32, uint64_t* cells,
+                                            uint32_t* occupancy, vkhr_stream stream) {
+    clear_error();
+    void* slots = nullptr;
+    int rc = static_scratch(kMinMaxBytes, &slots);  // binds the device and checks the size
+    if (rc) return rc;
+    // the min/max slots are re-initialised by a kernel at the start of every
+    // call that normalizes, so the previous call's contents never leak in
+    return vkhr::finalize(counts, tangent_acc, nx, ny, nz, normalize, density_u8, tangent_i8x4, density_f32, cells,
+                          occupancy, as_stream(stream), static_cast<uint32_t*>(slots));
 }
 
 extern "C" size_t vkhr_occupancy_words(uint32_t nx, uint32_t ny, uint32_t nz) { return occupancy_words(nx, ny, nz); }

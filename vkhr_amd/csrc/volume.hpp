@@ -20,12 +20,10 @@ __device__ __forceinline__ uint32_t voxel_count(const uint32_t* __restrict__ cou
 // reference keeps only the first 255 in its serial order, so saturated
 // voxels may differ).  The sums come from the fixed-point accumulator
 // (voxel.hpp); beyond 32767 contributions they are not decodable and the
-// tangent is 0, as for empty voxels.
-__device__ __forceinline__ char4 quantize_tangent_words(long long w0, long long w1, uint32_t c) {
+// tangent is 0, as for empty voxels.  (sx, sy, sz): the fixed-point sums
+// of the c contributions.
+__device__ __forceinline__ char4 quantize_tangent_sums(long long sx, long long sy, long long sz, uint32_t c) {
     if (!(c && c < 32768u)) return make_char4(0, 0, 0, 0);
-    const long long sy = (long long)(int)(unsigned)(w0 & 0xffffffffll);
-    const long long sx = (w0 - sy) / (1ll << 32);
-    const long long sz = w1 >> 32;  // w1 = z * 2^32 + count, 0 <= count < 2^32
     const float inv = 1.0f / 65536.0f;
     const float denom = float(c);
     const float x = float(sx) * inv / denom * 127.0f;
@@ -36,6 +34,15 @@ __device__ __forceinline__ char4 quantize_tangent_words(long long w0, long long 
         return static_cast<signed char>(static_cast<int>(v));
     };
     return make_char4(cvt(x), cvt(y), cvt(z), 0);
+}
+
+// The same from the two packed words of the accumulator (voxel.hpp).
+__device__ __forceinline__ char4 quantize_tangent_words(long long w0, long long w1, uint32_t c) {
+    if (!(c && c < 32768u)) return make_char4(0, 0, 0, 0);
+    const long long sy = (long long)(int)(unsigned)(w0 & 0xffffffffll);
+    const long long sx = (w0 - sy) / (1ll << 32);
+    const long long sz = w1 >> 32;  // w1 = z * 2^32 + count, 0 <= count < 2^32
+    return quantize_tangent_sums(sx, sy, sz, c);
 }
 
 __device__ __forceinline__ char4 quantize_tangent(const long long* __restrict__ acc, size_t i, uint32_t c) {
@@ -142,6 +149,13 @@ __device__ __forceinline__ DensityMap density_map(const uint32_t* minmax, bool n
 constexpr int kMinMaxSlots = 32;
 constexpr int kMinMaxStride = 16;  // words between slots
 constexpr size_t kMinMaxBytes = size_t(kMinMaxSlots) * kMinMaxStride * sizeof(uint32_t);
+
+// The passes after the uint8 view and its min/max slots (mm, nullable when
+// normalize is 0): the normalize in place and the float view, then the
+// corner-packed cells, their brick occupancy and the brick distance field
+// (each output nullable).  Enqueued on s (volume.hip).
+int finish_volume(uint8_t* density_u8, uint32_t nx, uint32_t ny, uint32_t nz, int normalize, float* density_f32,
+                  uint64_t* cells, uint32_t* occupancy, const uint32_t* mm, hipStream_t s);
 
 // Min/max over the slots; called by every lane of a wave (shuffles).
 __device__ __forceinline__ DensityMap density_map_slots(const uint32_t* minmax, bool normalize) {

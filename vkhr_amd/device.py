@@ -81,9 +81,10 @@ def occupancy_words(res) -> int:
 
 
 def finalize(counts: torch.Tensor | None, res, sums: torch.Tensor | None = None, normalize: bool = True,
-             want_f32: bool = False, want_cells: bool = True, want_occupancy: bool = True):
+             want_f32: bool = False, want_cells: bool = True, want_occupancy: bool = True, noalloc: bool = False):
     """counts may be None when sums holds them packed.  The brick occupancy
-    comes with the cells unless want_occupancy is False."""
+    comes with the cells unless want_occupancy is False.  noalloc: through
+    vkhr_volume_finalize_noalloc (static scratch, no library allocation)."""
     nx, ny, nz = res
     n = nx * ny * nz
     dev = (counts if counts is not None else sums).device
@@ -93,7 +94,8 @@ def finalize(counts: torch.Tensor | None, res, sums: torch.Tensor | None = None,
     cells = torch.empty((nx + 1) * (ny + 1) * (nz + 1), dtype=torch.int64, device=dev) if want_cells else None
     occ = (torch.empty(occupancy_words(res), dtype=torch.int32, device=dev)
            if want_cells and want_occupancy else None)
-    call("vkhr_volume_finalize", ptr(counts), ptr(sums), nx, ny, nz, int(normalize), ptr(dens), ptr(tan),
+    call("vkhr_volume_finalize_noalloc" if noalloc else "vkhr_volume_finalize", ptr(counts), ptr(sums), nx, ny, nz,
+         int(normalize), ptr(dens), ptr(tan),
          ptr(f32), ptr(cells), ptr(occ), stream_of())
     return dict(density=dens, tangents=tan, density_f32=f32, cells=cells, occupancy=occ)
 
