@@ -9,7 +9,8 @@
  * the only hidden state is a scratch area per device and stream that the
  * library allocates on first use and keeps until vkhr_release_scratch on
  * that stream (or the process ends): reductions, counters, the adjoint's
- * cell accumulator, used by vkhr_aabb_compute, vkhr_volume_finalize and
+ * cell accumulator, the bins of the volume build, used by
+ * vkhr_aabb_compute, vkhr_volume_finalize, vkhr_volume_build_segments and
  * vkhr_raymarch_adjoint.  Calls on one stream are ordered by it; calls on
  * different streams use different areas, so they may run concurrently.
  * hipStreamPerThread gets one area per calling thread.  An area that must
@@ -281,6 +282,30 @@ int vkhr_volume_finalize_noalloc(const uint32_t* counts, const int64_t* tangent_
                                  uint8_t* density_u8, int8_t* tangent_i8x4,
                                  float* density_f32, uint64_t* cells, uint32_t* occupancy,
                                  vkhr_stream stream);
+
+/* HairStyle::voxelize_segments (hair_style.cc:296-342) followed by
+ * HairStyle::Volume::normalize (:344-357) -- the reference's load-time
+ * volume build (rasterizer/hair_style.cc:75-77) -- in one call, with the
+ * outputs of vkhr_voxelize_segments + vkhr_volume_finalize and the same
+ * bits: counts (nullable, uint32 per voxel), density_u8, tangent_i8x4
+ * (nullable; needs tangents), density_f32, cells and occupancy (nullable),
+ * as documented there.  The voxel sums are kept in LDS: the index pairs
+ * are binned by the 8x8x8-voxel bricks their samples visit, and one
+ * workgroup per brick accumulates and finalizes its brick, so nothing of
+ * the size of the volume is cleared or read back beyond the outputs.  The
+ * bins live in the per-stream scratch (4 entries per pair plus 65536);
+ * workspace (16 bytes per voxel, 16-byte aligned, contents undefined
+ * afterwards) is used only when the pairs visit more bricks than that
+ * (long segments): a flag set on the device then switches to the packed
+ * accumulator path of vkhr_voxelize_segments, with the same results.
+ * Same limits as vkhr_voxelize_segments (nx*ny*nz <= 2^24). */
+int vkhr_volume_build_segments(const float* xyz, uint32_t n_vertices,
+                               const uint32_t* indices, uint32_t n_indices,
+                               const float* tangents, const vkhr_aabb* bounds,
+                               uint32_t nx, uint32_t ny, uint32_t nz, int normalize,
+                               uint32_t* counts, uint8_t* density_u8, int8_t* tangent_i8x4,
+                               float* density_f32, uint64_t* cells, uint32_t* occupancy,
+                               int64_t* workspace, vkhr_stream stream);
 
 /* Words of the brick occupancy buffer of an nx x ny x nz volume (bitmask,
  * distance nibbles and working space, see vkhr_volume_finalize). */

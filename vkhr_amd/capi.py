@@ -129,6 +129,7 @@ SIGNATURES = {
     "vkhr_voxelize_segments": [P, U32, P, U32, P, C.POINTER(AABB), U32, U32, U32, P, P, P],
     "vkhr_volume_finalize": [P, P, U32, U32, U32, I32, P, P, P, P, P, P],
     "vkhr_volume_finalize_noalloc": [P, P, U32, U32, U32, I32, P, P, P, P, P, P],
+    "vkhr_volume_build_segments": [P, U32, P, U32, P, C.POINTER(AABB), U32, U32, U32, I32, P, P, P, P, P, P, P, P],
     "vkhr_raymarch": [C.POINTER(Volume), C.POINTER(Camera), C.POINTER(Light), C.POINTER(StrandParams),
                       C.POINTER(RenderParams), U32, U32, P, P, P, P, P],
     "vkhr_raymarch_ppll": [C.POINTER(Volume), C.POINTER(Camera), C.POINTER(Light), C.POINTER(StrandParams),

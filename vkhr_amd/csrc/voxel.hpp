@@ -21,7 +21,8 @@ struct VoxelGrid {
     uint32_t nx, ny, nz;
 };
 
-__device__ __forceinline__ uint32_t voxel_of(const VoxelGrid& g, float px, float py, float pz) {
+// Voxel coordinates of a sample in voxel units.
+__device__ __forceinline__ uint3 voxel_coords(const VoxelGrid& g, float px, float py, float pz) {
     // glm::min(glm::floor(p), resolution - 1): the upper clamp of the
     // reference; the lower clamp at 0 only guards samples that leave the box
     // through float rounding (undefined behaviour in the reference).
@@ -31,7 +32,51 @@ __device__ __forceinline__ uint32_t voxel_of(const VoxelGrid& g, float px, float
     int ix = fx > 0.0f ? int(fx) : 0;
     int iy = fy > 0.0f ? int(fy) : 0;
     int iz = fz > 0.0f ? int(fz) : 0;
-    return uint32_t(ix) + g.nx * (uint32_t(iy) + g.ny * uint32_t(iz));
+    return make_uint3(uint32_t(ix), uint32_t(iy), uint32_t(iz));
+}
+
+__device__ __forceinline__ uint32_t voxel_of(const VoxelGrid& g, float px, float py, float pz) {
+    const uint3 c = voxel_coords(g, px, py, pz);
+    return c.x + g.nx * (c.y + g.ny * c.z);
+}
+
+// The DDA of one index pair (hair_style.cc:311-329): root and per-sample
+// direction in voxel units and the remaining step count; the samples are
+// taken while steps > 0, each one decrementing steps by 1 and then
+// advancing root by the direction (advance()).
+struct SegmentDda {
+    float rx, ry, rz, dx, dy, dz, steps;
+
+    __device__ __forceinline__ void advance() {
+        rx += dx;
+        ry += dy;
+        rz += dz;
+    }
+};
+
+// The set-up of the reference's DDA; a malformed pair, or a non-finite or
+// absurd length (vertices far outside the given box, which would never
+// finish), gives no samples (steps = 0).
+__device__ __forceinline__ SegmentDda segment_dda(const VoxelGrid& g, const float* __restrict__ xyz,
+                                                  uint32_t n_vertices, uint32_t ia, uint32_t ib) {
+    SegmentDda w = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (!(ia < n_vertices && ib < n_vertices)) return w;
+    float3 a = load_xyz(xyz, ia);
+    float3 b = load_xyz(xyz, ib);
+    // root / tip in voxel units (hair_style.cc:312-313)
+    w.rx = (a.x - g.ox) / g.vx; w.ry = (a.y - g.oy) / g.vy; w.rz = (a.z - g.oz) / g.vz;
+    float tx = (b.x - g.ox) / g.vx, ty = (b.y - g.oy) / g.vy, tz = (b.z - g.oz) / g.vz;
+    w.dx = tx - w.rx; w.dy = ty - w.ry; w.dz = tz - w.rz;
+    // steps = compMax(abs(direction)); direction /= steps (:315-317)
+    w.steps = fmaxf(fmaxf(fabsf(w.dx), fabsf(w.dy)), fabsf(w.dz));
+    if (w.steps <= kMaxSteps) {
+        w.dx /= w.steps;
+        w.dy /= w.steps;
+        w.dz /= w.steps;
+    } else {
+        w.steps = 0.0f;
+    }
+    return w;
 }
 
 // Tangent accumulation in fixed point, 2^-16 per unit, packed with the count
@@ -108,43 +153,23 @@ __device__ __forceinline__ void voxelize_segment(const VoxelGrid& g, const float
                                                  uint32_t n_vertices, bool has, uint32_t ia, uint32_t ib,
                                                  const float* __restrict__ tangents, uint32_t* __restrict__ counts,
                                                  long long* __restrict__ sums) {
-    float rx = 0.0f, ry = 0.0f, rz = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f, steps = 0.0f;
+    SegmentDda w = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     unsigned long long w0 = 0, w1 = 0;
-    if (has && ia < n_vertices && ib < n_vertices) {  // a malformed pair is skipped
-        float3 a = load_xyz(xyz, ia);
-        float3 b = load_xyz(xyz, ib);
-        // root / tip in voxel units (hair_style.cc:312-313)
-        rx = (a.x - g.ox) / g.vx; ry = (a.y - g.oy) / g.vy; rz = (a.z - g.oz) / g.vz;
-        float tx = (b.x - g.ox) / g.vx, ty = (b.y - g.oy) / g.vy, tz = (b.z - g.oz) / g.vz;
-        dx = tx - rx; dy = ty - ry; dz = tz - rz;
-        // steps = compMax(abs(direction)); direction /= steps (:315-317)
-        steps = fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fabsf(dz));
-        // A non-finite or absurd length (vertices far outside the given box)
-        // would never finish; such segments are skipped.
-        if (steps <= kMaxSteps) {
-            dx /= steps;
-            dy /= steps;
-            dz /= steps;
-            if (sums) packed_words(load_xyz(tangents, ia), &w0, &w1);
-        } else {
-            steps = 0.0f;
-        }
+    if (has) {
+        w = segment_dda(g, xyz, n_vertices, ia, ib);
+        if (sums && w.steps > 0.0f) packed_words(load_xyz(tangents, ia), &w0, &w1);
     }
     // while (steps-- > 0.0f): ceil(steps) samples, root accumulated (:319-328)
-    while (__ballot(steps > 0.0f)) {
-        const bool on = steps > 0.0f;
+    while (__ballot(w.steps > 0.0f)) {
+        const bool on = w.steps > 0.0f;
         uint32_t v = 0;
         if (on) {
-            steps -= 1.0f;
-            v = voxel_of(g, rx, ry, rz);
+            w.steps -= 1.0f;
+            v = voxel_of(g, w.rx, w.ry, w.rz);
             if (!sums) atomicAdd(&counts[v], 1u);
         }
         add_packed_wave(on && sums ? &sums[2ull * v] : nullptr, w0, w1);
-        if (on) {
-            rx += dx;
-            ry += dy;
-            rz += dz;
-        }
+        if (on) w.advance();
     }
 }
 

@@ -100,6 +100,35 @@ def finalize(counts: torch.Tensor | None, res, sums: torch.Tensor | None = None,
     return dict(density=dens, tangents=tan, density_f32=f32, cells=cells, occupancy=occ)
 
 
+def build_volume(xyz: torch.Tensor, bounds, res, indices: torch.Tensor, tangents: torch.Tensor | None = None,
+                 normalize: bool = True, want_counts: bool = False, want_tangents: bool = True,
+                 want_f32: bool = False, want_cells: bool = True, want_occupancy: bool = True,
+                 workspace: torch.Tensor | None = None, out: dict | None = None):
+    """vkhr_volume_build_segments: the voxelization and the finalize in one
+    call (LDS-binned).  Returns the dict of finalize plus counts (or None);
+    out: a previous result whose buffers are reused."""
+    nx, ny, nz = res
+    n = nx * ny * nz
+    dev = xyz.device
+    if out is None:
+        out = dict(
+            counts=torch.empty(n, dtype=torch.int32, device=dev) if want_counts else None,
+            density=torch.empty(n, dtype=torch.uint8, device=dev),
+            tangents=torch.empty(4 * n, dtype=torch.int8, device=dev) if want_tangents else None,
+            density_f32=torch.empty(n, dtype=torch.float32, device=dev) if want_f32 else None,
+            cells=torch.empty((nx + 1) * (ny + 1) * (nz + 1), dtype=torch.int64, device=dev) if want_cells else None,
+            occupancy=(torch.empty(occupancy_words(res), dtype=torch.int32, device=dev)
+                       if want_cells and want_occupancy else None))
+    if workspace is None:
+        workspace = torch.empty(2 * n, dtype=torch.int64, device=dev)
+    b = capi.make_aabb(bounds)
+    call("vkhr_volume_build_segments", ptr(xyz), xyz.shape[0], ptr(indices), indices.shape[0],
+         ptr(tangents) if out["tangents"] is not None else None, C.byref(b), nx, ny, nz, int(normalize),
+         ptr(out["counts"]), ptr(out["density"]), ptr(out["tangents"]), ptr(out["density_f32"]), ptr(out["cells"]),
+         ptr(out["occupancy"]), ptr(workspace), stream_of())
+    return out
+
+
 def volume_struct(vol: dict, res) -> capi.Volume:
     v = capi.Volume()
     v.density = ptr(vol.get("density"))
