@@ -334,7 +334,9 @@ PROFILE_SUMMARY = ROOT / "profiles" / "raymarch_counters_136k_strands_1080p.md"
 # files as they were when it was taken, and its counters are used only for
 # the same sources.
 RAYMARCH_SOURCES = ("vkhr_amd/csrc/raymarch.hip", "vkhr_amd/csrc/march.hpp", "vkhr_amd/csrc/render.hpp",
-                    "vkhr_amd/csrc/common.hpp", "include/vkhr_hip.h", "vkhr_amd/Makefile")
+                    "vkhr_amd/csrc/common.hpp")
+# the lines of vkhr_amd/Makefile that set the device compile flags
+MAKEFILE_FLAG_LINES = ("HIPFLAGS", "STRICT", "SRC_FAST", "             -Xclang", "             -I")
 RAYMARCH_KERNEL = "raymarch_kernel<false>"
 CLOCK_HZ = 2.4e9          # MI355X peak engine clock
 SIMDS = 1024              # 256 CUs x 4 SIMDs
@@ -346,6 +348,9 @@ def source_hash(files=RAYMARCH_SOURCES) -> str:
     for f in files:
         h.update(f.encode())
         h.update((ROOT / f).read_bytes())
+    for line in (ROOT / "vkhr_amd" / "Makefile").read_text().splitlines():
+        if line.startswith(MAKEFILE_FLAG_LINES):
+            h.update(line.encode())
     return h.hexdigest()[:16]
 
 
@@ -440,19 +445,32 @@ def summarize_profile(root: Path, out: Path):
     w_scale = copy_bytes / (cw[copy_name]["WRITE_SIZE"] * 1024.0)
     names = sorted(set(fetch) | set(sq), key=lambda n: -float(stats.get(n, {}).get("TotalDurationNs", 0)))
     rm = next(n for n in names if RAYMARCH_KERNEL in n)
-    try:
-        commit = subprocess.run(["git", "-C", str(ROOT), "rev-parse", "--short=12", "HEAD"], capture_output=True,
-                                text=True).stdout.strip()
-        dirty = subprocess.run(["git", "-C", str(ROOT), "status", "--porcelain", "--", *RAYMARCH_SOURCES],
-                               capture_output=True, text=True).stdout.strip()
-    except OSError:
-        commit, dirty = "unknown", ""
-    avg_ns = float(stats[rm]["AverageNs"]) if rm in stats else None
+    # the tree the counters were taken of: VKHR_PROFILE_TREE when the run
+    # happened in a copy without git metadata, else the checkout's HEAD
+    commit, dirty = os.environ.get("VKHR_PROFILE_TREE", ""), ""
+    if not commit:
+        try:
+            commit = subprocess.run(["git", "-C", str(ROOT), "rev-parse", "--short=12", "HEAD"], capture_output=True,
+                                    text=True).stdout.strip() or "unknown"
+            dirty = subprocess.run(["git", "-C", str(ROOT), "status", "--porcelain", "--", *RAYMARCH_SOURCES],
+                                   capture_output=True, text=True).stdout.strip()
+        except OSError:
+            commit = "unknown"
+    # the average over the dispatches that ran alone: in the pipelined steps
+    # a raymarch overlaps the next step's volume build on the other stream,
+    # and bench.py times the kernel alone
+    trace = _csv_rows(root / "stats", "kernel_trace.csv")
+    spans = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]) for r in trace]
+    solo = [e - b for b, e, n in spans if n == rm and not any(b2 < e and e2 > b for b2, e2, n2 in spans
+                                                              if (b2, e2, n2) != (b, e, n))]
+    avg_all_ns = float(stats[rm]["AverageNs"]) if rm in stats else None
+    avg_ns = sum(solo) / len(solo) if solo else avg_all_ns
     valu = sq[rm].get("SQ_INSTS_VALU")
     traffic = fetch[rm]["FETCH_SIZE"] * 1024.0 * f_scale + write[rm]["WRITE_SIZE"] * 1024.0 * w_scale
     block = {
         "source_hash": source_hash(), "tree": commit + ("+uncommitted kernel sources" if dirty else ""),
-        "kernel": rm, "avg_duration_us": avg_ns / 1e3 if avg_ns else None,
+        "kernel": rm, "avg_duration_us": avg_ns / 1e3 if avg_ns else None, "solo_dispatches": len(solo),
+        "avg_duration_all_dispatches_us": avg_all_ns / 1e3 if avg_all_ns else None,
         "valu_wave_instructions": valu, "salu_wave_instructions": sq[rm].get("SQ_INSTS_SALU"),
         "vmem_read_wave_instructions": sq[rm].get("SQ_INSTS_VMEM_RD"), "lds_wave_instructions": sq[rm].get("SQ_INSTS_LDS"),
         "waves": sq[rm].get("SQ_WAVES"),
@@ -477,8 +495,9 @@ def summarize_profile(root: Path, out: Path):
         f"{cw[copy_name]['WRITE_SIZE']:.0f} KiB (WRITE_SIZE) per dispatch for 262144 KiB each way: scale factors "
         f"{f_scale:.3f} (fetch) and {w_scale:.3f} (write).", "",
         f"VALU issue fraction = VALU wave-instructions x {VALU_CYCLES} cycles / ({SIMDS} SIMDs x {CLOCK_HZ / 1e9:.1f} GHz x "
-        "average duration).  `bench.py` reads the block below only while the hash of the raymarch kernel's sources "
-        f"({', '.join(RAYMARCH_SOURCES)}) equals its `source_hash`.", "",
+        "average duration), the average over the dispatches that ran with no other kernel beside them (the "
+        "pipelined steps overlap a raymarch with the next volume build).  `bench.py` reads the block below only while the hash of the raymarch kernel's sources "
+        f"({', '.join(RAYMARCH_SOURCES)} and the device flags of vkhr_amd/Makefile) equals its `source_hash`.", "",
         "| kernel | avg us | VALU | SALU | VMEM rd | LDS | waves | FETCH KiB | WRITE KiB | HBM MB (scaled) |",
         "|---|---|---|---|---|---|---|---|---|---|"]
     for n in names:
@@ -684,39 +703,36 @@ def main():
     del idx_generated
     n_seg = idx.shape[0] // 2
     nvox = res[0] * res[1] * res[2]
-    sums = torch.empty(2 * nvox, dtype=torch.int64, device=dev)
     scene = synth.make_scene(bounds, W, H)
     cam, light = params.camera(scene), params.light(scene)
     sp, rp = params.strand(bounds, res), params.render(raycast_steps=float(S))
-    rgba = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+    rgbas = [torch.empty((H, W, 4), dtype=torch.float32, device=dev) for _ in range(2)]
+    rgba = rgbas[0]
+    # the volume build's workspace: read or written only if the brick bins
+    # overflow (they do not at this workload); one per buffer set
+    workspaces = [torch.empty(2 * nvox, dtype=torch.int64, device=dev) for _ in range(2)]
+    vols = [None, None]
 
-    def voxelize():
-        # the counts stay packed with the tangent sums; finalize unpacks them
-        return D.voxelize(xyz, bounds, res, indices=idx, tangents=tan, want_sums=True, sums=sums, want_counts=False)
-
-    state = {}
+    def build(k):
+        """voxelize_segments + normalize + cells in one call into buffer set k."""
+        vols[k] = D.build_volume(xyz, bounds, res, idx, tan, normalize=True, want_cells=True,
+                                 workspace=workspaces[k], out=vols[k])
+        return vols[k]
 
     def step():
-        voxelize()
-        vol = D.finalize(None, res, sums=sums, normalize=True, want_cells=True)
-        state["vol"] = vol
+        vol = build(0)
         D.raymarch(vol, res, cam, light, sp, rp, W, H, want_depth=True, out=rgba)
 
-    # Two steps in flight: step k runs on stream k % 2 with its own
-    # accumulator, volume and image, so step k+1's voxelization (atomic
-    # bound) and finalize (HBM bound) overlap step k's raymarch (VALU
-    # bound).  Both streams have the default priority.
+    # Two steps in flight: step k runs on stream k % 2 with its own volume
+    # and image, so step k+1's volume build (LDS atomics and HBM streams)
+    # overlaps step k's raymarch (VALU bound).  Both streams have the
+    # default priority; the library keeps a scratch area per stream.
     streams = [torch.cuda.Stream(), torch.cuda.Stream()]
-    sets = [dict(sums=sums, rgba=rgba),
-            dict(sums=torch.empty_like(sums), rgba=torch.empty_like(rgba))]
 
     def pipelined_step(k):
-        st = sets[k % 2]
         with torch.cuda.stream(streams[k % 2]):
-            D.voxelize(xyz, bounds, res, indices=idx, tangents=tan, want_sums=True, sums=st["sums"],
-                       want_counts=False)
-            st["vol"] = D.finalize(None, res, sums=st["sums"], normalize=True, want_cells=True)
-            D.raymarch(st["vol"], res, cam, light, sp, rp, W, H, want_depth=True, out=st["rgba"])
+            vol = build(k % 2)
+            D.raymarch(vol, res, cam, light, sp, rp, W, H, want_depth=True, out=rgbas[k % 2])
 
     for k in range(args.warmup):
         step()
@@ -734,7 +750,7 @@ def main():
     # every step's image equals the serial one
     step()
     torch.cuda.synchronize()
-    images_match = all(bool(torch.equal(st["rgba"], rgba)) for st in sets[1:])
+    images_match = bool(torch.equal(rgbas[1], rgba))
 
     # one step at a time (the latency of a step)
     torch.cuda.synchronize()
@@ -746,26 +762,36 @@ def main():
     t_serial = ev[0].elapsed_time(ev[1]) / args.steps * 1e-3
 
     # the raymarch kernel alone (events on the stream it is launched on)
-    vol = state["vol"]
+    vol = vols[0]
     ev[2].record()
     for _ in range(args.steps):
         D.raymarch(vol, res, cam, light, sp, rp, W, H, want_depth=True, out=rgba)
     ev[3].record()
-    # the voxelizer alone
+    # the volume build alone
     ev[4].record()
     for _ in range(args.steps):
-        voxelize()
+        build(1)
     ev[5].record()
-    # the finalize alone (unpack, u8 densities, tangents, min/max, normalize, cells)
-    ev_fin = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    ev_fin[0].record()
+    # the two-call path it replaces: voxelize_segments into the packed
+    # accumulator (zeroed first), then finalize (unpack, uint8 view,
+    # tangents, normalize, cells, distance field)
+    sums = workspaces[1]
+    ev_two = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev_two[0].record()
+    for _ in range(args.steps):
+        D.voxelize(xyz, bounds, res, indices=idx, tangents=tan, want_sums=True, sums=sums, want_counts=False)
+    ev_two[1].record()
     for _ in range(args.steps):
         D.finalize(None, res, sums=sums, normalize=True, want_cells=True)
-    ev_fin[1].record()
+    ev_two[2].record()
     torch.cuda.synchronize()
     t_ray = ev[2].elapsed_time(ev[3]) / args.steps * 1e-3
-    t_vox = ev[4].elapsed_time(ev[5]) / args.steps * 1e-3
-    t_fin = ev_fin[0].elapsed_time(ev_fin[1]) / args.steps * 1e-3
+    t_build = ev[4].elapsed_time(ev[5]) / args.steps * 1e-3
+    t_vox = ev_two[0].elapsed_time(ev_two[1]) / args.steps * 1e-3
+    t_fin = ev_two[1].elapsed_time(ev_two[2]) / args.steps * 1e-3
+    # the built volume equals the two-call path's
+    two = D.finalize(None, res, sums=sums, normalize=True, want_cells=True)
+    build_matches = all(bool(torch.equal(two[k], vols[1][k])) for k in ("density", "tangents", "cells", "occupancy"))
 
     out = D.raymarch(vol, res, cam, light, sp, rp, W, H, want_depth=True, want_info=True)
     info = out["info"].reshape(-1, 4).cpu().numpy()
@@ -777,24 +803,28 @@ def main():
     # 8 AO taps and one tangent tap
     ray_bytes = 8 * S * covered + 8 * (S + 8) * frags + 32 * frags + 16 * covered
     samples = int((sums.view(-1, 2)[:, 1] & 0xFFFFFFFF).sum().item())  # the packed counts
-    # every voxelizer sample is one line request (two 8-byte adds, paired lanes)
+    # every sample of the two-call voxelizer is one device-wide line request
+    # (two 8-byte adds, paired lanes); the build adds them in LDS instead
     peak = atomic_peak(dev)
     voxelize_atomics = {"requests_per_s": samples / t_vox, "peak_uniform_requests_per_s": peak["u32_uniform"],
                         "frac": samples / t_vox / peak["u32_uniform"],
                         "peak_lane_contiguous_atomics_per_s": peak["u32_stream"],
-                        "note": "voxelize_ms includes zeroing the 256 MiB accumulator"}
-    # algorithmic bytes of the volume build per segment stream: 12 B per vertex,
-    # 4 B per strand offset, and per voxel the 4-byte count clear, the
-    # finalize's 4-byte count read and its 1-byte density write
+                        "note": "the two-call path's voxelize_segments (voxelize_ms, including zeroing the 256 MiB "
+                                "accumulator); the one-call build issues no device-wide voxel atomics"}
+    # algorithmic bytes of the volume build: 12 B per vertex, 4 B per strand
+    # offset, and per voxel the 4-byte count clear, the finalize's 4-byte
+    # count read and its 1-byte density write
     nvert = int(xyz.shape[0])
     vox_bytes = 12 * nvert + 4 * (style.n_strands + 1) + 9 * nvox
-    vox_hbm = vox_bytes / (t_vox + t_fin) / 1e9
-    voxelize_roofline = {"algorithmic_bytes": vox_bytes, "seconds": t_vox + t_fin, "achieved_gbs": vox_hbm,
-                         "hbm_frac": vox_hbm / HBM_PEAK_GBS, "atomic_frac": voxelize_atomics["frac"],
-                         "binding_frac": max(vox_hbm / HBM_PEAK_GBS, voxelize_atomics["frac"]),
-                         "finalize_ms": t_fin * 1e3,
-                         "note": "bytes over voxelize + finalize; atomics over voxelize (one line request per "
-                                 "sample) against the measured uniform-random request peak"}
+    vox_hbm = vox_bytes / t_build / 1e9
+    voxelize_roofline = {"algorithmic_bytes": vox_bytes, "seconds": t_build, "achieved_gbs": vox_hbm,
+                         "hbm_frac": vox_hbm / HBM_PEAK_GBS, "build_ms": t_build * 1e3,
+                         "two_call_voxelize_ms": t_vox * 1e3, "two_call_finalize_ms": t_fin * 1e3,
+                         "build_speedup_vs_two_call": (t_vox + t_fin) / t_build,
+                         "build_matches_two_call": build_matches,
+                         "note": "vkhr_volume_build_segments (brick-binned, voxel sums in LDS; normalize, cells and "
+                                 "the brick distance field included) against the algorithmic bytes of the volume "
+                                 "build; the DRAM it moves is in the profile summary"}
 
     result = {
         "metric": METRIC,
@@ -819,7 +849,8 @@ def main():
                    "segments": n_seg, "voxel_samples": samples, "covered_rays": covered, "fragments": frags},
         "raymarch_mrays_per_s": covered / t_ray / 1e6,
         "raymarch_ms": t_ray * 1e3,
-        "segments_per_s": n_seg / t_vox,
+        "segments_per_s": n_seg / t_build,
+        "build_ms": t_build * 1e3,
         "voxelize_ms": t_vox * 1e3,
         "voxelize_atomics": voxelize_atomics,
         "voxelize_roofline": voxelize_roofline,

@@ -172,7 +172,8 @@ struct GridRay {
     uint32_t pos;
 };
 
-__device__ __forceinline__ GridRay grid_ray(const double xs[3], const double dx[3], double step, bool skip) {
+__device__ __forceinline__ GridRay grid_ray(const double xs[3], const double dx[3], double step, bool skip,
+                                            double margin = 1e-6) {
     GridRay g;
     g.pos = 0u;
 #pragma unroll
@@ -184,9 +185,9 @@ __device__ __forceinline__ GridRay grid_ray(const double xs[3], const double dx[
         if (skip && g.dxs[c] != 0.0) {
             const double inv = 1.0 / g.dxs[c];
             g.ea[c] = 8.0 * inv;
-            // the plane 1e-6 texels inside the region (float64 rounding of a
-            // sample's position cannot cross it)
-            g.eb[c] = (g.dxs[c] > 0.0 ? -1e-6 - g.x1s[c] : 1e-6 - g.x1s[c]) * inv;
+            // the plane `margin` texels inside the region (by default 1e-6:
+            // float64 rounding of a sample's position cannot cross it)
+            g.eb[c] = (g.dxs[c] > 0.0 ? -margin - g.x1s[c] : margin - g.x1s[c]) * inv;
             if (g.dxs[c] > 0.0) g.pos |= 1u << c;
         }
     }
@@ -208,6 +209,161 @@ __device__ __forceinline__ double region_exit(const GridRay& g, uint3 bk, int r)
         te = fmin(te, fma(double(k), g.ea[c], g.eb[c]));
     }
     return te;
+}
+
+// The power-of-two sample grid of the corner-packed cells in 32.32 fixed
+// point: sample i of a ray lies at texel coordinate + 1  P(i) = p0 + i * d
+// (per axis, in units of 2^-32 texel), with p0 and d the float64 x1s and
+// dxs of grid_ray rounded to 2^-32, so P(i) is within (i + 1) 2^-33 texel
+// of the float64 position (6e-8 texel at 512 steps).  The cell of a sample
+// is the high word of P and its trilinear weight the low word, and the
+// ranges of sample indices -- the volume's support, the extent of an empty
+// brick region -- are exact integer solutions, so a march over them needs
+// no per-sample range test and jumps exactly to the first sample past a
+// region.  Used while |x1s| < 2^17 and |dxs| * steps < 2^18 texels (then
+// every P and every j * d below stays under 2^52, exact in float64 too).
+struct FixedRay {
+    long long p0[3], d[3];
+};
+
+constexpr double kFixedOne = 4294967296.0;  // 2^32
+
+__device__ __forceinline__ bool fixed_ray_fits(const double x1s[3], const double dxs[3], double steps) {
+    bool fits = true;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) fits = fits && fabs(x1s[c]) < 131072.0 && fabs(dxs[c]) * steps < 262144.0;
+    return fits;
+}
+
+__device__ __forceinline__ FixedRay fixed_ray(const double x1s[3], const double dxs[3]) {
+    FixedRay r;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        r.p0[c] = llrint(x1s[c] * kFixedOne);   // the products are exact (power-of-two scale)
+        r.d[c] = llrint(dxs[c] * kFixedOne);
+    }
+    return r;
+}
+
+// First j in [0, cap] with p + j * d >= k, for d > 0 (cap when no j below
+// cap qualifies).  The float64 quotient is within an ulp of the exact one,
+// so its ceiling is at most one off, and one integer check each way fixes it.
+__device__ __forceinline__ int fixed_first_ge(long long p, long long d, long long k, int cap) {
+    if (p >= k) return 0;
+    const double q = double(k - p) / double(d);
+    if (!(q < double(cap))) return cap;
+    long long j = (long long)ceil(q);
+    if (j > 0 && p + (j - 1) * d >= k)
+        --j;
+    else if (p + j * d < k)
+        ++j;
+    return int(j < cap ? j : cap);
+}
+
+// The samples [lo, hi) of a ray (indices below steps) whose cells lie in
+// the volume's support, cell indices 0..n along every axis, P in
+// [0, (n + 1) 2^32); lo >= hi when there are none.
+__device__ __forceinline__ void fixed_support(const FixedRay& r, const int n[3], int steps, int* lo, int* hi) {
+    int l = 0, h = steps;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const long long top = (long long)(n[c] + 1) << 32;
+        const long long p = r.p0[c], d = r.d[c];
+        int in = 0, out = steps;
+        if (d > 0) {
+            in = fixed_first_ge(p, d, 0, steps);
+            out = fixed_first_ge(p, d, top, steps);
+        } else if (d < 0) {
+            in = fixed_first_ge(-p, -d, -top + 1, steps);
+            out = fixed_first_ge(-p, -d, 1, steps);
+        } else if (!(p >= 0 && p < top)) {
+            in = steps;
+        }
+        l = max(l, in);
+        h = min(h, out);
+    }
+    *lo = l;
+    *hi = h;
+}
+
+// The plane margin of the region exits on the fixed-point grid: the
+// float64 planes of grid_ray taken far enough inside each region that the
+// fixed-point positions, within (steps + 1) 2^-33 texel of the float64
+// ones, cannot cross them either.
+__device__ __forceinline__ double fixed_exit_margin(int steps) {
+    return 1e-6 + double(steps + 1) * 1.1641532182693481e-10;  // 2^-33
+}
+
+// One trilinear tap of the cells at cell (cx, cy, cz) (texel + 1, inside
+// the support) with fractions (fx, fy, fz) in units of 2^-32: the blend of
+// march_tap.
+__device__ __forceinline__ float fixed_cells_tap(const DensityVolume& v, uint32_t cx, uint32_t cy, uint32_t cz,
+                                                 uint32_t fx, uint32_t fy, uint32_t fz) {
+    const uint32_t c = cx + __umul24(uint32_t(v.nx) + 1u, cy + __umul24(uint32_t(v.ny) + 1u, cz));
+    const unsigned long long q = v.cells[c];
+    const uint32_t lo = uint32_t(q), hi = uint32_t(q >> 32);
+    const float b0 = float(lo & 0xffu), b1 = float((lo >> 8) & 0xffu);
+    const float b2 = float((lo >> 16) & 0xffu), b3 = float(lo >> 24);
+    const float b4 = float(hi & 0xffu), b5 = float((hi >> 8) & 0xffu);
+    const float b6 = float((hi >> 16) & 0xffu), b7 = float(hi >> 24);
+    constexpr float kInv = 2.3283064365386963e-10f;  // 2^-32
+    const float wx = float(fx) * kInv, wy = float(fy) * kInv, wz = float(fz) * kInv;
+    const float c00 = fmaf(wx, b1 - b0, b0);
+    const float c10 = fmaf(wx, b3 - b2, b2);
+    const float c01 = fmaf(wx, b5 - b4, b4);
+    const float c11 = fmaf(wx, b7 - b6, b6);
+    const float c0 = fmaf(wy, c10 - c00, c00);
+    const float c1 = fmaf(wy, c11 - c01, c01);
+    return fmaf(wz, c1 - c0, c0) * (1.0f / 255.0f);
+}
+
+// The distance nibble of the brick of cell (cx, cy, cz) (0 without the
+// staged field: nothing is skipped).
+__device__ __forceinline__ uint32_t fixed_brick_distance(const DensityVolume& v, const OccupancyView& occ,
+                                                         uint32_t cx, uint32_t cy, uint32_t cz) {
+    if (!occ.use) return 0u;
+    const uint32_t b = (cx >> kBrickShift) +
+                       __umul24(v.bx, (cy >> kBrickShift) + __umul24(v.by, cz >> kBrickShift));
+    return brick_distance(occ, b);
+}
+
+// The ADSM march of the cells over the fixed-point grid: the sum of the
+// taps of samples [0, steps) from texel coordinate + 1 x1s by dxs per
+// sample, jumping over empty brick regions.
+__device__ __forceinline__ double fixed_shadow_sum(const DensityVolume& v, const OccupancyView& occ,
+                                                   const double x1s[3], const double dxs[3], int steps) {
+    const FixedRay r = fixed_ray(x1s, dxs);
+    const int n[3] = {v.nx, v.ny, v.nz};
+    int j, j_end;
+    fixed_support(r, n, steps, &j, &j_end);
+    // region exits on the float64 grid of the same samples (x1s - 1 + j dxs)
+    const double xs[3] = {x1s[0] - 1.0, x1s[1] - 1.0, x1s[2] - 1.0};
+    const GridRay g = grid_ray(xs, dxs, 1.0, occ.use, fixed_exit_margin(steps));
+    long long P[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) P[c] = r.p0[c] + (long long)j * r.d[c];
+    double sum = 0.0;
+    while (j < j_end) {
+        const uint32_t cx = uint32_t(P[0] >> 32), cy = uint32_t(P[1] >> 32), cz = uint32_t(P[2] >> 32);
+        const uint32_t nib = fixed_brick_distance(v, occ, cx, cy, cz);
+        if (nib == 0u) {
+            sum += double(fixed_cells_tap(v, cx, cy, cz, uint32_t(P[0]), uint32_t(P[1]), uint32_t(P[2])));
+            P[0] += r.d[0];
+            P[1] += r.d[1];
+            P[2] += r.d[2];
+            ++j;
+        } else {
+            // the last sample inside the region (planes taken inside by the
+            // margin), then the next one
+            const uint3 bk = make_uint3(cx >> kBrickShift, cy >> kBrickShift, cz >> kBrickShift);
+            const int next = max(j + 1, int(floor(fmin(region_exit(g, bk, int(nib) - 1), double(j_end)))) + 1);
+            const long long k = next - j;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) P[c] += k * r.d[c];
+            j = next;
+        }
+    }
+    return sum;
 }
 
 // First grid index i >= 0 with i / steps >= t (steps when there is none
@@ -277,7 +433,91 @@ __device__ __forceinline__ void raymarch_pixel(const RaymarchArgs& a, int px, in
         const double iso = a.iso;
         const float step = a.step_size;
         int i = 0;
-        if (a.steps_per_unit > 0.0f) {
+        bool fixed = false;
+        if constexpr (!kF32) {
+            if (a.steps_per_unit > 0.0f) {
+                double x1s[3], dxs[3];
+                const double st = double(step);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    x1s[c] = xs[c] + 1.0;
+                    dxs[c] = dx[c] * st;
+                }
+                fixed = fixed_ray_fits(x1s, dxs, double(a.steps_per_unit));
+                if (fixed) {
+                    // The fixed-point grid: the samples [i_lo, i_end) in the
+                    // volume's support, gathered without range tests; one in
+                    // an empty brick region jumps to the first sample past
+                    // the region, but not past i_depth, from where the depth
+                    // test runs on every sample (as on the float64 grid).
+                    const int steps = int(a.steps_per_unit);
+                    const FixedRay r = fixed_ray(x1s, dxs);
+                    const int n[3] = {vol.nx, vol.ny, vol.nz};
+                    int i_lo, i_end;
+                    fixed_support(r, n, steps, &i_lo, &i_end);
+                    const GridRay g = grid_ray(xs, dx, st, occ.use, fixed_exit_margin(steps));
+                    const int i_depth = grid_first(t_depth, double(steps));
+                    const double dzs = dz * st, dws = dw * st;
+                    bool behind = false;
+                    // the depth test over samples before the support (they read 0)
+                    for (i = i_depth; i < i_lo; ++i) {
+                        const double di = double(i);
+                        if (db * fma(di, dws, ws) < fma(di, dzs, zs)) {
+                            behind = true;
+                            break;
+                        }
+                    }
+                    i = i_lo;
+                    long long P[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) P[c] = r.p0[c] + (long long)i * r.d[c];
+                    while (!behind && i < i_end) {
+                        if (i >= i_depth) {
+                            const double di = double(i);
+                            // depth test: db < z/w  <=>  db*w < z  (w > 0 in front of the eye)
+                            if (db * fma(di, dws, ws) < fma(di, dzs, zs)) break;
+                        }
+                        const uint32_t cx = uint32_t(P[0] >> 32), cy = uint32_t(P[1] >> 32),
+                                       cz = uint32_t(P[2] >> 32);
+                        const uint32_t nib = fixed_brick_distance(vol, occ, cx, cy, cz);
+                        float d = 0.0f;
+                        int next = i + 1;
+                        if (nib == 0u) {
+                            d = fixed_cells_tap(vol, cx, cy, cz, uint32_t(P[0]), uint32_t(P[1]), uint32_t(P[2]));
+                            P[0] += r.d[0];
+                            P[1] += r.d[1];
+                            P[2] += r.d[2];
+                        } else {
+                            if (next < i_depth) {
+                                // the last sample inside the region (planes taken
+                                // inside by the margin), then the next one
+                                const uint3 bk = make_uint3(cx >> kBrickShift, cy >> kBrickShift, cz >> kBrickShift);
+                                const int j = int(floor(fmin(region_exit(g, bk, int(nib) - 1), double(i_end))));
+                                next = max(next, min(j + 1, i_depth));
+                            }
+                            const long long k = next - i;
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) P[c] += k * r.d[c];
+                        }
+                        acc += double(d);
+                        // surface / entry bookkeeping as selects
+                        const bool nonzero = d != 0.0f;
+                        const bool below = nonzero & (acc <= iso);
+                        surf = below ? i : surf;
+                        found = found | (nonzero & (acc >= iso));
+                        entry = (nonzero & (entry < 0)) ? i : entry;
+                        i = next;
+                    }
+                    t_surf = float(surf) * step;
+                    t_entry = float(entry) * step;
+                }
+            }
+        }
+        // the cells path takes the float64 grid only where the fixed-point
+        // one does not fit, and then the shader's loop below (same samples)
+        if (fixed) {
+            // marched above
+        } else if (kF32 && a.steps_per_unit > 0.0f) {
             // The loop parameter is exactly i * step (a power-of-two step
             // count): the march runs over integer sample indices, bounded by
             // the support interval, with the depth test from the first index
@@ -383,7 +623,22 @@ __device__ __forceinline__ void raymarch_pixel(const RaymarchArgs& a, int px, in
                 support_interval(a, sx, xl, &t_lo, &t_hi);
                 const DensityVolume vol = a.vol;
                 const float step = a.step_size;
-                if (a.steps_per_unit > 0.0f) {
+                bool adsm_fixed = false;
+                if constexpr (!kF32) {
+                    if (a.steps_per_unit > 0.0f) {
+                        double x1s[3], dxs[3];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            x1s[c] = sx[c] + 1.0;
+                            dxs[c] = xl[c] * double(step);
+                        }
+                        adsm_fixed = fixed_ray_fits(x1s, dxs, double(a.steps_per_unit));
+                        if (adsm_fixed) strands = fixed_shadow_sum(vol, occ, x1s, dxs, int(a.steps_per_unit));
+                    }
+                }
+                if (adsm_fixed) {
+                    // summed on the fixed-point grid above
+                } else if (kF32 && a.steps_per_unit > 0.0f) {
                     // over the sample grid from the support's start, jumping
                     // over empty brick regions (their samples read 0)
                     const double steps = a.steps_per_unit;
