@@ -89,6 +89,20 @@ def test_without_tangent_volume(device):
     _compare(got, _two_call(xyz, b, res, idx, tan, True, want_tangents=False), "no tangents")
 
 
+@pytest.mark.parametrize("normalize", [True, False])
+def test_without_cells(device, normalize):
+    """No cells: the brick pass writes the uint8 view into the output and
+    the normalize runs in place, as in vkhr_volume_finalize."""
+    s = synth.hemisphere_style(4000, 16, seed=12)
+    res = (64, 48, 80)
+    xyz, idx, tan, b = _style_inputs(s, device)
+    got = D.build_volume(xyz, b, res, idx, tan, normalize=normalize, want_counts=True, want_f32=True,
+                         want_cells=False)
+    want = _two_call(xyz, b, res, idx, tan, normalize)
+    want["cells"] = want["occupancy"] = None
+    _compare(got, want, f"no cells, normalize={normalize}")
+
+
 def _brick_runs(xyz, idx, b, res):
     """Bricks entered per pair: the float32 walk of hair_style.cc:311-329,
     vectorized over the pairs (numpy float32, no contraction)."""

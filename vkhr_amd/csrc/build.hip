@@ -5,17 +5,23 @@
 // LDS instead of device-wide atomics.
 //
 // Bricks of 8x8x8 voxels.  The index pairs are binned by the bricks their
-// samples visit -- a counting sort: count (one add per brick a pair visits,
-// aggregated over the lanes of a wave), scan, fill -- then one workgroup
+// samples visit -- a counting sort: count, scan, fill -- then one workgroup
 // per brick walks the DDA of every pair in its bin and adds the samples
 // that fall inside its brick into an LDS accumulator (count and fixed-point
 // tangent sums: integer adds, so the order of the adds does not matter).
 // The workgroup finalizes its brick straight from LDS: the saturating uint8
 // view, the quantized tangent (the sums and the expression of
 // vkhr_volume_finalize, so both give the same bits), the counts when asked
-// for, and the min/max of the uint8 view for the normalize.  The normalize,
-// the float view, the corner-packed cells and the brick distance field
-// follow as in vkhr_volume_finalize.
+// for, and the min/max of the uint8 view.  The corner-packed cells pass
+// then applies the normalize as it loads the uint8 view (writing the
+// normalized volume and the float view as it goes) and sets the brick
+// occupancy; the brick distance field follows as in vkhr_volume_finalize.
+//
+// The count and fill passes give each workgroup a contiguous range of
+// pairs and count the runs of its range in an LDS hash table first, so a
+// brick takes one device-wide atomic per workgroup that reaches it rather
+// than one per run (keys that find no free slot fall back to a device-wide
+// atomic per run).
 //
 // A pair's voxel coordinates are monotone in its sample index (the root is
 // advanced by a fixed direction and the clamps are monotone), so the bricks
@@ -23,11 +29,11 @@
 //
 // The bin entries live in the per-stream scratch, 4 per pair plus 64 Ki.
 // If the pairs visit more bricks than that (long segments), a flag set on
-// the device switches the build to the workspace path: the packed
-// accumulator of vkhr_voxelize_segments in the caller's workspace,
-// device-wide atomics, then the finalize of vkhr_volume_finalize.  Its
-// kernels are always enqueued and return at once while the flag is clear,
-// so the host never waits for the count.
+// the device switches the build to the workspace path: the fill pass zeroes
+// the caller's workspace instead, the brick pass voxelizes into it with the
+// packed device-wide atomics of vkhr_voxelize_segments, and one more kernel
+// (which returns at once while the flag is clear) finalizes from it, so the
+// host never waits for the count.
 //
 // Compiled with -ffp-contract=off: the DDA and the quantization must round
 // as the reference's float arithmetic does.
@@ -61,19 +67,21 @@ struct BuildLayout {
     size_t cursor;      // bin starts, then (after the fill) bin ends
     size_t status;      // [0] total entries, [1] overflow flag
     size_t minmax;      // the min/max slots (kMinMaxBytes)
-    size_t entries;     // bin entries: pair indices
+    size_t raw;         // the uint8 view before the normalize (with cells), n voxels
+    size_t entries;     // bin entries: the pairs' vertex indices (uint2)
     size_t words;
 };
 
-__host__ __forceinline__ BuildLayout build_layout(size_t n_bricks, size_t capacity) {
+__host__ __forceinline__ BuildLayout build_layout(size_t n_bricks, size_t n_voxels, bool raw, size_t capacity) {
     auto up = [](size_t w) { return (w + 63) & ~size_t(63); };  // 256-byte aligned sections
     BuildLayout l;
     l.bins = 0;
     l.cursor = up(n_bricks);
     l.status = l.cursor + up(n_bricks);
     l.minmax = l.status + 64;
-    l.entries = l.minmax + up(kMinMaxBytes / 4);
-    l.words = l.entries + up(capacity);
+    l.raw = l.minmax + up(kMinMaxBytes / 4);
+    l.entries = l.raw + (raw ? up((n_voxels + 3) / 4) : 0);
+    l.words = l.entries + up(2 * capacity);
     return l;
 }
 
@@ -84,99 +92,186 @@ struct BuildArgs {
     const uint32_t* indices;
     const float* tangents;          // per vertex (null: no tangent volume)
     uint32_t n_vertices, n_pairs;
+    uint32_t n_bricks;
+    uint32_t pairs_per_block;       // the contiguous range of a bin workgroup
     uint32_t* bins;
     uint32_t* cursor;
     uint32_t* status;
     uint32_t* minmax;
-    uint32_t* entries;
+    uint2* entries;
     uint32_t capacity;
+    uint32_t* occupancy;            // zeroed by the init pass (nullable)
+    size_t occupancy_words;
     uint32_t* counts;               // outputs (counts and tangents nullable)
-    uint8_t* density;
+    uint8_t* density;               // the uint8 view the brick pass writes (raw, or the output)
     int8_t* tangent_i8x4;
     long long* acc;                 // workspace of the overflow path
 };
 
-// The bricks a wave's lanes enter in this step (kNone for none), one
-// atomic per distinct brick.  Every lane of the wave calls it.
-__device__ __forceinline__ void count_runs(uint32_t key, uint32_t* bins) {
-    const int lane = __lane_id();
-    unsigned long long pending = __ballot(key != kNone);
-    while (pending) {
-        const int leader = __ffsll(pending) - 1;
-        const uint32_t k = __shfl(key, leader, kWave);
-        const unsigned long long same = __ballot(key == k);
-        if (lane == leader) atomicAdd(&bins[k], uint32_t(__popcll(same)));
-        pending &= ~same;
-    }
-}
-
-// As count_runs, taking a slot in each entered brick's bin for pair s.
-__device__ __forceinline__ void fill_runs(uint32_t key, uint32_t s, uint32_t* cursor, uint32_t* entries,
-                                          uint32_t capacity) {
-    const int lane = __lane_id();
-    unsigned long long pending = __ballot(key != kNone);
-    while (pending) {
-        const int leader = __ffsll(pending) - 1;
-        const uint32_t k = __shfl(key, leader, kWave);
-        const unsigned long long same = __ballot(key == k);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&cursor[k], uint32_t(__popcll(same)));
-        base = __shfl(base, leader, kWave);
-        if (key == k) {
-            const uint32_t slot = base + uint32_t(__popcll(same & ((1ull << lane) - 1ull)));
-            if (slot < capacity) entries[slot] = s;
+// Zeroes the bin sizes, the status and the occupancy, and empties the
+// min/max slots.
+__global__ void __launch_bounds__(256) build_init_kernel(BuildArgs a) {
+    const size_t stride = size_t(gridDim.x) * blockDim.x;
+    const size_t first = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (size_t i = first; i < a.n_bricks; i += stride) a.bins[i] = 0u;
+    for (size_t i = first; i < a.occupancy_words; i += stride) a.occupancy[i] = 0u;
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < 2) a.status[threadIdx.x] = 0u;
+        if (threadIdx.x < kMinMaxSlots) {
+            a.minmax[threadIdx.x * kMinMaxStride] = 255u;
+            a.minmax[threadIdx.x * kMinMaxStride + 1] = 0u;
         }
-        pending &= ~same;
     }
 }
 
-// Walks the DDA of every pair (one per lane, waves stepping together) and
-// reports each brick run: to the bin sizes (kFill = false) or as a bin
-// entry (kFill = true).
-template <bool kFill>
-__global__ void __launch_bounds__(kBuildBlock) bin_kernel(BuildArgs a) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~uint32_t(kWave - 1)); base < a.n_pairs;
-         base += stride) {
-        const uint32_t s = base + (threadIdx.x & uint32_t(kWave - 1));
-        SegmentDda w = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        if (s < a.n_pairs) w = segment_dda(a.g, a.xyz, a.n_vertices, a.indices[2u * s], a.indices[2u * s + 1u]);
+// --- the bin passes
+
+// An LDS hash table of the bricks one workgroup's pairs reach: open
+// addressing over kTableSlots slots, kTableProbes probes.  A key that finds
+// no free slot among its probes never will (slots are only ever filled),
+// so every lookup of it fails alike and its runs take the device-wide path.
+constexpr uint32_t kTableBits = 10, kTableSlots = 1u << kTableBits, kTableProbes = 16;
+
+struct BinTable {
+    uint32_t key[kTableSlots];
+    uint32_t count[kTableSlots];
+    uint32_t base[kTableSlots];
+};
+
+__device__ __forceinline__ uint32_t table_home(uint32_t k) { return (k * 2654435761u) >> (32 - kTableBits); }
+
+__device__ __forceinline__ uint32_t table_insert(BinTable& t, uint32_t k) {
+    const uint32_t h = table_home(k);
+    for (uint32_t p = 0; p < kTableProbes; ++p) {
+        const uint32_t slot = (h + p) & (kTableSlots - 1u);
+        const uint32_t prev = atomicCAS(&t.key[slot], kNone, k);
+        if (prev == kNone || prev == k) return slot;
+    }
+    return kNone;
+}
+
+__device__ __forceinline__ uint32_t table_find(const BinTable& t, uint32_t k) {
+    const uint32_t h = table_home(k);
+    for (uint32_t p = 0; p < kTableProbes; ++p) {
+        const uint32_t slot = (h + p) & (kTableSlots - 1u);
+        const uint32_t key = t.key[slot];
+        if (key == k) return slot;
+        if (key == kNone) return kNone;
+    }
+    return kNone;
+}
+
+__device__ __forceinline__ void table_clear(BinTable& t) {
+    for (uint32_t i = threadIdx.x; i < kTableSlots; i += blockDim.x) {
+        t.key[i] = kNone;
+        t.count[i] = 0u;
+    }
+}
+
+// Calls run(pair's vertex indices, brick) for each brick run of each pair
+// of this workgroup's range (at most kPairsPerThread per thread: the index
+// loads of all of a thread's pairs are issued before the first walk).
+constexpr uint32_t kPairsPerThread = 4;
+
+template <class Run>
+__device__ __forceinline__ void walk_runs(const BuildArgs& a, Run run) {
+    const uint32_t r0 = blockIdx.x * a.pairs_per_block;
+    const uint32_t r1 = min(a.n_pairs, r0 + a.pairs_per_block);
+    uint2 pr[kPairsPerThread];
+#pragma unroll
+    for (uint32_t j = 0; j < kPairsPerThread; ++j) {
+        const uint32_t s = r0 + threadIdx.x + j * blockDim.x;
+        pr[j] = s < r1 ? make_uint2(a.indices[2u * s], a.indices[2u * s + 1u]) : make_uint2(kNone, kNone);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kPairsPerThread; ++j) {
+        SegmentDda w = segment_dda(a.g, a.xyz, a.n_vertices, pr[j].x, pr[j].y);  // kNone: no samples
         uint32_t last = kNone;
-        while (__ballot(w.steps > 0.0f)) {
-            uint32_t key = kNone;
-            if (w.steps > 0.0f) {
-                w.steps -= 1.0f;
-                const uint32_t k = brick_index(a.b, voxel_coords(a.g, w.rx, w.ry, w.rz));
-                key = k != last ? k : kNone;
-                last = k;
-                w.advance();
-            }
-            if constexpr (kFill)
-                fill_runs(key, s, a.cursor, a.entries, a.capacity);
-            else
-                count_runs(key, a.bins);
+        while (w.steps > 0.0f) {
+            w.steps -= 1.0f;
+            const uint32_t k = brick_index(a.b, voxel_coords(a.g, w.rx, w.ry, w.rz));
+            if (k != last) run(pr[j], k);
+            last = k;
+            w.advance();
         }
     }
 }
 
-// Exclusive scan of the bin sizes into the bin starts (one block), the
-// total, and the overflow flag.
-__global__ void __launch_bounds__(1024) scan_kernel(BuildArgs a, uint32_t n_bricks) {
-    constexpr uint32_t kPer = 4, kThreads = 1024;
+// The bin sizes: runs counted per brick in the table, then one add per
+// table entry.
+__global__ void __launch_bounds__(kBuildBlock) bin_count_kernel(BuildArgs a) {
+    __shared__ BinTable t;
+    table_clear(t);
+    __syncthreads();
+    walk_runs(a, [&](uint2, uint32_t k) {
+        const uint32_t slot = table_insert(t, k);
+        if (slot != kNone)
+            atomicAdd(&t.count[slot], 1u);
+        else
+            atomicAdd(&a.bins[k], 1u);
+    });
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kTableSlots; i += blockDim.x)
+        if (t.key[i] != kNone) atomicAdd(&a.bins[t.key[i]], t.count[i]);
+}
+
+// The bin entries: the same counts in the table, one range reserved per
+// table entry in its brick's bin, then a second walk that places each run
+// in its range (table misses take a slot of their own).  With the overflow
+// flag set the workgroups zero the workspace instead.
+__global__ void __launch_bounds__(kBuildBlock) bin_fill_kernel(BuildArgs a, size_t workspace_words) {
+    __shared__ BinTable t;
+    if (a.status[1]) {
+        uint32_t* acc = reinterpret_cast<uint32_t*>(a.acc);
+        const size_t stride = size_t(gridDim.x) * blockDim.x;
+        for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < workspace_words; i += stride) acc[i] = 0u;
+        return;
+    }
+    table_clear(t);
+    __syncthreads();
+    walk_runs(a, [&](uint2, uint32_t k) {
+        const uint32_t slot = table_insert(t, k);
+        if (slot != kNone) atomicAdd(&t.count[slot], 1u);
+    });
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kTableSlots; i += blockDim.x) {
+        if (t.key[i] != kNone) t.base[i] = atomicAdd(&a.cursor[t.key[i]], t.count[i]);
+        t.count[i] = 0u;
+    }
+    __syncthreads();
+    walk_runs(a, [&](uint2 pair, uint32_t k) {
+        const uint32_t slot = table_find(t, k);
+        const uint32_t pos = slot != kNone ? t.base[slot] + atomicAdd(&t.count[slot], 1u) : atomicAdd(&a.cursor[k], 1u);
+        if (pos < a.capacity) a.entries[pos] = pair;
+    });
+}
+
+// Exclusive scan of the bin sizes into the bin starts (one block, 16 sizes
+// per thread per round), the total, and the overflow flag.
+__global__ void __launch_bounds__(1024) scan_kernel(BuildArgs a) {
+    constexpr uint32_t kPer = 16, kThreads = 1024;
     __shared__ unsigned long long wave_sums[kThreads / kWave];
     __shared__ unsigned long long carry_lds;  // 64-bit: the total of absurd inputs may pass 2^32
     const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const uint32_t n = a.n_bricks;
     if (threadIdx.x == 0) carry_lds = 0;
     __syncthreads();
-    for (uint32_t chunk = 0; chunk < n_bricks; chunk += kPer * kThreads) {
+    for (uint32_t chunk = 0; chunk < n; chunk += kPer * kThreads) {
+        const uint32_t i0 = chunk + kPer * threadIdx.x;
         uint32_t v[kPer];
+        if (i0 + kPer <= n) {  // the sections are 256-byte aligned and i0 a multiple of 16
+#pragma unroll
+            for (uint32_t q = 0; q < kPer / 4; ++q) {
+                const uint4 u = reinterpret_cast<const uint4*>(a.bins + i0)[q];
+                v[4 * q] = u.x; v[4 * q + 1] = u.y; v[4 * q + 2] = u.z; v[4 * q + 3] = u.w;
+            }
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < kPer; ++j) v[j] = i0 + j < n ? a.bins[i0 + j] : 0u;
+        }
         unsigned long long sum = 0;
 #pragma unroll
-        for (uint32_t j = 0; j < kPer; ++j) {
-            const uint32_t i = chunk + kPer * threadIdx.x + j;
-            v[j] = i < n_bricks ? a.bins[i] : 0u;
-            sum += v[j];
-        }
+        for (uint32_t j = 0; j < kPer; ++j) sum += v[j];
         // inclusive wave scan of the per-thread sums
         unsigned long long inc = sum;
         for (int o = 1; o < kWave; o <<= 1) {
@@ -188,13 +283,23 @@ __global__ void __launch_bounds__(1024) scan_kernel(BuildArgs a, uint32_t n_bric
         unsigned long long before = carry_lds;
         for (uint32_t w = 0; w < wave; ++w) before += wave_sums[w];
         before += inc - sum;
+        // a wrapped start only happens with the overflow flag set, and then
+        // no bin is read
+        uint32_t out[kPer];
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) {
-            const uint32_t i = chunk + kPer * threadIdx.x + j;
-            // a wrapped start only happens with the overflow flag set, and
-            // then no bin is read
-            if (i < n_bricks) a.cursor[i] = uint32_t(before);
+            out[j] = uint32_t(before);
             before += v[j];
+        }
+        if (i0 + kPer <= n) {
+#pragma unroll
+            for (uint32_t q = 0; q < kPer / 4; ++q)
+                reinterpret_cast<uint4*>(a.cursor + i0)[q] =
+                    make_uint4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < kPer; ++j)
+                if (i0 + j < n) a.cursor[i0 + j] = out[j];
         }
         __syncthreads();
         if (threadIdx.x == kThreads - 1) carry_lds = before;
@@ -227,15 +332,35 @@ __host__ __forceinline__ uint32_t brick_blocks(const BrickGrid& b) {
 }
 
 // One brick per workgroup: the samples of its bin's pairs that fall inside
-// it, accumulated in LDS, then the brick's outputs.
-__global__ void __launch_bounds__(kBuildBlock) brick_kernel(BuildArgs a) {
+// it, accumulated in LDS, then the brick's outputs.  The tangent sums are
+// 32-bit: at 2^16 per unit they hold up to 32767 contributions exactly,
+// and a voxel of more gets tangent 0 (as from the packed accumulator), so
+// the quantized tangents are those of vkhr_volume_finalize.  With the
+// overflow flag set the workgroups voxelize every pair into the workspace
+// instead.
+constexpr uint32_t kBrickBlock = 128;
+
+__global__ void __launch_bounds__(kBrickBlock) brick_kernel(BuildArgs a) {
     constexpr uint32_t kVox = 512;
     __shared__ uint32_t count[kVox];
-    __shared__ long long sum[3][kVox];
-    __shared__ uint32_t red[2][kBuildBlock / kWave];
+    __shared__ int sum[3][kVox];
+    __shared__ uint32_t red[2][kBrickBlock / kWave];
+    if (a.status[1]) {
+        // grid-stride over wave-aligned ranges (voxelize_segment is wave-cooperative)
+        const float* tangents = a.tangent_i8x4 ? a.tangents : a.xyz;  // sums unused without a tangent volume
+        const uint32_t stride = gridDim.x * blockDim.x;
+        for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~uint32_t(kWave - 1)); base < a.n_pairs;
+             base += stride) {
+            const uint32_t s = base + (threadIdx.x & uint32_t(kWave - 1));
+            const bool has = s < a.n_pairs;
+            voxelize_segment(a.g, a.xyz, a.n_vertices, has, has ? a.indices[2u * s] : 0u,
+                             has ? a.indices[2u * s + 1u] : 0u, tangents, nullptr, a.acc);
+        }
+        return;
+    }
     const uint32_t brick = brick_of_block(a.b, blockIdx.x);
-    if (brick == kNone || a.status[1]) return;  // the overflow path writes the outputs
-    for (uint32_t l = threadIdx.x; l < kVox; l += kBuildBlock) {
+    if (brick == kNone) return;
+    for (uint32_t l = threadIdx.x; l < kVox; l += kBrickBlock) {
         count[l] = 0u;
         sum[0][l] = sum[1][l] = sum[2][l] = 0;
     }
@@ -243,16 +368,15 @@ __global__ void __launch_bounds__(kBuildBlock) brick_kernel(BuildArgs a) {
     __syncthreads();
     const uint32_t end = a.cursor[brick], n = a.bins[brick];
     const bool with_tangents = a.tangent_i8x4 != nullptr;
-    for (uint32_t e = end - n + threadIdx.x; e < end; e += kBuildBlock) {
-        const uint32_t s = a.entries[e];
-        const uint32_t ia = a.indices[2u * s], ib = a.indices[2u * s + 1u];
-        SegmentDda w = segment_dda(a.g, a.xyz, a.n_vertices, ia, ib);
-        long long tx = 0, ty = 0, tz = 0;
+    for (uint32_t e = end - n + threadIdx.x; e < end; e += kBrickBlock) {
+        const uint2 pair = a.entries[e];
+        SegmentDda w = segment_dda(a.g, a.xyz, a.n_vertices, pair.x, pair.y);
+        int tx = 0, ty = 0, tz = 0;
         if (with_tangents && w.steps > 0.0f) {
-            const float3 t = load_xyz(a.tangents, ia);
-            tx = tangent_fixed(t.x);
-            ty = tangent_fixed(t.y);
-            tz = tangent_fixed(t.z);
+            const float3 t = load_xyz(a.tangents, pair.x);
+            tx = int(tangent_fixed(t.x));
+            ty = int(tangent_fixed(t.y));
+            tz = int(tangent_fixed(t.z));
         }
         bool entered = false;
         while (w.steps > 0.0f) {
@@ -262,10 +386,10 @@ __global__ void __launch_bounds__(kBuildBlock) brick_kernel(BuildArgs a) {
             if (inside) {
                 const uint32_t l = (v.x & 7u) + 8u * ((v.y & 7u) + 8u * (v.z & 7u));
                 atomicAdd(&count[l], 1u);
-                if (with_tangents) {
-                    atomicAdd(reinterpret_cast<unsigned long long*>(&sum[0][l]), (unsigned long long)tx);
-                    atomicAdd(reinterpret_cast<unsigned long long*>(&sum[1][l]), (unsigned long long)ty);
-                    atomicAdd(reinterpret_cast<unsigned long long*>(&sum[2][l]), (unsigned long long)tz);
+                if (with_tangents) {  // wraps only past 32767 contributions, where the tangent is 0
+                    atomicAdd(reinterpret_cast<unsigned*>(&sum[0][l]), unsigned(tx));
+                    atomicAdd(reinterpret_cast<unsigned*>(&sum[1][l]), unsigned(ty));
+                    atomicAdd(reinterpret_cast<unsigned*>(&sum[2][l]), unsigned(tz));
                 }
             } else if (entered) {
                 break;  // the brick run is over: the pair never comes back
@@ -276,7 +400,7 @@ __global__ void __launch_bounds__(kBuildBlock) brick_kernel(BuildArgs a) {
     }
     __syncthreads();
     uint32_t lo = 255u, hi = 0u;
-    for (uint32_t l = threadIdx.x; l < kVox; l += kBuildBlock) {
+    for (uint32_t l = threadIdx.x; l < kVox; l += kBrickBlock) {
         const uint32_t x = bxo * 8u + (l & 7u), y = byo * 8u + ((l >> 3) & 7u), z = bzo * 8u + (l >> 6);
         if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz) continue;
         const size_t v = size_t(x) + size_t(a.g.nx) * (size_t(y) + size_t(a.g.ny) * z);
@@ -301,49 +425,17 @@ __global__ void __launch_bounds__(kBuildBlock) brick_kernel(BuildArgs a) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (uint32_t w = 1; w < kBuildBlock / kWave; ++w) {
+        for (uint32_t w = 1; w < kBrickBlock / kWave; ++w) {
             lo = red[0][w] < lo ? red[0][w] : lo;
             hi = red[1][w] > hi ? red[1][w] : hi;
         }
-        if (lo <= hi) {  // the brick holds a voxel of the volume
-            uint32_t* slot = a.minmax + (blockIdx.x % kMinMaxSlots) * kMinMaxStride;
-            atomicMin(&slot[0], lo);
-            atomicMax(&slot[1], hi);
-        }
+        if (lo <= hi)  // the brick holds a voxel of the volume
+            fold_minmax(a.minmax + (blockIdx.x % kMinMaxSlots) * kMinMaxStride, lo, hi);
     }
 }
 
-__global__ void __launch_bounds__(64) build_init_kernel(BuildArgs a) {
-    if (threadIdx.x < kMinMaxSlots) {
-        a.minmax[threadIdx.x * kMinMaxStride] = 255u;
-        a.minmax[threadIdx.x * kMinMaxStride + 1] = 0u;
-    }
-}
-
-// --- the workspace path, taken only when the bins overflowed
-
-__global__ void __launch_bounds__(256) overflow_clear_kernel(BuildArgs a, size_t words) {
-    if (!a.status[1]) return;
-    uint32_t* acc = reinterpret_cast<uint32_t*>(a.acc);
-    const size_t stride = size_t(gridDim.x) * blockDim.x;
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += stride) acc[i] = 0u;
-}
-
-__global__ void __launch_bounds__(256) overflow_voxelize_kernel(BuildArgs a) {
-    if (!a.status[1]) return;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~uint32_t(kWave - 1)); base < a.n_pairs;
-         base += stride) {
-        const uint32_t s = base + (threadIdx.x & uint32_t(kWave - 1));
-        const bool has = s < a.n_pairs;
-        // tangent sums only matter for the tangent volume; the packed count
-        // is exact either way
-        const float* tangents = a.tangent_i8x4 ? a.tangents : nullptr;
-        voxelize_segment(a.g, a.xyz, a.n_vertices, has, has ? a.indices[2u * s] : 0u,
-                         has ? a.indices[2u * s + 1u] : 0u, tangents ? tangents : a.xyz, nullptr, a.acc);
-    }
-}
-
+// The finalize of the workspace path (returns at once while the flag is
+// clear): the outputs of the brick pass from the packed accumulator.
 __global__ void __launch_bounds__(256) overflow_finalize_kernel(BuildArgs a, size_t n) {
     if (!a.status[1]) return;
     uint32_t lo = 255u, hi = 0u;
@@ -364,10 +456,34 @@ __global__ void __launch_bounds__(256) overflow_finalize_kernel(BuildArgs a, siz
         lo = p < lo ? p : lo;
         hi = q > hi ? q : hi;
     }
-    if (threadIdx.x % kWave == 0 && lo <= hi) {
-        uint32_t* slot = a.minmax + (blockIdx.x % kMinMaxSlots) * kMinMaxStride;
-        atomicMin(&slot[0], lo);
-        atomicMax(&slot[1], hi);
+    if (threadIdx.x % kWave == 0 && lo <= hi)
+        fold_minmax(a.minmax + (blockIdx.x % kMinMaxSlots) * kMinMaxStride, lo, hi);
+}
+
+// The corner-packed cells from the uint8 view with the normalize applied as
+// the texels load (the map from the min/max slots), writing the normalized
+// volume and its float view on the way (cells_block<true>).
+__global__ void __launch_bounds__(256)
+mapped_cells_kernel(const uint8_t* __restrict__ raw, uint32_t nx, uint32_t ny, uint32_t nz,
+                    const uint32_t* __restrict__ minmax, int normalize, unsigned long long* __restrict__ cells,
+                    uint32_t* __restrict__ occupancy, uint8_t* __restrict__ out_u8, float* __restrict__ out_f32) {
+    __shared__ uint8_t lut[256];
+    const DensityMap m = density_map_slots(minmax, normalize != 0);
+    // the map of every uint8 value; the identity (no normalize, or a volume
+    // spanning 0..255) needs none
+    bool identity = true;
+    for (uint32_t v = threadIdx.x; v < 256u; v += blockDim.x) {
+        const uint32_t d = texel_density(v, m);
+        lut[v] = uint8_t(d);
+        identity = identity && d == v;
+    }
+    identity = __syncthreads_and(identity);
+    const uint32_t n_blocks = cell_blocks(nx, ny, nz);
+    for (uint32_t b = blockIdx.x; b < n_blocks; b += gridDim.x) {
+        if (identity)
+            cells_block<1>(raw, nx, ny, nz, b, cells, occupancy, nullptr, out_u8, out_f32);
+        else
+            cells_block<2>(raw, nx, ny, nz, b, cells, occupancy, lut, out_u8, out_f32);
     }
 }
 
@@ -399,8 +515,12 @@ extern "C" int vkhr_volume_build_segments(const float* xyz, uint32_t n_vertices,
     hipStream_t s = as_stream(stream);
     const BrickGrid bg = brick_grid(nx, ny, nz);
     const size_t n_bricks = size_t(bg.bx) * bg.by * bg.bz;  // <= 2^24 (make_grid bounds the voxels)
+    const size_t n = size_t(nx) * ny * nz;
     const size_t capacity = std::min<size_t>(4 * size_t(n_pairs) + 65536, size_t(1) << 28);
-    const BuildLayout lay = build_layout(n_bricks, capacity);
+    // with cells the brick pass writes the uint8 view into the scratch and
+    // the cells pass writes the normalized volume
+    const bool mapped = cells != nullptr;
+    const BuildLayout lay = build_layout(n_bricks, n, mapped, capacity);
     void* scr = nullptr;
     if ((rc = stream_scratch(lay.words * 4, s, &scr))) return rc;
     uint32_t* w = static_cast<uint32_t*>(scr);
@@ -412,43 +532,46 @@ extern "C" int vkhr_volume_build_segments(const float* xyz, uint32_t n_vertices,
     a.tangents = tangents;
     a.n_vertices = n_vertices;
     a.n_pairs = n_pairs;
+    a.n_bricks = uint32_t(n_bricks);
+    // ranges of 256 .. 1024 pairs (kPairsPerThread per thread), about 2048 workgroups when that fits
+    a.pairs_per_block = std::min(kPairsPerThread * kBuildBlock, std::max(kBuildBlock, (n_pairs + 2047) / 2048));
+    const uint32_t bin_blocks = std::max(1u, (n_pairs + a.pairs_per_block - 1) / a.pairs_per_block);
     a.bins = w + lay.bins;
     a.cursor = w + lay.cursor;
     a.status = w + lay.status;
     a.minmax = w + lay.minmax;
-    a.entries = w + lay.entries;
+    a.entries = reinterpret_cast<uint2*>(w + lay.entries);
     a.capacity = uint32_t(capacity);
+    a.occupancy = occupancy;
+    a.occupancy_words = occupancy ? occupancy_words(nx, ny, nz) : 0;
     a.counts = counts;
-    a.density = density_u8;
+    a.density = mapped ? reinterpret_cast<uint8_t*>(w + lay.raw) : density_u8;
     a.tangent_i8x4 = tangent_i8x4;
     a.acc = reinterpret_cast<long long*>(workspace);
 
-    if ((rc = fill_u32(a.bins, n_bricks, 0u, s))) return rc;
-    build_init_kernel<<<1, 64, 0, s>>>(a);
-    VKHR_CHECK_LAUNCH();
-    const unsigned pair_blocks = grid_for(n_pairs, kBuildBlock, 65536);
-    if (n_pairs) {
-        bin_kernel<false><<<pair_blocks, kBuildBlock, 0, s>>>(a);
-        VKHR_CHECK_LAUNCH();
-    }
-    scan_kernel<<<1, 1024, 0, s>>>(a, uint32_t(n_bricks));
+    build_init_kernel<<<grid_for(std::max(n_bricks, a.occupancy_words), 256, 1024), 256, 0, s>>>(a);
     VKHR_CHECK_LAUNCH();
     if (n_pairs) {
-        bin_kernel<true><<<pair_blocks, kBuildBlock, 0, s>>>(a);
+        bin_count_kernel<<<bin_blocks, kBuildBlock, 0, s>>>(a);
         VKHR_CHECK_LAUNCH();
     }
-    brick_kernel<<<brick_blocks(bg), kBuildBlock, 0, s>>>(a);
-    VKHR_CHECK_LAUNCH();
-    // the workspace path: every block returns at once unless the bins overflowed
-    const size_t n = size_t(nx) * ny * nz;
-    overflow_clear_kernel<<<1024, 256, 0, s>>>(a, 4 * n);
+    scan_kernel<<<1, 1024, 0, s>>>(a);
     VKHR_CHECK_LAUNCH();
     if (n_pairs) {
-        overflow_voxelize_kernel<<<grid_for(n_pairs, 256, 1024), 256, 0, s>>>(a);
+        bin_fill_kernel<<<bin_blocks, kBuildBlock, 0, s>>>(a, 4 * n);
         VKHR_CHECK_LAUNCH();
     }
+    brick_kernel<<<brick_blocks(bg), kBrickBlock, 0, s>>>(a);
+    VKHR_CHECK_LAUNCH();
+    // the workspace path's finalize: every block returns at once unless the bins overflowed
     overflow_finalize_kernel<<<1024, 256, 0, s>>>(a, n);
     VKHR_CHECK_LAUNCH();
-    return finish_volume(density_u8, nx, ny, nz, normalize, density_f32, cells, occupancy,
-                         normalize ? a.minmax : nullptr, s);
+    if (!mapped)
+        return finish_volume(density_u8, nx, ny, nz, normalize, density_f32, nullptr, nullptr,
+                             normalize ? a.minmax : nullptr, s);
+    mapped_cells_kernel<<<grid_for(cell_blocks(nx, ny, nz), 1, 1u << 20), 256, 0, s>>>(
+        a.density, nx, ny, nz, a.minmax, normalize, reinterpret_cast<unsigned long long*>(cells), occupancy,
+        density_u8, density_f32);
+    VKHR_CHECK_LAUNCH();
+    return occupancy ? launch_distance_field(occupancy, nx, ny, nz, s) : VKHR_OK;
 }

@@ -55,9 +55,7 @@ finalize_counts_kernel(const uint32_t* __restrict__ counts, const long long* __r
             lo = red[0][w] < lo ? red[0][w] : lo;
             hi = red[1][w] > hi ? red[1][w] : hi;
         }
-        uint32_t* slot = minmax + (blockIdx.x % kMinMaxSlots) * kMinMaxStride;
-        atomicMin(&slot[0], lo);
-        atomicMax(&slot[1], hi);
+        fold_minmax(minmax + (blockIdx.x % kMinMaxSlots) * kMinMaxStride, lo, hi);
     }
 }
 
@@ -247,13 +245,17 @@ int finish_volume(uint8_t* density_u8, uint32_t nx, uint32_t ny, uint32_t nz, in
         cells_kernel<<<grid_for(cell_blocks(nx, ny, nz), 1, 1u << 20), 256, 0, s>>>(
             density_u8, nx, ny, nz, reinterpret_cast<unsigned long long*>(cells), occupancy);
         VKHR_CHECK_LAUNCH();
-        if (occupancy && distance_field_fits(nx, ny, nz)) {
-            for (int axis = 0; axis < 3; ++axis) {
-                distance_kernel<<<grid_for(distance_lines(nx, ny, nz, axis), 256 / kWave), 256, 0, s>>>(
-                    occupancy, nx, ny, nz, axis);
-                VKHR_CHECK_LAUNCH();
-            }
-        }
+        if (occupancy && (rc = launch_distance_field(occupancy, nx, ny, nz, s))) return rc;
+    }
+    return VKHR_OK;
+}
+
+int launch_distance_field(uint32_t* occupancy, uint32_t nx, uint32_t ny, uint32_t nz, hipStream_t s) {
+    if (!distance_field_fits(nx, ny, nz)) return VKHR_OK;
+    for (int axis = 0; axis < 3; ++axis) {
+        distance_kernel<<<grid_for(distance_lines(nx, ny, nz, axis), 256 / kWave), 256, 0, s>>>(occupancy, nx, ny,
+                                                                                               nz, axis);
+        VKHR_CHECK_LAUNCH();
     }
     return VKHR_OK;
 }

@@ -157,6 +157,20 @@ constexpr size_t kMinMaxBytes = size_t(kMinMaxSlots) * kMinMaxStride * sizeof(ui
 int finish_volume(uint8_t* density_u8, uint32_t nx, uint32_t ny, uint32_t nz, int normalize, float* density_f32,
                   uint64_t* cells, uint32_t* occupancy, const uint32_t* mm, hipStream_t s);
 
+// The three passes of the brick distance field over an occupancy buffer
+// whose bitmask is set (nothing when a side is too long to transform).
+int launch_distance_field(uint32_t* occupancy, uint32_t nx, uint32_t ny, uint32_t nz, hipStream_t s);
+
+// Folds (lo, hi) into a min/max slot.  A slot only moves toward the
+// result, so an atomic whose value a read of the slot already reaches (or
+// passes) changes nothing and is skipped -- a stale read is never more
+// extreme than the slot, so at worst the atomic is issued -- which keeps
+// thousands of workgroups from queueing on the same two words.
+__device__ __forceinline__ void fold_minmax(uint32_t* slot, uint32_t lo, uint32_t hi) {
+    if (lo < __hip_atomic_load(&slot[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(&slot[0], lo);
+    if (hi > __hip_atomic_load(&slot[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&slot[1], hi);
+}
+
 // Min/max over the slots; called by every lane of a wave (shuffles).
 __device__ __forceinline__ DensityMap density_map_slots(const uint32_t* minmax, bool normalize) {
     const int lane = threadIdx.x % kWave;
@@ -203,9 +217,19 @@ __host__ __device__ __forceinline__ uint32_t cell_blocks(uint32_t nx, uint32_t n
     return ((nx + 1u + 63u) / 64u) * bricks_along(ny) * ((nz + 1u + slab - 1u) / slab);
 }
 
+// kMode 0: density is the final volume.  kMode 1 and 2: density holds the
+// saturating uint8 view before the normalize, and each cell (i+1, j+1, k+1)
+// with i, j, k >= 0 writes its lower corner texel (i, j, k) of the final
+// volume -- every texel exactly once -- to out_u8 (and out_f32, nullable),
+// so the normalize needs no pass of its own: in mode 1 the normalize is the
+// identity, in mode 2 every loaded texel goes through lut (256 entries,
+// the map of the normalize).
+template <int kMode = 0>
 __device__ __forceinline__ void cells_block(const uint8_t* __restrict__ density, uint32_t nx, uint32_t ny,
                                             uint32_t nz, uint32_t block, unsigned long long* __restrict__ cells,
-                                            uint32_t* occupancy) {
+                                            uint32_t* occupancy, const uint8_t* lut = nullptr,
+                                            uint8_t* __restrict__ out_u8 = nullptr,
+                                            float* __restrict__ out_f32 = nullptr) {
     const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
     const uint32_t xbs = (nx + 1u + 63u) / 64u, bys = bricks_along(ny);
     const uint32_t xb = block % xbs, by = (block / xbs) % bys, zc = block / (xbs * bys);
@@ -227,8 +251,12 @@ __device__ __forceinline__ void cells_block(const uint8_t* __restrict__ density,
     auto pair = [&](int y, int z) -> uint32_t {
         if (!col || y < 0 || z < 0 || y >= int(ny) || z >= int(nz)) return 0u;
         const uint8_t* src = density + uint32_t(y) * nx + uint32_t(z) * nx * ny;
-        const uint32_t lo = cxi >= 1u ? src[cxi - 1u] : 0u;
-        const uint32_t hi = cxi < nx ? src[cxi] : 0u;
+        uint32_t lo = cxi >= 1u ? src[cxi - 1u] : 0u;
+        uint32_t hi = cxi < nx ? src[cxi] : 0u;
+        if constexpr (kMode == 2) {  // texels outside the volume stay 0 (the border)
+            lo = cxi >= 1u ? uint32_t(lut[lo]) : 0u;
+            hi = cxi < nx ? uint32_t(lut[hi]) : 0u;
+        }
         return lo | (hi << 8);
     };
     uint32_t below[2];  // slice k (texel z = k) of each row: rows j, j+1 -> 4 bytes
@@ -256,6 +284,15 @@ __device__ __forceinline__ void cells_block(const uint8_t* __restrict__ density,
                 if (col && row[r]) {
                     cells[size_t(cj[r] + (ny + 1u) * ck) * (nx + 1u) + cxi] = packed;
                     any |= packed != 0ull ? 1u : 0u;
+                    if constexpr (kMode != 0) {
+                        // the lower corner texel (cxi-1, cj-1, ck-1): byte 0 of the cell
+                        if (cxi >= 1u && cj[r] >= 1u && ck >= 1u && cxi <= nx && cj[r] <= ny && ck <= nz) {
+                            const size_t t = size_t(cxi - 1u) + size_t(nx) * (size_t(cj[r] - 1u) + size_t(ny) * (ck - 1u));
+                            const uint32_t v = uint32_t(packed & 0xffull);
+                            out_u8[t] = uint8_t(v);
+                            if (out_f32) out_f32[t] = float(v) / 255.0f;  // R8_UNORM
+                        }
+                    }
                 }
                 below[r] = above[u][r];
             }
