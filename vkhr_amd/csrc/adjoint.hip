@@ -15,9 +15,10 @@
 // stored).  The scatter of weight * trilinear weight goes to a corner-packed
 // accumulator of 8 floats per cell (the cell layout of the density: cell
 // (i,j,k) for i,j,k in [-1, n-1] holds texels i..i+1 x j..j+1 x k..k+1), and
-// the lanes of a wave are transposed so that 8 lanes add the 8 corners of one
-// sample: one 32-byte line per sample instead of 8 scattered words (device
-// atomics are bound by line requests).  A gather pass then folds the cells
+// the lanes of a wave are transposed (through LDS) so that 8 lanes add the
+// 8 corners of one sample: one 32-byte line per sample instead of 8
+// scattered words (device atomics are bound by line requests), and
+// neighbouring rays of a pixel row whose samples share a cell add once.  A gather pass then folds the cells
 // into the texel gradient.  Parameter gradients are reduced per block, then
 // one atomic each.
 #include "march.hpp"
@@ -67,56 +68,62 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
-// The weights c of the 8 lanes of this lane's group of 8, which its rounds
-// of scatter_wave take their samples from (a ray's weight is the same for
-// all its samples, so this is formed once per ray family).  Every lane of
-// the wave must call it (shuffles).
-struct GroupWeights {
-    float c[8];
+// A wave's staging area of the scatter: every lane's sample (its cell and
+// the 8 corner weights times the ray's gradient weight).
+struct ScatterLds {
+    uint32_t cell[kWave];
+    float w[8][kWave];
 };
-__device__ __forceinline__ GroupWeights group_weights(float c) {
-    GroupWeights g;
-    const int group = __lane_id() & 56;
-#pragma unroll
-    for (int r = 0; r < 8; ++r) g.c[r] = __shfl(c, group | r, kWave);
-    return g;
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
 // Adds c * (trilinear corner weights) of every lane's sample to the cell
-// accumulator; lanes without a sample pass cell == kNoCell.  In round r the
-// 8 lanes of each group of 8 add the 8 corners of the sample of lane r of
-// the group, whose weight c is gw.c[r].  Every lane of the wave must call it
-// (shuffles).
-__device__ __forceinline__ void scatter_wave(float* __restrict__ cg, uint32_t cell, float wx, float wy, float wz,
-                                             const GroupWeights& gw) {
-    const unsigned long long live = __ballot(cell != kNoCell);
-    if (!live) return;
+// accumulator; lanes without a sample pass cell == kNoCell.  The samples
+// are staged in LDS and transposed: the 8 lanes of each group of 8 (one
+// row of the wave's 8x8 pixel tile) walk the group's 8 samples in order,
+// lane k adding corner k, so one sample is one 32-byte line request; a run
+// of samples of the same cell along the row (neighbouring rays in one cell)
+// is summed first and added once.  Every lane of the wave must call it.
+__device__ __forceinline__ void scatter_wave(float* __restrict__ cg, ScatterLds& L, uint32_t cell, float wx,
+                                             float wy, float wz, float c) {
+    if (!__ballot(cell != kNoCell)) return;
     const int lane = __lane_id();
     const int k = lane & 7, group = lane & 56;
+    L.cell[lane] = cell;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        L.w[q][lane] = c * (((q & 1) ? wx : 1.0f - wx) * ((q & 2) ? wy : 1.0f - wy) * ((q & 4) ? wz : 1.0f - wz));
+    wave_lds_sync();
+    uint32_t run = kNoCell;
+    float acc = 0.0f;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
-        if (!((live >> r) & 0x0101010101010101ull)) continue;
-        const int src = group | r;
-        const uint32_t sc = __shfl(cell, src, kWave);
-        const float sx = __shfl(wx, src, kWave), sy = __shfl(wy, src, kWave), sz = __shfl(wz, src, kWave);
-        if (sc != kNoCell) {
-            const float wt = ((k & 1) ? sx : 1.0f - sx) * ((k & 2) ? sy : 1.0f - sy) * ((k & 4) ? sz : 1.0f - sz);
-            unsafeAtomicAdd(&cg[size_t(sc) * 8u + size_t(k)], gw.c[r] * wt);
+        const uint32_t sc = L.cell[group | r];
+        const float v = L.w[k][group | r];
+        if (sc != run) {
+            if (run != kNoCell) unsafeAtomicAdd(&cg[size_t(run) * 8u + size_t(k)], acc);
+            run = sc;
+            acc = 0.0f;
         }
+        acc += v;  // (kNoCell runs are never added)
     }
+    if (run != kNoCell) unsafeAtomicAdd(&cg[size_t(run) * 8u + size_t(k)], acc);
+    wave_lds_sync();  // the area is rewritten by the next call
 }
 
 // One family of rays (primary or shadow): x(t) = x0 + t * dx for the
 // sample parameters t of the shader's float loop, samples [0, n) of each
 // active lane's ray, weight c per sample.  The loop runs to the wave's
 // longest ray so that every lane takes part in the scatter.
-__device__ void scatter_rays(const AdjointArgs& a, bool active, const double x0[3], const double dx[3], int n,
-                             float c) {
+__device__ void scatter_rays(const AdjointArgs& a, ScatterLds& L, bool active, const double x0[3], const double dx[3],
+                             int n, float c) {
     const RaymarchArgs& f = a.f;
     const float step = f.step_size;
     const int n_lane = active ? n : 0;
     const int n_wave = wave_max(n_lane);
-    const GroupWeights gw = group_weights(c);
     float t = 0.0f;  // the march state carried across chunks
     for (int i0 = 0; i0 < n_wave; i0 += kChunk) {
         const int i1 = min(i0 + kChunk, n_wave);
@@ -127,7 +134,7 @@ __device__ void scatter_rays(const AdjointArgs& a, bool active, const double x0[
             float w[3] = {0.0f, 0.0f, 0.0f};
             uint32_t cell = kNoCell;
             if (i < n_lane && in_range(f, x)) cell = cell_of(f, x, w);
-            scatter_wave(a.cell_grad, cell, w[0], w[1], w[2], gw);
+            scatter_wave(a.cell_grad, L, cell, w[0], w[1], w[2], c);
         }
         // advance the carried state past this chunk
         for (int i = 0; i < kChunk; ++i) t += step;
@@ -154,13 +161,21 @@ gather_cells_kernel(const float* __restrict__ cg, float* __restrict__ grad, int 
     }
 }
 
+// Which walks adjoint_tile takes: the primary rays with the LAO taps and
+// the parameter gradients (and the loss), the shadow rays, or both.  The
+// per-style kernel gives each family a workgroup of its own, so a tile's
+// critical path is the longer walk rather than the sum of the two.
+enum AdjointWalks { kWalkPrimary = 1, kWalkShadow = 2, kWalkBoth = 3 };
+
 // The adjoint of one pixel (volume.frag:31-102 differentiated).  Called by
 // all 256 threads of a block: 2x2 waves of 8x8 pixels from (x0, y0); red is
 // the block's reduction buffer for the parameter gradients.
-__device__ __forceinline__ void adjoint_tile(const AdjointArgs& a, int x0, int y0, float (*red)[256 / kWave]) {
+__device__ __forceinline__ void adjoint_tile(const AdjointArgs& a, int x0, int y0, float (*red)[256 / kWave],
+                                             ScatterLds* scatter, int walks = kWalkBoth) {
     // red: 4 x (waves per block) floats
     const RaymarchArgs& f = a.f;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    ScatterLds& L = scatter[wave];
     const int px = x0 + (wave & 1) * 8 + (lane & 7);
     const int py = y0 + (wave >> 1) * 8 + (lane >> 3);
     const bool inside = px < f.width && py < f.height;
@@ -210,25 +225,28 @@ __device__ __forceinline__ void adjoint_tile(const AdjointArgs& a, int x0, int y
             d_alpha += G * (f.adsm_k * double(r.shadow_sum)) * (-1.0 / (1.0 - f.alpha));
         }
     }
-    // parameter gradients (and the loss): wave, then block, then one atomic each
-    const float pv[4] = {float(d_alpha), float(d_k), float(d_p), sq};
-    for (int k = 0; k < 4; ++k) {
-        float v = pv[k];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-        if (lane == 0) red[k][wave] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        float v = 0.0f;
-        for (int w = 0; w < int(blockDim.x / kWave); ++w) v += red[threadIdx.x][w];
-        float* dst = threadIdx.x < 3 ? &a.d_params[threadIdx.x] : a.loss;
-        if (v != 0.0f && dst) atomicAdd(dst, v);
-    }
-    __syncthreads();  // red is reused by the next tile of a persistent block
+    const bool primary = (walks & kWalkPrimary) != 0, shadow = (walks & kWalkShadow) != 0;
+    if (primary) {  // workgroup-uniform
+        // parameter gradients (and the loss): wave, then block, then one atomic each
+        const float pv[4] = {float(d_alpha), float(d_k), float(d_p), sq};
+        for (int k = 0; k < 4; ++k) {
+            float v = pv[k];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+            if (lane == 0) red[k][wave] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            float v = 0.0f;
+            for (int w = 0; w < int(blockDim.x / kWave); ++w) v += red[threadIdx.x][w];
+            float* dst = threadIdx.x < 3 ? &a.d_params[threadIdx.x] : a.loss;
+            if (v != 0.0f && dst) atomicAdd(dst, v);
+        }
+        __syncthreads();  // red is reused by the next tile of a persistent block
 
-    // primary rays
-    const Ray ray = setup_ray(f, px, py, inside && frag && c_primary != 0.0);
-    scatter_rays(a, ray.hit && c_primary != 0.0, ray.xs, ray.dx, frag ? r.n_primary : 0, float(c_primary));
+        // primary rays
+        const Ray ray = setup_ray(f, px, py, inside && frag && c_primary != 0.0);
+        scatter_rays(a, L, ray.hit && c_primary != 0.0, ray.xs, ray.dx, frag ? r.n_primary : 0, float(c_primary));
+    }
     // shadow rays: from the surface point to the light origin, every sample
     double s[3] = {0, 0, 0}, sl[3] = {0, 0, 0};
     if (frag) {
@@ -238,12 +256,13 @@ __device__ __forceinline__ void adjoint_tile(const AdjointArgs& a, int x0, int y
     // samples past the volume's support carry no gradient: the walk stops
     // there (an upper bound on their count is enough, in_range does the rest)
     int n_shadow = 0;
-    if (frag) {
+    if (frag && shadow) {
         double lo, hi;
         support_interval(f, s, sl, &lo, &hi);
         if (hi >= 0.0 && lo <= hi) n_shadow = min(a.n_steps, int(fmin(hi, 2.0) / double(f.step_size) * 1.0001) + 2);
     }
-    scatter_rays(a, frag && adsm_on && c_shadow != 0.0, s, sl, n_shadow, float(c_shadow));
+    if (shadow) scatter_rays(a, L, frag && adsm_on && c_shadow != 0.0, s, sl, n_shadow, float(c_shadow));
+    if (!primary) return;
     // LAO taps: d ln(lao)/dD = -e / (8 - D), through min(v, clamp)
     const bool lao = frag && lao_on && G != 0.0;
     double v[8], D = 0.0;
@@ -257,20 +276,21 @@ __device__ __forceinline__ void adjoint_tile(const AdjointArgs& a, int x0, int y
         D += fmin(v[n], f.ao_clamp);
     }
     const float cl = lao ? float(G * (-f.ao_exponent) / (8.0 - D)) : 0.0f;
-    const GroupWeights gl = group_weights(cl);
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
         float w[3] = {0.0f, 0.0f, 0.0f};
         uint32_t cell = kNoCell;
         if (lao && v[n] < f.ao_clamp && in_range(f, tap[n])) cell = cell_of(f, tap[n], w);
-        scatter_wave(a.cell_grad, cell, w[0], w[1], w[2], gl);
+        scatter_wave(a.cell_grad, L, cell, w[0], w[1], w[2], cl);
     }
 }
 
+// blockIdx.z: 0 the primary rays, LAO and parameters, 1 the shadow rays.
 __global__ void __launch_bounds__(256)
 adjoint_kernel(AdjointArgs a) {
     __shared__ float red[4][256 / kWave];
-    adjoint_tile(a, blockIdx.x * 16, blockIdx.y * 16, red);
+    __shared__ ScatterLds scatter[256 / kWave];
+    adjoint_tile(a, blockIdx.x * 16, blockIdx.y * 16, red, scatter, blockIdx.z == 0 ? kWalkPrimary : kWalkShadow);
 }
 
 // ---------------------------------------------------------- batched form
@@ -331,9 +351,10 @@ adjoint_batch_forward_kernel(AdjointBatchDevice d, const AdjointArgs* __restrict
 __global__ void __launch_bounds__(256)
 adjoint_batch_backward_kernel(AdjointBatchDevice d, const AdjointArgs* __restrict__ args) {
     __shared__ float red[4][256 / kWave];
+    __shared__ ScatterLds scatter[256 / kWave];
     const uint32_t item = uint32_t(__builtin_amdgcn_readfirstlane(int(blockIdx.x)));
     const uint32_t r = item / d.tiles_per_render, tile = item % d.tiles_per_render;
-    adjoint_tile(args[r], int(tile % d.tiles_x) * 16, int(tile / d.tiles_x) * 16, red);
+    adjoint_tile(args[r], int(tile % d.tiles_x) * 16, int(tile / d.tiles_x) * 16, red, scatter);
 }
 
 // d_density = the fold of each style's cells (overwritten).
@@ -442,7 +463,9 @@ extern "C" int vkhr_raymarch_adjoint(const vkhr_volume* volume, const vkhr_camer
         delete a;
         return rc;
     }
-    dim3 grid((width + 15) / 16, (height + 15) / 16);
+    // each walk family in workgroups of its own: z = 0 the primary rays
+    // (with LAO and parameters), z = 1 the shadow rays
+    dim3 grid((width + 15) / 16, (height + 15) / 16, 2);
     adjoint_kernel<<<grid, 256, 0, st>>>(*a);
     const size_t n_vox = size_t(volume->nx) * volume->ny * volume->nz;
     gather_cells_kernel<<<grid_for(n_vox, 256, 8192), 256, 0, st>>>(a->cell_grad, d_density, int(volume->nx),
