@@ -520,7 +520,8 @@ def summarize_profile(root: Path, out: Path):
 def time_batch(dev, reps=3):
     """Batched form: 32 styles of uneven size at 960x540; returns (per-style
     launches back to back ms, batched graph ms, sum over styles of one
-    style's launches waited for, ms)."""
+    style's launches waited for, ms, per-style launches with the one-call
+    volume build ms)."""
     import torch
     from vkhr_amd import synth
     from vkhr_amd.batch import Batch, StyleBuffers, load_tuning
@@ -533,7 +534,7 @@ def time_batch(dev, reps=3):
         for b in bufs:
             b.run_plain()
     torch.cuda.synchronize()
-    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
     e[0].record()
     for _ in range(reps):
         for b in bufs:
@@ -545,6 +546,16 @@ def time_batch(dev, reps=3):
         batch.launch(wait=False)
     e[3].record(batch.stream)
     torch.cuda.synchronize()
+    # the per-style sequences with the one-call volume build
+    for b in bufs:
+        b.run_plain(one_call=True)
+    torch.cuda.synchronize()
+    e[4].record()
+    for _ in range(reps):
+        for b in bufs:
+            b.run_plain(one_call=True)
+    e[5].record()
+    torch.cuda.synchronize()
     # each style run on its own and waited for (wall clock, host issue included)
     synced = 0.0
     for b in bufs:
@@ -552,7 +563,8 @@ def time_batch(dev, reps=3):
         b.run_plain()
         torch.cuda.synchronize()
         synced += time.perf_counter() - t0
-    return e[0].elapsed_time(e[1]) / reps, e[2].elapsed_time(e[3]) / reps, synced * 1e3
+    return (e[0].elapsed_time(e[1]) / reps, e[2].elapsed_time(e[3]) / reps, synced * 1e3,
+            e[4].elapsed_time(e[5]) / reps)
 
 
 def embree_present() -> bool:
@@ -887,12 +899,16 @@ def main():
         result["adjoint"] = {"workload": "128^3 float32 density, 512x512, 256 steps", "forward_ms": t_f,
                              "adjoint_ms": t_b, "bwd_over_fwd": t_b / t_f,
                              "note": "adjoint_fwd_ratio = (forward_ms + adjoint_ms) / forward_ms"}
-        t_plain, t_batch, t_synced = time_batch(dev)
+        t_plain, t_batch, t_synced, t_plain_build = time_batch(dev)
         result["batched"] = {"workload": "32 styles, 5k-140k strands, 64^3/128^3/256^3, 960x540, 512 steps",
                              "per_style_ms": t_plain, "batched_graph_ms": t_batch, "speedup": t_plain / t_batch,
                              "per_style_synced_ms": t_synced, "speedup_vs_synced": t_synced / t_batch,
-                             "note": "per_style_ms: the 32 per-style sequences issued back to back; "
-                                     "per_style_synced_ms: the sum of each style's sequence timed alone"}
+                             "per_style_one_call_build_ms": t_plain_build,
+                             "speedup_vs_one_call_build": t_plain_build / t_batch,
+                             "note": "per_style_ms: the 32 per-style sequences (voxelize_segments, finalize, "
+                                     "raymarch) issued back to back; per_style_synced_ms: the sum of each style's "
+                                     "sequence timed alone; per_style_one_call_build_ms: the per-style sequences "
+                                     "with the one-call volume build instead of voxelize + finalize"}
         t_plain, t_batch = time_adjoint_batch(dev)
         result["batched_adjoint"] = {"workload": "8 styles x 8 views, 128^3 float32, 512x512, 256 steps, "
                                                  "forward + L2 loss gradient + adjoint",

@@ -68,6 +68,22 @@ def test_batch_matches_per_style_bit_for_bit(batch32, tuning):
     assert sum(int((b.rgba[..., 3] > 0).sum()) for b in batch32) > 1_000_000
 
 
+def test_batch_matches_per_style_one_call_build(batch32):
+    """The per-style sequence with the one-call volume build gives the
+    batched outputs too (density, tangents, cells, occupancy, image)."""
+    batch = Batch(batch32, W, H, tuning=load_tuning("batch"))
+    batch.launch()
+    torch.cuda.synchronize()
+    for b in batch32[::3]:
+        out = _plain_outputs(b)   # buffers shaped like the style's own
+        for k in ("counts", "density", "tangents", "cells", "occupancy", "rgba"):
+            out[k].zero_()
+        b.run_plain(out, one_call=True)
+        for k, mine in [("counts", b.counts), ("density", b.density), ("tangents", b.tangents), ("cells", b.cells),
+                        ("occupancy", b.occupancy), ("rgba", b.rgba)]:
+            assert torch.equal(mine, out[k]), (k, b.n)
+
+
 def test_invalid_batch_arguments(batch32):
     bad = capi.Tuning()
     bad.tile_w, bad.tile_h, bad.blocks_per_cu = 8, 4, 4    # 32 pixels per wave

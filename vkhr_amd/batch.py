@@ -67,17 +67,25 @@ class StyleBuffers:
         b.camera, b.light, b.strand, b.params = self.cam, self.light, self.sp, self.rp
         return b
 
-    def run_plain(self, out: dict | None = None):
-        """The per-style sequence: voxelize_segments -> finalize -> raymarch,
-        into separate output buffers (out) or this style's own."""
+    def run_plain(self, out: dict | None = None, one_call: bool = False):
+        """The per-style sequence: voxelize_segments -> finalize -> raymarch
+        (one_call: the one-call volume build -> raymarch), into separate
+        output buffers (out) or this style's own."""
         o = out if out is not None else dict(counts=self.counts, acc=self.acc, density=self.density,
                                              tangents=self.tangents, cells=self.cells, occupancy=self.occupancy,
                                              rgba=self.rgba)
         res = (self.n,) * 3
-        D.voxelize(self.xyz, self.bounds, res, indices=self.idx, tangents=self.tan, want_sums=True,
-                   counts=o["counts"], sums=o["acc"], want_counts=o["counts"] is not None)
-        call("vkhr_volume_finalize", ptr(o["counts"]), ptr(o["acc"]), self.n, self.n, self.n, 1, ptr(o["density"]),
-             ptr(o["tangents"]), None, ptr(o["cells"]), ptr(o.get("occupancy")), capi.stream_of())
+        if one_call:
+            vol = dict(counts=o["counts"], density=o["density"], tangents=o["tangents"], density_f32=None,
+                       cells=o["cells"], occupancy=o.get("occupancy"))
+            D.build_volume(self.xyz, self.bounds, res, self.idx, self.tan, normalize=True, workspace=o["acc"],
+                           out=vol)
+        else:
+            D.voxelize(self.xyz, self.bounds, res, indices=self.idx, tangents=self.tan, want_sums=True,
+                       counts=o["counts"], sums=o["acc"], want_counts=o["counts"] is not None)
+            call("vkhr_volume_finalize", ptr(o["counts"]), ptr(o["acc"]), self.n, self.n, self.n, 1,
+                 ptr(o["density"]), ptr(o["tangents"]), None, ptr(o["cells"]), ptr(o.get("occupancy")),
+                 capi.stream_of())
         vol = dict(density=o["density"], cells=o["cells"], tangents=o["tangents"], occupancy=o.get("occupancy"))
         D.raymarch(vol, res, self.cam, self.light, self.sp, self.rp, self.width, self.height, want_depth=False,
                    out=o["rgba"])

@@ -38,8 +38,8 @@ __device__ __forceinline__ void batch_item(const BatchDevice& d, const RaymarchA
 }
 
 // One block per work item: the hardware dispatcher balances the tiles of all
-// styles.
-__global__ void __launch_bounds__(256)
+// styles.  Five waves per SIMD, as the per-style raymarch kernel.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 batch_raymarch_grid_kernel(BatchDevice d, const RaymarchArgs* __restrict__ args) {
     extern __shared__ uint32_t occ_lds[];   // d.march_lds_bytes
     uint32_t staged = 0xffffffffu;

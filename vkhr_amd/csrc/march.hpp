@@ -294,13 +294,9 @@ __device__ __forceinline__ double fixed_exit_margin(int steps) {
     return 1e-6 + double(steps + 1) * 1.1641532182693481e-10;  // 2^-33
 }
 
-// One trilinear tap of the cells at cell (cx, cy, cz) (texel + 1, inside
-// the support) with fractions (fx, fy, fz) in units of 2^-32: the blend of
-// march_tap.
-__device__ __forceinline__ float fixed_cells_tap(const DensityVolume& v, uint32_t cx, uint32_t cy, uint32_t cz,
-                                                 uint32_t fx, uint32_t fy, uint32_t fz) {
-    const uint32_t c = cx + __umul24(uint32_t(v.nx) + 1u, cy + __umul24(uint32_t(v.ny) + 1u, cz));
-    const unsigned long long q = v.cells[c];
+// The trilinear blend (march_tap's) of a gathered cell at fractions (fx, fy, fz)
+// in units of 2^-32.
+__device__ __forceinline__ float fixed_cells_blend(unsigned long long q, uint32_t fx, uint32_t fy, uint32_t fz) {
     const uint32_t lo = uint32_t(q), hi = uint32_t(q >> 32);
     const float b0 = float(lo & 0xffu), b1 = float((lo >> 8) & 0xffu);
     const float b2 = float((lo >> 16) & 0xffu), b3 = float(lo >> 24);
@@ -315,6 +311,14 @@ __device__ __forceinline__ float fixed_cells_tap(const DensityVolume& v, uint32_
     const float c0 = fmaf(wy, c10 - c00, c00);
     const float c1 = fmaf(wy, c11 - c01, c01);
     return fmaf(wz, c1 - c0, c0) * (1.0f / 255.0f);
+}
+
+// One trilinear tap of the cells at cell (cx, cy, cz) (texel + 1, inside
+// the support) with fractions (fx, fy, fz) in units of 2^-32.
+__device__ __forceinline__ float fixed_cells_tap(const DensityVolume& v, uint32_t cx, uint32_t cy, uint32_t cz,
+                                                 uint32_t fx, uint32_t fy, uint32_t fz) {
+    const uint32_t c = cx + __umul24(uint32_t(v.nx) + 1u, cy + __umul24(uint32_t(v.ny) + 1u, cz));
+    return fixed_cells_blend(v.cells[c], fx, fy, fz);
 }
 
 // The distance nibble of the brick of cell (cx, cy, cz) (0 without the
@@ -347,11 +351,27 @@ __device__ __forceinline__ double fixed_shadow_sum(const DensityVolume& v, const
         const uint32_t cx = uint32_t(P[0] >> 32), cy = uint32_t(P[1] >> 32), cz = uint32_t(P[2] >> 32);
         const uint32_t nib = fixed_brick_distance(v, occ, cx, cy, cz);
         if (nib == 0u) {
-            sum += double(fixed_cells_tap(v, cx, cy, cz, uint32_t(P[0]), uint32_t(P[1]), uint32_t(P[2])));
-            P[0] += r.d[0];
-            P[1] += r.d[1];
-            P[2] += r.d[2];
-            ++j;
+            // sample j + 1 too when it lies in the same (occupied) brick
+            // before the end: both gathers are in flight together
+            const long long Q0 = P[0] + r.d[0], Q1 = P[1] + r.d[1], Q2 = P[2] + r.d[2];
+            const uint32_t qx = uint32_t(Q0 >> 32), qy = uint32_t(Q1 >> 32), qz = uint32_t(Q2 >> 32);
+            const bool pair = (j + 1 < j_end) & (((qx ^ cx) | (qy ^ cy) | (qz ^ cz)) < (1u << kBrickShift));
+            const uint32_t nx1 = uint32_t(v.nx) + 1u, ny1 = uint32_t(v.ny) + 1u;
+            const unsigned long long g0 = v.cells[cx + __umul24(nx1, cy + __umul24(ny1, cz))];
+            const unsigned long long g1 = pair ? v.cells[qx + __umul24(nx1, qy + __umul24(ny1, qz))] : 0ull;
+            sum += double(fixed_cells_blend(g0, uint32_t(P[0]), uint32_t(P[1]), uint32_t(P[2])));
+            if (pair) {
+                sum += double(fixed_cells_blend(g1, uint32_t(Q0), uint32_t(Q1), uint32_t(Q2)));
+                P[0] = Q0 + r.d[0];
+                P[1] = Q1 + r.d[1];
+                P[2] = Q2 + r.d[2];
+                j += 2;
+            } else {
+                P[0] = Q0;
+                P[1] = Q1;
+                P[2] = Q2;
+                ++j;
+            }
         } else {
             // the last sample inside the region (planes taken inside by the
             // margin), then the next one
@@ -483,10 +503,37 @@ __device__ __forceinline__ void raymarch_pixel(const RaymarchArgs& a, int px, in
                         float d = 0.0f;
                         int next = i + 1;
                         if (nib == 0u) {
-                            d = fixed_cells_tap(vol, cx, cy, cz, uint32_t(P[0]), uint32_t(P[1]), uint32_t(P[2]));
-                            P[0] += r.d[0];
-                            P[1] += r.d[1];
-                            P[2] += r.d[2];
+                            // sample i + 1 too when it lies in the same (occupied)
+                            // brick before the end and the depth test: both
+                            // gathers are in flight together
+                            const long long Q0 = P[0] + r.d[0], Q1 = P[1] + r.d[1], Q2 = P[2] + r.d[2];
+                            const uint32_t qx = uint32_t(Q0 >> 32), qy = uint32_t(Q1 >> 32), qz = uint32_t(Q2 >> 32);
+                            const bool pair = (next < i_end) & (next < i_depth) &
+                                              (((qx ^ cx) | (qy ^ cy) | (qz ^ cz)) < (1u << kBrickShift));
+                            const uint32_t nx1 = uint32_t(vol.nx) + 1u, ny1 = uint32_t(vol.ny) + 1u;
+                            const uint32_t c0 = cx + __umul24(nx1, cy + __umul24(ny1, cz));
+                            const uint32_t c1 = qx + __umul24(nx1, qy + __umul24(ny1, qz));
+                            const unsigned long long g0 = vol.cells[c0];
+                            const unsigned long long g1 = pair ? vol.cells[c1] : 0ull;
+                            d = fixed_cells_blend(g0, uint32_t(P[0]), uint32_t(P[1]), uint32_t(P[2]));
+                            if (pair) {
+                                const float d1 = fixed_cells_blend(g1, uint32_t(Q0), uint32_t(Q1), uint32_t(Q2));
+                                acc += double(d);
+                                const bool nz0 = d != 0.0f;
+                                surf = (nz0 & (acc <= iso)) ? i : surf;
+                                found = found | (nz0 & (acc >= iso));
+                                entry = (nz0 & (entry < 0)) ? i : entry;
+                                d = d1;
+                                ++i;
+                                next = i + 1;
+                                P[0] = Q0 + r.d[0];
+                                P[1] = Q1 + r.d[1];
+                                P[2] = Q2 + r.d[2];
+                            } else {
+                                P[0] = Q0;
+                                P[1] = Q1;
+                                P[2] = Q2;
+                            }
                         } else {
                             if (next < i_depth) {
                                 // the last sample inside the region (planes taken
