@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--summarize-profile", metavar="ROOT",
                     help="write the raymarch counter summary from rocprofv3 CSV runs under ROOT (no GPU) and exit")
     ap.add_argument("--out", default=str(PROFILE_SUMMARY), help="summary file of --summarize-profile")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only time the 32-style batched leg against its per-style sequences (for a kernel "
+                         "profile of that leg), print its JSON object and exit")
     return ap.parse_args()
 
 
@@ -567,6 +570,18 @@ def time_batch(dev, reps=3):
             e[4].elapsed_time(e[5]) / reps)
 
 
+def batched_result(t_plain, t_batch, t_synced, t_plain_build) -> dict:
+    return {"workload": "32 styles, 5k-140k strands, 64^3/128^3/256^3, 960x540, 512 steps",
+            "per_style_ms": t_plain, "batched_graph_ms": t_batch, "speedup": t_plain / t_batch,
+            "per_style_synced_ms": t_synced, "speedup_vs_synced": t_synced / t_batch,
+            "per_style_one_call_build_ms": t_plain_build,
+            "speedup_vs_one_call_build": t_plain_build / t_batch,
+            "note": "per_style_ms: the 32 per-style sequences (voxelize_segments, finalize, raymarch) issued back "
+                    "to back; per_style_synced_ms: the sum of each style's sequence timed alone; "
+                    "per_style_one_call_build_ms: the per-style sequences with the one-call volume build instead "
+                    "of voxelize + finalize"}
+
+
 def embree_present() -> bool:
     try:
         out = subprocess.run(["ldconfig", "-p"], capture_output=True, text=True, timeout=10).stdout
@@ -686,6 +701,12 @@ def main():
         print(json.dumps({"error": "this path runs on one GPU; --gpus must be 1 (the multi-GPU sharding is a "
                                    "design, not built)"}))
         sys.exit(2)
+    if args.batched_only:
+        import torch
+        dev = torch.device("cuda:0")
+        torch.cuda.set_device(dev)
+        print(json.dumps({"batched": batched_result(*time_batch(dev))}))
+        return
     import numpy as np
     import torch
 
@@ -899,16 +920,7 @@ def main():
         result["adjoint"] = {"workload": "128^3 float32 density, 512x512, 256 steps", "forward_ms": t_f,
                              "adjoint_ms": t_b, "bwd_over_fwd": t_b / t_f,
                              "note": "adjoint_fwd_ratio = (forward_ms + adjoint_ms) / forward_ms"}
-        t_plain, t_batch, t_synced, t_plain_build = time_batch(dev)
-        result["batched"] = {"workload": "32 styles, 5k-140k strands, 64^3/128^3/256^3, 960x540, 512 steps",
-                             "per_style_ms": t_plain, "batched_graph_ms": t_batch, "speedup": t_plain / t_batch,
-                             "per_style_synced_ms": t_synced, "speedup_vs_synced": t_synced / t_batch,
-                             "per_style_one_call_build_ms": t_plain_build,
-                             "speedup_vs_one_call_build": t_plain_build / t_batch,
-                             "note": "per_style_ms: the 32 per-style sequences (voxelize_segments, finalize, "
-                                     "raymarch) issued back to back; per_style_synced_ms: the sum of each style's "
-                                     "sequence timed alone; per_style_one_call_build_ms: the per-style sequences "
-                                     "with the one-call volume build instead of voxelize + finalize"}
+        result["batched"] = batched_result(*time_batch(dev))
         t_plain, t_batch = time_adjoint_batch(dev)
         result["batched_adjoint"] = {"workload": "8 styles x 8 views, 128^3 float32, 512x512, 256 steps, "
                                                  "forward + L2 loss gradient + adjoint",

@@ -500,14 +500,17 @@ typedef struct vkhr_batch_style {
     uint32_t n_vertices;
     uint32_t n_indices;
     const uint32_t* indices;
-    const float* tangents;        /* nullable with tangent_acc/tangent_i8x4 */
+    const float* tangents;        /* nullable without tangent_acc */
     vkhr_aabb bounds;             /* the style's AABB (vkhr_aabb_compute) */
     uint32_t volume;              /* n of the n^3 volume */
     uint32_t reserved_;
-    uint32_t* counts;             /* n^3; nullable with tangent_acc; 16-byte aligned */
-    int64_t* tangent_acc;         /* 2 n^3, nullable; 16-byte aligned */
+    uint32_t* counts;             /* n^3; nullable with tangent_acc (without it, also the
+                                     workspace of the build's overflow path); 16-byte aligned */
+    int64_t* tangent_acc;         /* 2 n^3, nullable: the workspace of the build's overflow path
+                                     (contents unspecified after a launch); with it, tangents and
+                                     tangent_i8x4 are required; 16-byte aligned */
     uint8_t* density;             /* n^3, normalized; 4-byte aligned */
-    int8_t* tangent_i8x4;         /* 4 n^3, nullable; 16-byte aligned */
+    int8_t* tangent_i8x4;         /* 4 n^3, nullable; needs tangent_acc; 16-byte aligned */
     uint64_t* cells;              /* (n+1)^3 */
     uint32_t* occupancy;          /* vkhr_occupancy_words(n, n, n), nullable */
     float* rgba;                  /* width*height float4 */
@@ -524,24 +527,29 @@ typedef struct vkhr_tuning {
                                      per (style, tile) work item */
     uint32_t groups;              /* styles split into this many groups of consecutive styles
                                      (about equal voxels + segments), pipelined in the graph: a
-                                     group's raymarch runs beside the next group's volume passes
+                                     group's raymarch runs beside the next group's volume build
                                      on a second stream; 0 or 1: one chain; at most 64 */
 } vkhr_tuning;
 
 typedef struct vkhr_batch vkhr_batch;
 
-/* Builds the device tables of a batch: voxelize_segments + finalize
- * (normalize, tangents, cells) + raymarch at width x height for every
- * style.  tuning may be NULL (8x8 wave tiles, one block per work item). */
+/* Builds the device tables of a batch: the volume build (voxelize_segments
+ * + finalize: normalize, tangents, cells) + raymarch at width x height for
+ * every style, and the build state of every style (bins, bin entries for
+ * 4 runs per index pair + 65536, the uint8 view).  tuning may be NULL (8x8
+ * wave tiles, one block per work item). */
 int vkhr_batch_create(const vkhr_batch_style* styles, uint32_t n_styles, uint32_t width, uint32_t height,
                       const vkhr_tuning* tuning, vkhr_batch** out);
 
 /* Runs the whole batch as one hipGraph (captured on the first call on a
- * created, non-NULL stream, replayed after): a clear, a voxelization over
- * the concatenated segments of all styles, chunked finalize and cell
- * kernels (the cells carry the normalized density), and a persistent
- * raymarch over (style, tile) work items.
- * Per-style results are bit-identical to vkhr_voxelize_segments +
+ * created, non-NULL stream, replayed after): the passes of
+ * vkhr_volume_build_segments (bin count, scan, bin fill, bricks summed in
+ * LDS, cells with the normalize, distance field), each one launch over the
+ * workgroups of all styles, and a raymarch over (style, tile) work items.
+ * A style whose runs pass its bin capacity takes the build's workspace
+ * path on its own, as vkhr_volume_build_segments does.
+ * Per-style results (counts, density, tangent_i8x4, cells, occupancy, rgba)
+ * are bit-identical to vkhr_voxelize_segments +
  * vkhr_volume_finalize(normalize=1) + vkhr_raymarch. */
 int vkhr_batch_launch(vkhr_batch* batch, vkhr_stream stream);
 int vkhr_batch_destroy(vkhr_batch* batch);

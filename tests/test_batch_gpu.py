@@ -23,12 +23,15 @@ def batch32(device):
 
 
 def _plain_outputs(b):
+    """The per-style sequence's outputs for style b (with a tangent volume
+    when b has one)."""
     nv = b.n ** 3
     dev = b.xyz.device
+    tangents = b.acc is not None
     out = dict(counts=torch.empty(nv, dtype=torch.int32, device=dev),
-               acc=torch.empty(2 * nv, dtype=torch.int64, device=dev),
+               acc=torch.empty(2 * nv, dtype=torch.int64, device=dev) if tangents else None,
                density=torch.empty(nv, dtype=torch.uint8, device=dev),
-               tangents=torch.empty(4 * nv, dtype=torch.int8, device=dev),
+               tangents=torch.empty(4 * nv, dtype=torch.int8, device=dev) if tangents else None,
                cells=torch.empty((b.n + 1) ** 3, dtype=torch.int64, device=dev),
                occupancy=torch.empty_like(b.occupancy),
                rgba=torch.empty((H, W, 4), dtype=torch.float32, device=dev))
@@ -55,7 +58,7 @@ def test_batch_matches_per_style_bit_for_bit(batch32, tuning):
     first = [b.rgba.clone() for b in batch32[:4]]
     for b in batch32:
         ref = _plain_outputs(b)
-        for k, mine in [("counts", b.counts), ("acc", b.acc), ("density", b.density), ("tangents", b.tangents),
+        for k, mine in [("counts", b.counts), ("density", b.density), ("tangents", b.tangents),
                         ("cells", b.cells), ("occupancy", b.occupancy), ("rgba", b.rgba)]:
             assert torch.equal(mine, ref[k]), (k, b.n)
         del ref
@@ -82,6 +85,49 @@ def test_batch_matches_per_style_one_call_build(batch32):
         for k, mine in [("counts", b.counts), ("density", b.density), ("tangents", b.tangents), ("cells", b.cells),
                         ("occupancy", b.occupancy), ("rgba", b.rgba)]:
             assert torch.equal(mine, out[k]), (k, b.n)
+
+
+def _long_segment_style(n_pairs, seed):
+    """Strands of one segment between random points of the box: each visits
+    ~16 bricks at 128^3, so 20,000 of them pass the bin capacity (4 runs per
+    pair + 65536) and the build takes the workspace path."""
+    rng = np.random.default_rng(seed)
+    verts = rng.uniform(-1.0, 1.0, size=(2 * n_pairs, 3)).astype(np.float32)
+    return synth.Style(vertices=verts, segments=np.ones(n_pairs, np.uint16),
+                       thickness=np.ones(2 * n_pairs, np.float32), seed=seed)
+
+
+@pytest.mark.parametrize("groups", [1, 3])
+def test_batch_counts_only_and_overflow_styles(device, groups):
+    """Styles the 32-style batch does not hold: one with counts only (no
+    tangent workspace or tangent volume: the counts are the overflow
+    accumulator, shaded with the ADSM-only model), one whose runs overflow
+    the bins (the workspace path, with tangents), one with no segments, and
+    a plain one, in one batch; every output equals the per-style sequence."""
+    from vkhr_amd import params
+    styles = [(synth.hemisphere_style(3000, 16, seed=5), 64, False), (_long_segment_style(20000, 3), 128, True),
+              (synth.hemisphere_style(50, 4, seed=7), 64, True), (synth.hemisphere_style(2000, 12, seed=6), 64, True)]
+    bufs = []
+    for i, (st, n, tangents) in enumerate(styles):
+        b = StyleBuffers(st, n, W, H, device, azimuth=30.0 * i)
+        if not tangents:
+            b.acc, b.tangents = None, None
+            b.rp = params.render(raycast_steps=512.0, shading_model=1)
+        bufs.append(b)
+    bufs[2].idx = bufs[2].idx[:0]
+    t = capi.Tuning()
+    t.tile_w, t.tile_h, t.blocks_per_cu, t.groups = 8, 8, 0, groups
+    batch = Batch(bufs, W, H, tuning=t)
+    batch.launch()
+    batch.launch()
+    torch.cuda.synchronize()
+    for b in bufs:
+        ref = _plain_outputs(b)
+        checks = [("counts", b.counts), ("density", b.density), ("cells", b.cells), ("occupancy", b.occupancy),
+                  ("rgba", b.rgba)] + ([("tangents", b.tangents)] if b.tangents is not None else [])
+        for k, mine in checks:
+            assert torch.equal(mine, ref[k]), (k, b.n)
+    assert int((bufs[1].counts > 0).sum()) > 100_000
 
 
 def test_invalid_batch_arguments(batch32):

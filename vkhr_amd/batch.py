@@ -81,8 +81,9 @@ class StyleBuffers:
             D.build_volume(self.xyz, self.bounds, res, self.idx, self.tan, normalize=True, workspace=o["acc"],
                            out=vol)
         else:
-            D.voxelize(self.xyz, self.bounds, res, indices=self.idx, tangents=self.tan, want_sums=True,
-                       counts=o["counts"], sums=o["acc"], want_counts=o["counts"] is not None)
+            sums = o["acc"] is not None   # else counts only (no tangent volume)
+            D.voxelize(self.xyz, self.bounds, res, indices=self.idx, tangents=self.tan if sums else None,
+                       want_sums=sums, counts=o["counts"], sums=o["acc"], want_counts=o["counts"] is not None)
             call("vkhr_volume_finalize", ptr(o["counts"]), ptr(o["acc"]), self.n, self.n, self.n, 1,
                  ptr(o["density"]), ptr(o["tangents"]), None, ptr(o["cells"]), ptr(o.get("occupancy")),
                  capi.stream_of())

@@ -1,53 +1,48 @@
 // Device tables of the batched launch: many hair styles of uneven size,
-// voxelized, finalized and raymarched by one graph of grid-stride kernels
-// over chunk lists (and a raymarch over (style, tile) work items).
-// The per-segment, per-voxel and per-pixel bodies are the per-style ones
-// (voxel.hpp, volume.hpp, march.hpp), so the outputs are bit-identical to
-// per-style launches.
+// built into volumes by the LDS-binned passes of the one-call build (each
+// pass one launch over the concatenated workgroups of all styles) and
+// raymarched over (style, tile) work items, all in one graph.
+// The per-pair, per-brick, per-cell and per-pixel bodies are the per-style
+// ones (build.hpp, volume.hpp, march.hpp), so the outputs are bit-identical
+// to per-style launches.
 #pragma once
 
+#include "build.hpp"
 #include "march.hpp"
 
 namespace vkhr {
 
+// A style of the batch: the arguments of its build passes, and where its
+// cells pass writes the corner-packed cells and the normalized volume.
 struct BatchVolStyle {
-    float ox, oy, oz, vx, vy, vz, mx, my, mz;   // the VoxelGrid of the style
-    uint32_t nx, ny, nz;
-    const float* xyz;
-    const uint32_t* indices;
-    const float* tangents;
-    uint32_t n_vertices, n_pairs;
-    uint32_t* counts;
-    long long* acc;
-    uint8_t* density;
-    int8_t* tangent_q;
+    BuildArgs a;
     unsigned long long* cells;
-    uint32_t* occupancy;                        // nullable
+    uint8_t* density;
 };
 
-// A contiguous range of one style's voxels (or one cells_block of its cells).
+// A contiguous range of one style's lines (of one distance pass).
 struct BatchChunk {
     uint32_t style;
     uint32_t reserved;
     uint64_t begin, end;
 };
 
+// Workgroup offsets of one pass over the styles: style s owns workgroups
+// [prefix[s], prefix[s + 1]) of the pass's launch (n_styles + 1 entries).
+enum BatchPass { kPassInit, kPassBins, kPassBricks, kPassFinalize, kPassCells, kPassCount };
+
 struct BatchDevice {
-    BatchVolStyle* styles;
+    const BatchVolStyle* styles;
     RaymarchArgs* args;             // per style
-    uint64_t* pair_prefix;          // n_styles + 1
     uint32_t* tile_prefix;          // n_styles + 1
-    BatchChunk* voxel_chunks;
-    BatchChunk* cell_chunks;
+    uint32_t* block_prefix[kPassCount];
+    uint32_t blocks[kPassCount];    // workgroups of each pass (its prefix's last entry)
     BatchChunk* line_chunks[3];     // four lines of bricks per chunk, per distance pass
     uint32_t n_line_chunks[3];
     uint32_t march_lds_bytes;       // the largest distance field the raymarch stages
-    uint32_t* minmax;               // 2 per style
     uint32_t* work_counter;
     uint32_t n_styles;
-    uint32_t n_voxel_chunks, n_cell_chunks;
     uint32_t total_tiles;
-    uint64_t total_pairs;
     uint32_t tile_w, tile_h;        // pixels per wave tile (tile_w * tile_h == 64)
     uint32_t tiles_x;               // block tiles per image row
 };

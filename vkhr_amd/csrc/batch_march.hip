@@ -1,12 +1,10 @@
 // Batched raymarch over (style, tile) work items -- one block per item, or
 // persistent blocks pulling items from an atomic queue -- and the host side
 // of the batched launch: device tables per group of styles, one hipGraph
-// capturing clear -> voxelize -> finalize -> normalize -> cells -> raymarch
-// of every group (pipelined: a group's raymarch beside the next group's
-// volume passes), replayed by vkhr_batch_launch.
+// capturing the volume build passes and the raymarch of every group
+// (pipelined: a group's raymarch beside the next group's build), replayed
+// by vkhr_batch_launch.
 #include "batch.hpp"
-#include "volume.hpp"
-#include "voxel.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -114,19 +112,23 @@ void release(vkhr_batch* b) {
     b->allocations.clear();
 }
 
-// The device tables of styles[first, last): per style its voxel grid and
-// buffers, raymarch arguments, segment and tile prefixes, finalize and
-// cells chunk lists, and the group's own min/max slots and work counter.
+// The device tables of styles[first, last): per style its build arguments
+// (voxel grid, inputs, outputs, and its section of the group's build
+// state: bins, status, min/max slots, uint8 view, bin entries), raymarch
+// arguments, the workgroup offsets of every pass, tile prefix and distance
+// line chunks.
 int build_group(vkhr_batch* b, const vkhr_batch_style* styles, uint32_t first, uint32_t last, uint32_t width,
                 uint32_t height, const vkhr_tuning& t, BatchDevice* out) {
     const uint32_t n = last - first;
     std::vector<BatchVolStyle> vs(n);
     std::vector<RaymarchArgs> args(n);
-    std::vector<uint64_t> pair_prefix(n + 1, 0);
     std::vector<uint32_t> tile_prefix(n + 1, 0);
-    std::vector<BatchChunk> vchunks, cchunks, lchunks[3];
+    std::vector<uint32_t> prefix[kPassCount];
+    for (auto& p : prefix) p.assign(n + 1, 0);
+    std::vector<BatchChunk> lchunks[3];
+    std::vector<BuildLayout> layouts(n);
+    std::vector<size_t> layout_base(n + 1, 0);
     size_t lds_bytes = 0;
-    const uint64_t kChunk = 1u << 16;   // voxels per finalize chunk
     const uint32_t bw = 2 * t.tile_w, bh = 2 * t.tile_h;
     const uint32_t tiles_x = (width + bw - 1) / bw, tiles_y = (height + bh - 1) / bh;
     int rc = 0;
@@ -134,8 +136,9 @@ int build_group(vkhr_batch* b, const vkhr_batch_style* styles, uint32_t first, u
         const vkhr_batch_style& in = styles[first + s];
         VoxelGrid g;
         if ((rc = make_grid(&in.bounds, in.volume, in.volume, in.volume, &g))) return rc;
+        if ((rc = check_volume(in.volume, in.volume, in.volume, true, "batch style"))) return rc;
         if (!((in.counts || in.tangent_acc) && in.density && in.cells && in.rgba &&
-              (in.n_indices < 2 || (in.xyz && in.indices)))) {
+              (in.n_indices < 2 || (in.xyz && in.indices && in.n_vertices)))) {
             set_error("style %u: missing buffer", first + s);
             return VKHR_ERR_INVALID;
         }
@@ -146,32 +149,58 @@ int build_group(vkhr_batch* b, const vkhr_batch_style* styles, uint32_t first, u
                       first + s);
             return VKHR_ERR_INVALID;
         }
-        BatchVolStyle& v = vs[s];
-        v.ox = g.ox; v.oy = g.oy; v.oz = g.oz; v.vx = g.vx; v.vy = g.vy; v.vz = g.vz;
-        v.mx = g.mx; v.my = g.my; v.mz = g.mz; v.nx = g.nx; v.ny = g.ny; v.nz = g.nz;
-        v.xyz = in.xyz;
-        v.indices = in.indices;
-        v.tangents = in.tangents;
-        v.n_vertices = in.n_vertices;
-        v.n_pairs = in.n_indices / 2;
-        v.counts = in.counts;
-        v.acc = reinterpret_cast<long long*>(in.tangent_acc);
-        v.density = in.density;
-        v.tangent_q = in.tangent_i8x4;
-        v.cells = reinterpret_cast<unsigned long long*>(in.cells);
-        v.occupancy = in.occupancy;
-        if (v.acc && !(v.tangents && v.tangent_q)) {
+        if (in.tangent_acc && !(in.tangents && in.tangent_i8x4)) {
             set_error("style %u: tangent_acc needs tangents and tangent_i8x4", first + s);
             return VKHR_ERR_INVALID;
         }
-        pair_prefix[s + 1] = pair_prefix[s] + v.n_pairs;
-        uint64_t nvox = uint64_t(in.volume) * in.volume * in.volume;
-        for (uint64_t a = 0; a < nvox; a += kChunk) vchunks.push_back({s, 0, a, std::min(nvox, a + kChunk)});
-        const uint64_t cb = cell_blocks(in.volume, in.volume, in.volume);
-        for (uint64_t k = 0; k < cb; ++k) cchunks.push_back({s, 0, k, k + 1});
-        if (in.occupancy && distance_field_fits(in.volume, in.volume, in.volume))
+        if (in.tangent_i8x4 && !in.tangent_acc) {
+            set_error("style %u: tangent_i8x4 needs tangent_acc", first + s);
+            return VKHR_ERR_INVALID;
+        }
+        const uint32_t n_pairs = in.n_indices / 2;  // the reference loops while i < size-1
+        const BrickGrid bg = brick_grid(g.nx, g.ny, g.nz);
+        const size_t n_bricks = size_t(bg.bx) * bg.by * bg.bz;
+        const size_t nvox = size_t(g.nx) * g.ny * g.nz;
+        const size_t capacity = build_capacity(n_pairs);
+        layouts[s] = build_layout(n_bricks, nvox, true, capacity);
+        layout_base[s + 1] = layout_base[s] + layouts[s].words;
+        BuildArgs& a = vs[s].a;
+        std::memset(&a, 0, sizeof(a));
+        a.g = g;
+        a.b = bg;
+        a.xyz = in.xyz;
+        a.indices = in.indices;
+        a.tangents = in.tangents;
+        a.n_vertices = in.n_vertices;
+        a.n_pairs = n_pairs;
+        a.n_bricks = uint32_t(n_bricks);
+        a.pairs_per_block = build_pairs_per_block(n_pairs);
+        a.capacity = uint32_t(capacity);
+        a.occupancy = in.occupancy;
+        a.occupancy_words = in.occupancy ? occupancy_words(g.nx, g.ny, g.nz) : 0;
+        a.counts = in.counts;
+        a.tangent_i8x4 = in.tangent_i8x4;
+        a.acc = reinterpret_cast<long long*>(in.tangent_acc);
+        a.acc_counts = in.tangent_acc ? nullptr : in.counts;
+        vs[s].cells = reinterpret_cast<unsigned long long*>(in.cells);
+        vs[s].density = in.density;
+        const uint32_t blocks[kPassCount] = {
+            grid_for(std::max(n_bricks, a.occupancy_words), 256, 1024),                           // init
+            n_pairs ? (n_pairs + a.pairs_per_block - 1) / a.pairs_per_block : 0u,                 // bins
+            brick_blocks(bg),                                                                     // bricks
+            grid_for(nvox, 256, 128),                                                             // finalize
+            cell_blocks(g.nx, g.ny, g.nz)};                                                       // cells
+        for (int p = 0; p < kPassCount; ++p) {
+            const uint64_t next = uint64_t(prefix[p][s]) + blocks[p];
+            if (next >= (uint64_t(1) << 31)) {
+                set_error("batch group: more than 2^31 workgroups in one pass");
+                return VKHR_ERR_OVERFLOW;
+            }
+            prefix[p][s + 1] = uint32_t(next);
+        }
+        if (in.occupancy && distance_field_fits(g.nx, g.ny, g.nz))
             for (int axis = 0; axis < 3; ++axis) {
-                const uint32_t lines = distance_lines(in.volume, in.volume, in.volume, axis);
+                const uint32_t lines = distance_lines(g.nx, g.ny, g.nz, axis);
                 for (uint32_t l = 0; l < lines; l += 256 / kWave)
                     lchunks[axis].push_back({s, 0, l, std::min<uint64_t>(lines, l + 256 / kWave)});
             }
@@ -188,26 +217,40 @@ int build_group(vkhr_batch* b, const vkhr_batch_style* styles, uint32_t first, u
         lds_bytes = std::max(lds_bytes, distance_lds_bytes(args[s].vol));
         tile_prefix[s + 1] = tile_prefix[s] + tiles_x * tiles_y;
     }
+    // the build state of every style of the group, one allocation; the
+    // work counter of the persistent raymarch after it
+    void* state = nullptr;
+    VKHR_CHECK_HIP(hipMalloc(&state, (layout_base[n] + 64) * sizeof(uint32_t)));
+    b->allocations.push_back(state);
+    uint32_t* words = static_cast<uint32_t*>(state);
+    for (uint32_t s = 0; s < n; ++s) {
+        uint32_t* w = words + layout_base[s];
+        const BuildLayout& l = layouts[s];
+        BuildArgs& a = vs[s].a;
+        a.bins = w + l.bins;
+        a.cursor = w + l.cursor;
+        a.status = w + l.status;
+        a.minmax = w + l.minmax;
+        a.density = reinterpret_cast<uint8_t*>(w + l.raw);
+        a.entries = reinterpret_cast<uint2*>(w + l.entries);
+    }
     BatchDevice& d = *out;
     std::memset(&d, 0, sizeof(d));
-    std::vector<uint32_t> zeros(2 * n + 1, 0);
-    uint32_t* mm = nullptr;
-    if ((rc = upload(b, vs, &d.styles)) || (rc = upload(b, args, &d.args)) ||
-        (rc = upload(b, pair_prefix, &d.pair_prefix)) || (rc = upload(b, tile_prefix, &d.tile_prefix)) ||
-        (rc = upload(b, vchunks, &d.voxel_chunks)) || (rc = upload(b, cchunks, &d.cell_chunks)) ||
-        (rc = upload(b, lchunks[0], &d.line_chunks[0])) || (rc = upload(b, lchunks[1], &d.line_chunks[1])) ||
-        (rc = upload(b, lchunks[2], &d.line_chunks[2])) ||
-        (rc = upload(b, zeros, &mm)))
+    BatchVolStyle* dev_styles = nullptr;
+    if ((rc = upload(b, vs, &dev_styles)) || (rc = upload(b, args, &d.args)) ||
+        (rc = upload(b, tile_prefix, &d.tile_prefix)) || (rc = upload(b, lchunks[0], &d.line_chunks[0])) ||
+        (rc = upload(b, lchunks[1], &d.line_chunks[1])) || (rc = upload(b, lchunks[2], &d.line_chunks[2])))
         return rc;
-    d.minmax = mm;
-    d.work_counter = mm + 2 * n;
+    for (int p = 0; p < kPassCount; ++p) {
+        if ((rc = upload(b, prefix[p], &d.block_prefix[p]))) return rc;
+        d.blocks[p] = prefix[p][n];
+    }
+    d.styles = dev_styles;
+    d.work_counter = words + layout_base[n];
     d.n_styles = n;
-    d.n_voxel_chunks = uint32_t(vchunks.size());
-    d.n_cell_chunks = uint32_t(cchunks.size());
     for (int axis = 0; axis < 3; ++axis) d.n_line_chunks[axis] = uint32_t(lchunks[axis].size());
     d.march_lds_bytes = uint32_t(lds_bytes);
     d.total_tiles = tile_prefix[n];
-    d.total_pairs = pair_prefix[n];
     d.tile_w = t.tile_w;
     d.tile_h = t.tile_h;
     d.tiles_x = tiles_x;

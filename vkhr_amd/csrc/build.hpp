@@ -1,0 +1,514 @@
+// Device side of the LDS-binned volume build (HairStyle::voxelize_segments,
+// src/vkhr/scene_graph/hair_style.cc:296-342, then Volume::normalize,
+// :344-357), shared by the one-call build of one style and the batched
+// build of many.
+//
+// Bricks of 8x8x8 voxels.  The index pairs are binned by the bricks their
+// samples visit -- a counting sort: count, scan, fill -- then one workgroup
+// per brick walks the DDA of every pair in its bin and adds the samples
+// that fall inside its brick into an LDS accumulator (count and fixed-point
+// tangent sums: integer adds, so the order of the adds does not matter).
+// The workgroup finalizes its brick straight from LDS: the saturating uint8
+// view, the quantized tangent (the sums and the expression of
+// vkhr_volume_finalize, so both give the same bits), the counts when asked
+// for, and the min/max of the uint8 view.  The corner-packed cells pass
+// then applies the normalize as it loads the uint8 view (writing the
+// normalized volume and the float view as it goes) and sets the brick
+// occupancy.
+//
+// The count and fill passes give each workgroup a contiguous range of
+// pairs and count the runs of its range in an LDS hash table first, so a
+// brick takes one device-wide atomic per workgroup that reaches it rather
+// than one per run (keys that find no free slot fall back to a device-wide
+// atomic per run).
+//
+// A pair's voxel coordinates are monotone in its sample index (the root is
+// advanced by a fixed direction and the clamps are monotone), so the bricks
+// it visits form runs that never repeat: a pair lands in a bin at most once.
+//
+// If the pairs visit more bricks than the entry capacity (long segments), a
+// flag set on the device by the scan switches the build to the workspace
+// path: the fill pass zeroes the workspace instead, the brick pass
+// voxelizes into it with the device-wide atomics of vkhr_voxelize_segments,
+// and one more pass (which returns at once while the flag is clear)
+// finalizes from it, so the host never waits for the count.
+//
+// Every body takes its workgroup's index within the style's own launch
+// (block) and that launch's size (blocks), so one style's passes run as
+// plain kernels and many styles' as one launch over the concatenated
+// blocks.  Compiled with -ffp-contract=off: the DDA and the quantization
+// must round as the reference's float arithmetic does.
+#pragma once
+
+#include "volume.hpp"
+#include "voxel.hpp"
+
+namespace vkhr {
+
+constexpr uint32_t kBuildBlock = 256;
+constexpr uint32_t kBrickBlock = 128;
+constexpr uint32_t kBuildNone = 0xffffffffu;
+
+// Bricks of the voxel grid (8 voxels a side, the last ones partial).
+struct BrickGrid {
+    uint32_t bx, by, bz;
+};
+
+__host__ __device__ __forceinline__ BrickGrid brick_grid(uint32_t nx, uint32_t ny, uint32_t nz) {
+    return {(nx + 7u) >> 3, (ny + 7u) >> 3, (nz + 7u) >> 3};
+}
+
+__device__ __forceinline__ uint32_t brick_index(const BrickGrid& b, uint3 v) {
+    return (v.x >> 3) + b.bx * ((v.y >> 3) + b.by * (v.z >> 3));
+}
+
+// Where the build keeps its state (32-bit words from one base).
+struct BuildLayout {
+    size_t bins;        // bin sizes, one per brick
+    size_t cursor;      // bin starts, then (after the fill) bin ends
+    size_t status;      // [0] total entries, [1] overflow flag
+    size_t minmax;      // the min/max slots (kMinMaxBytes)
+    size_t raw;         // the uint8 view before the normalize (with cells), n voxels
+    size_t entries;     // bin entries: the pairs' vertex indices (uint2)
+    size_t words;
+};
+
+__host__ __forceinline__ BuildLayout build_layout(size_t n_bricks, size_t n_voxels, bool raw, size_t capacity) {
+    auto up = [](size_t w) { return (w + 63) & ~size_t(63); };  // 256-byte aligned sections
+    BuildLayout l;
+    l.bins = 0;
+    l.cursor = up(n_bricks);
+    l.status = l.cursor + up(n_bricks);
+    l.minmax = l.status + 64;
+    l.raw = l.minmax + up(kMinMaxBytes / 4);
+    l.entries = l.raw + (raw ? up((n_voxels + 3) / 4) : 0);
+    l.words = l.entries + up(2 * capacity);
+    return l;
+}
+
+// Bin entries per style: 4 per pair plus 64 Ki, at most 2^28.
+__host__ __forceinline__ size_t build_capacity(uint32_t n_pairs) {
+    const size_t c = 4 * size_t(n_pairs) + 65536;
+    return c < (size_t(1) << 28) ? c : (size_t(1) << 28);
+}
+
+// Pairs per bin workgroup: ranges of 256 .. 1024 (kPairsPerThread per
+// thread), about 2048 workgroups when that fits.
+__host__ __forceinline__ uint32_t build_pairs_per_block(uint32_t n_pairs) {
+    const uint32_t want = (n_pairs + 2047) / 2048;
+    return want < kBuildBlock ? kBuildBlock : (want > 4 * kBuildBlock ? 4 * kBuildBlock : want);
+}
+
+struct BuildArgs {
+    VoxelGrid g;
+    BrickGrid b;
+    const float* xyz;
+    const uint32_t* indices;
+    const float* tangents;          // per vertex (null: no tangent volume)
+    uint32_t n_vertices, n_pairs;
+    uint32_t n_bricks;
+    uint32_t pairs_per_block;       // the contiguous range of a bin workgroup
+    uint32_t* bins;
+    uint32_t* cursor;
+    uint32_t* status;
+    uint32_t* minmax;
+    uint2* entries;
+    uint32_t capacity;
+    uint32_t* occupancy;            // zeroed by the init pass (nullable)
+    size_t occupancy_words;
+    uint32_t* counts;               // outputs (counts and tangents nullable)
+    uint8_t* density;               // the uint8 view the brick pass writes (raw, or the output)
+    int8_t* tangent_i8x4;
+    long long* acc;                 // workspace of the overflow path: count + tangent sums, 16 B per voxel,
+    uint32_t* acc_counts;           // or (acc null, no tangent volume) the counts alone
+};
+
+// The words the overflow path zeroes and accumulates into.
+__host__ __device__ __forceinline__ size_t overflow_words(const BuildArgs& a, size_t n_voxels) {
+    return a.acc ? 4 * n_voxels : n_voxels;
+}
+
+// Zeroes the bin sizes, the status and the occupancy, and empties the
+// min/max slots.
+__device__ __forceinline__ void build_init_body(const BuildArgs& a, uint32_t block, uint32_t blocks) {
+    const size_t stride = size_t(blocks) * blockDim.x;
+    const size_t first = size_t(block) * blockDim.x + threadIdx.x;
+    for (size_t i = first; i < a.n_bricks; i += stride) a.bins[i] = 0u;
+    for (size_t i = first; i < a.occupancy_words; i += stride) a.occupancy[i] = 0u;
+    if (block == 0) {
+        if (threadIdx.x < 2) a.status[threadIdx.x] = 0u;
+        if (threadIdx.x < kMinMaxSlots) {
+            a.minmax[threadIdx.x * kMinMaxStride] = 255u;
+            a.minmax[threadIdx.x * kMinMaxStride + 1] = 0u;
+        }
+    }
+}
+
+// --- the bin passes
+
+// An LDS hash table of the bricks one workgroup's pairs reach: open
+// addressing over kTableSlots slots, kTableProbes probes.  A key that finds
+// no free slot among its probes never will (slots are only ever filled),
+// so every lookup of it fails alike and its runs take the device-wide path.
+constexpr uint32_t kTableBits = 10, kTableSlots = 1u << kTableBits, kTableProbes = 16;
+
+struct BinTable {
+    uint32_t key[kTableSlots];
+    uint32_t count[kTableSlots];
+    uint32_t base[kTableSlots];
+};
+
+__device__ __forceinline__ uint32_t table_home(uint32_t k) { return (k * 2654435761u) >> (32 - kTableBits); }
+
+__device__ __forceinline__ uint32_t table_insert(BinTable& t, uint32_t k) {
+    const uint32_t h = table_home(k);
+    for (uint32_t p = 0; p < kTableProbes; ++p) {
+        const uint32_t slot = (h + p) & (kTableSlots - 1u);
+        const uint32_t prev = atomicCAS(&t.key[slot], kBuildNone, k);
+        if (prev == kBuildNone || prev == k) return slot;
+    }
+    return kBuildNone;
+}
+
+__device__ __forceinline__ uint32_t table_find(const BinTable& t, uint32_t k) {
+    const uint32_t h = table_home(k);
+    for (uint32_t p = 0; p < kTableProbes; ++p) {
+        const uint32_t slot = (h + p) & (kTableSlots - 1u);
+        const uint32_t key = t.key[slot];
+        if (key == k) return slot;
+        if (key == kBuildNone) return kBuildNone;
+    }
+    return kBuildNone;
+}
+
+__device__ __forceinline__ void table_clear(BinTable& t) {
+    for (uint32_t i = threadIdx.x; i < kTableSlots; i += blockDim.x) {
+        t.key[i] = kBuildNone;
+        t.count[i] = 0u;
+    }
+}
+
+// Calls run(pair's vertex indices, brick) for each brick run of each pair
+// of workgroup block's range (at most kPairsPerThread per thread: the index
+// loads of all of a thread's pairs are issued before the first walk).
+constexpr uint32_t kPairsPerThread = 4;
+
+template <class Run>
+__device__ __forceinline__ void walk_runs(const BuildArgs& a, uint32_t block, Run run) {
+    const uint32_t r0 = block * a.pairs_per_block;
+    const uint32_t r1 = min(a.n_pairs, r0 + a.pairs_per_block);
+    uint2 pr[kPairsPerThread];
+#pragma unroll
+    for (uint32_t j = 0; j < kPairsPerThread; ++j) {
+        const uint32_t s = r0 + threadIdx.x + j * blockDim.x;
+        pr[j] = s < r1 ? make_uint2(a.indices[2u * s], a.indices[2u * s + 1u]) : make_uint2(kBuildNone, kBuildNone);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kPairsPerThread; ++j) {
+        SegmentDda w = segment_dda(a.g, a.xyz, a.n_vertices, pr[j].x, pr[j].y);  // kBuildNone: no samples
+        uint32_t last = kBuildNone;
+        while (w.steps > 0.0f) {
+            w.steps -= 1.0f;
+            const uint32_t k = brick_index(a.b, voxel_coords(a.g, w.rx, w.ry, w.rz));
+            if (k != last) run(pr[j], k);
+            last = k;
+            w.advance();
+        }
+    }
+}
+
+// The bin sizes: runs counted per brick in the table, then one add per
+// table entry.
+__device__ __forceinline__ void bin_count_body(const BuildArgs& a, BinTable& t, uint32_t block) {
+    table_clear(t);
+    __syncthreads();
+    walk_runs(a, block, [&](uint2, uint32_t k) {
+        const uint32_t slot = table_insert(t, k);
+        if (slot != kBuildNone)
+            atomicAdd(&t.count[slot], 1u);
+        else
+            atomicAdd(&a.bins[k], 1u);
+    });
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kTableSlots; i += blockDim.x)
+        if (t.key[i] != kBuildNone) atomicAdd(&a.bins[t.key[i]], t.count[i]);
+}
+
+// The bin entries: the same counts in the table, one range reserved per
+// table entry in its brick's bin, then a second walk that places each run
+// in its range (table misses take a slot of their own).  With the overflow
+// flag set the workgroups zero the workspace instead.
+__device__ __forceinline__ void bin_fill_body(const BuildArgs& a, BinTable& t, uint32_t block, uint32_t blocks,
+                                              size_t n_voxels) {
+    if (a.status[1]) {
+        uint32_t* acc = a.acc ? reinterpret_cast<uint32_t*>(a.acc) : a.acc_counts;
+        const size_t words = overflow_words(a, n_voxels);
+        const size_t stride = size_t(blocks) * blockDim.x;
+        for (size_t i = size_t(block) * blockDim.x + threadIdx.x; i < words; i += stride) acc[i] = 0u;
+        return;
+    }
+    table_clear(t);
+    __syncthreads();
+    walk_runs(a, block, [&](uint2, uint32_t k) {
+        const uint32_t slot = table_insert(t, k);
+        if (slot != kBuildNone) atomicAdd(&t.count[slot], 1u);
+    });
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kTableSlots; i += blockDim.x) {
+        if (t.key[i] != kBuildNone) t.base[i] = atomicAdd(&a.cursor[t.key[i]], t.count[i]);
+        t.count[i] = 0u;
+    }
+    __syncthreads();
+    walk_runs(a, block, [&](uint2 pair, uint32_t k) {
+        const uint32_t slot = table_find(t, k);
+        const uint32_t pos =
+            slot != kBuildNone ? t.base[slot] + atomicAdd(&t.count[slot], 1u) : atomicAdd(&a.cursor[k], 1u);
+        if (pos < a.capacity) a.entries[pos] = pair;
+    });
+}
+
+// Exclusive scan of the bin sizes into the bin starts (one workgroup of
+// kScanThreads, 16 sizes per thread per round), the total, and the
+// overflow flag.
+constexpr uint32_t kScanThreads = 1024;
+
+struct ScanLds {
+    unsigned long long wave_sums[kScanThreads / kWave];
+    unsigned long long carry;  // 64-bit: the total of absurd inputs may pass 2^32
+};
+
+__device__ __forceinline__ void scan_body(const BuildArgs& a, ScanLds& l) {
+    constexpr uint32_t kPer = 16;
+    const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const uint32_t n = a.n_bricks;
+    if (threadIdx.x == 0) l.carry = 0;
+    __syncthreads();
+    for (uint32_t chunk = 0; chunk < n; chunk += kPer * kScanThreads) {
+        const uint32_t i0 = chunk + kPer * threadIdx.x;
+        uint32_t v[kPer];
+        if (i0 + kPer <= n) {  // the sections are 256-byte aligned and i0 a multiple of 16
+#pragma unroll
+            for (uint32_t q = 0; q < kPer / 4; ++q) {
+                const uint4 u = reinterpret_cast<const uint4*>(a.bins + i0)[q];
+                v[4 * q] = u.x; v[4 * q + 1] = u.y; v[4 * q + 2] = u.z; v[4 * q + 3] = u.w;
+            }
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < kPer; ++j) v[j] = i0 + j < n ? a.bins[i0 + j] : 0u;
+        }
+        unsigned long long sum = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) sum += v[j];
+        // inclusive wave scan of the per-thread sums
+        unsigned long long inc = sum;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const unsigned long long t = __shfl_up(inc, o, kWave);
+            if (int(lane) >= o) inc += t;
+        }
+        if (lane == kWave - 1) l.wave_sums[wave] = inc;
+        __syncthreads();
+        unsigned long long before = l.carry;
+        for (uint32_t w = 0; w < wave; ++w) before += l.wave_sums[w];
+        before += inc - sum;
+        // a wrapped start only happens with the overflow flag set, and then
+        // no bin is read
+        uint32_t out[kPer];
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) {
+            out[j] = uint32_t(before);
+            before += v[j];
+        }
+        if (i0 + kPer <= n) {
+#pragma unroll
+            for (uint32_t q = 0; q < kPer / 4; ++q)
+                reinterpret_cast<uint4*>(a.cursor + i0)[q] =
+                    make_uint4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
+        } else {
+#pragma unroll
+            for (uint32_t j = 0; j < kPer; ++j)
+                if (i0 + j < n) a.cursor[i0 + j] = out[j];
+        }
+        __syncthreads();
+        if (threadIdx.x == kScanThreads - 1) l.carry = before;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.status[0] = uint32_t(l.carry < 0xffffffffull ? l.carry : 0xffffffffull);
+        a.status[1] = l.carry > a.capacity ? 1u : 0u;
+    }
+}
+
+// Brick of a workgroup: groups of 8 consecutive bricks along x go to the
+// same XCD (workgroups are dealt to the 8 XCDs in turn), so the rows of a
+// group's output are written through one L2.  Bricks past the grid
+// (groups padded to 8 along x) return kBuildNone.
+__device__ __forceinline__ uint32_t brick_of_block(const BrickGrid& b, uint32_t block) {
+    const uint32_t xcd = block & 7u, j = block >> 3;
+    const uint32_t group = (j >> 3) * 8u + xcd, within = j & 7u;
+    const uint32_t gx = (b.bx + 7u) >> 3;  // groups along x
+    const uint32_t row = group / gx, x = (group % gx) * 8u + within;
+    if (x >= b.bx || row >= b.by * b.bz) return kBuildNone;
+    return x + b.bx * row;
+}
+
+// Workgroups of the brick pass: the groups rounded up to a multiple of 8,
+// so that brick_of_block reaches every group (a multiple of 64).
+__host__ __forceinline__ uint32_t brick_blocks(const BrickGrid& b) {
+    const uint32_t groups = ((b.bx + 7u) >> 3) * b.by * b.bz;
+    return ((groups + 7u) & ~7u) * 8u;
+}
+
+// One brick per workgroup: the samples of its bin's pairs that fall inside
+// it, accumulated in LDS, then the brick's outputs.  The tangent sums are
+// 32-bit: at 2^16 per unit they hold up to 32767 contributions exactly,
+// and a voxel of more gets tangent 0 (as from the packed accumulator), so
+// the quantized tangents are those of vkhr_volume_finalize.  With the
+// overflow flag set the workgroups voxelize every pair into the workspace
+// instead.  block is a multiple-of-8 offset from the launch's own block
+// index, so block & 7 is the XCD.
+struct BrickLds {
+    uint32_t count[512];
+    int sum[3][512];
+    uint32_t red[2][kBrickBlock / kWave];
+};
+
+__device__ __forceinline__ void brick_body(const BuildArgs& a, BrickLds& l, uint32_t block, uint32_t blocks) {
+    constexpr uint32_t kVox = 512;
+    if (a.status[1]) {
+        // grid-stride over wave-aligned ranges (voxelize_segment is wave-cooperative)
+        const float* tangents = a.tangent_i8x4 ? a.tangents : a.xyz;  // sums unused without a tangent volume
+        const uint32_t stride = blocks * blockDim.x;
+        for (uint32_t base = block * blockDim.x + (threadIdx.x & ~uint32_t(kWave - 1)); base < a.n_pairs;
+             base += stride) {
+            const uint32_t s = base + (threadIdx.x & uint32_t(kWave - 1));
+            const bool has = s < a.n_pairs;
+            voxelize_segment(a.g, a.xyz, a.n_vertices, has, has ? a.indices[2u * s] : 0u,
+                             has ? a.indices[2u * s + 1u] : 0u, tangents, a.acc ? nullptr : a.acc_counts, a.acc);
+        }
+        return;
+    }
+    const uint32_t brick = brick_of_block(a.b, block);
+    if (brick == kBuildNone) return;
+    for (uint32_t v = threadIdx.x; v < kVox; v += kBrickBlock) {
+        l.count[v] = 0u;
+        l.sum[0][v] = l.sum[1][v] = l.sum[2][v] = 0;
+    }
+    const uint32_t bxo = brick % a.b.bx, byo = (brick / a.b.bx) % a.b.by, bzo = brick / (a.b.bx * a.b.by);
+    __syncthreads();
+    const uint32_t end = a.cursor[brick], n = a.bins[brick];
+    const bool with_tangents = a.tangent_i8x4 != nullptr;
+    for (uint32_t e = end - n + threadIdx.x; e < end; e += kBrickBlock) {
+        const uint2 pair = a.entries[e];
+        SegmentDda w = segment_dda(a.g, a.xyz, a.n_vertices, pair.x, pair.y);
+        int tx = 0, ty = 0, tz = 0;
+        if (with_tangents && w.steps > 0.0f) {
+            const float3 t = load_xyz(a.tangents, pair.x);
+            tx = int(tangent_fixed(t.x));
+            ty = int(tangent_fixed(t.y));
+            tz = int(tangent_fixed(t.z));
+        }
+        bool entered = false;
+        while (w.steps > 0.0f) {
+            w.steps -= 1.0f;
+            const uint3 v = voxel_coords(a.g, w.rx, w.ry, w.rz);
+            const bool inside = (v.x >> 3) == bxo && (v.y >> 3) == byo && (v.z >> 3) == bzo;
+            if (inside) {
+                const uint32_t i = (v.x & 7u) + 8u * ((v.y & 7u) + 8u * (v.z & 7u));
+                atomicAdd(&l.count[i], 1u);
+                if (with_tangents) {  // wraps only past 32767 contributions, where the tangent is 0
+                    atomicAdd(reinterpret_cast<unsigned*>(&l.sum[0][i]), unsigned(tx));
+                    atomicAdd(reinterpret_cast<unsigned*>(&l.sum[1][i]), unsigned(ty));
+                    atomicAdd(reinterpret_cast<unsigned*>(&l.sum[2][i]), unsigned(tz));
+                }
+            } else if (entered) {
+                break;  // the brick run is over: the pair never comes back
+            }
+            entered = entered || inside;
+            w.advance();
+        }
+    }
+    __syncthreads();
+    uint32_t lo = 255u, hi = 0u;
+    for (uint32_t i = threadIdx.x; i < kVox; i += kBrickBlock) {
+        const uint32_t x = bxo * 8u + (i & 7u), y = byo * 8u + ((i >> 3) & 7u), z = bzo * 8u + (i >> 6);
+        if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz) continue;
+        const size_t v = size_t(x) + size_t(a.g.nx) * (size_t(y) + size_t(a.g.ny) * z);
+        const uint32_t c = l.count[i];
+        const uint32_t d = c < 255u ? c : 255u;
+        a.density[v] = uint8_t(d);
+        if (a.counts) a.counts[v] = c;
+        if (with_tangents)
+            reinterpret_cast<char4*>(a.tangent_i8x4)[v] = quantize_tangent_sums(l.sum[0][i], l.sum[1][i], l.sum[2][i], c);
+        lo = d < lo ? d : lo;
+        hi = d > hi ? d : hi;
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const uint32_t p = __shfl_xor(lo, o, kWave), q = __shfl_xor(hi, o, kWave);
+        lo = p < lo ? p : lo;
+        hi = q > hi ? q : hi;
+    }
+    const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    if (lane == 0) {
+        l.red[0][wave] = lo;
+        l.red[1][wave] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < kBrickBlock / kWave; ++w) {
+            lo = l.red[0][w] < lo ? l.red[0][w] : lo;
+            hi = l.red[1][w] > hi ? l.red[1][w] : hi;
+        }
+        if (lo <= hi)  // the brick holds a voxel of the volume
+            fold_minmax(a.minmax + (block % kMinMaxSlots) * kMinMaxStride, lo, hi);
+    }
+}
+
+// The finalize of the workspace path (returns at once while the flag is
+// clear): the outputs of the brick pass from the workspace.
+__device__ __forceinline__ void overflow_finalize_body(const BuildArgs& a, size_t n, uint32_t block, uint32_t blocks) {
+    if (!a.status[1]) return;
+    uint32_t lo = 255u, hi = 0u;
+    const size_t stride = size_t(blocks) * blockDim.x;
+    const size_t first = size_t(block) * blockDim.x + threadIdx.x;
+    for (size_t i = first; i < n; i += stride) {
+        long long w0 = 0, w1 = 0;
+        uint32_t c;
+        if (a.acc) {
+            w0 = a.acc[2 * i];
+            w1 = a.acc[2 * i + 1];
+            c = uint32_t((unsigned long long)(w1) & 0xffffffffull);
+        } else {
+            c = a.acc_counts[i];
+        }
+        const uint32_t d = c < 255u ? c : 255u;
+        a.density[i] = uint8_t(d);
+        if (a.counts) a.counts[i] = c;
+        if (a.tangent_i8x4) reinterpret_cast<char4*>(a.tangent_i8x4)[i] = quantize_tangent_words(w0, w1, c);
+        lo = d < lo ? d : lo;
+        hi = d > hi ? d : hi;
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        const uint32_t p = __shfl_xor(lo, o, kWave), q = __shfl_xor(hi, o, kWave);
+        lo = p < lo ? p : lo;
+        hi = q > hi ? q : hi;
+    }
+    if (threadIdx.x % kWave == 0 && lo <= hi)
+        fold_minmax(a.minmax + (block % kMinMaxSlots) * kMinMaxStride, lo, hi);
+}
+
+// The map of every uint8 value of the normalize (from the min/max slots)
+// into lut; returns whether it is the identity (no normalize, or a volume
+// spanning 0..255), for which the cells pass needs no lookups.  All
+// threads of the workgroup call it.
+__device__ __forceinline__ bool build_lut(const uint32_t* minmax, bool normalize, uint8_t* lut) {
+    const DensityMap m = density_map_slots(minmax, normalize);
+    bool identity = true;
+    for (uint32_t v = threadIdx.x; v < 256u; v += blockDim.x) {
+        const uint32_t d = texel_density(v, m);
+        lut[v] = uint8_t(d);
+        identity = identity && d == v;
+    }
+    return __syncthreads_and(identity);
+}
+
+}  // namespace vkhr
