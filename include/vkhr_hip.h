@@ -298,6 +298,8 @@ int vkhr_volume_finalize_noalloc(const uint32_t* counts, const int64_t* tangent_
  * afterwards) is used only when the pairs visit more bricks than that
  * (long segments): a flag set on the device then switches to the packed
  * accumulator path of vkhr_voxelize_segments, with the same results.
+ * density_u8 is 4-byte and counts, tangent_i8x4 and density_f32 16-byte
+ * aligned, as for vkhr_volume_finalize.
  * Same limits as vkhr_voxelize_segments (nx*ny*nz <= 2^24). */
 int vkhr_volume_build_segments(const float* xyz, uint32_t n_vertices,
                                const uint32_t* indices, uint32_t n_indices,

@@ -189,7 +189,7 @@ int build_group(vkhr_batch* b, const vkhr_batch_style* styles, uint32_t first, u
             n_pairs ? (n_pairs + a.pairs_per_block - 1) / a.pairs_per_block : 0u,                 // bins
             brick_blocks(bg),                                                                     // bricks
             grid_for(nvox, 256, 128),                                                             // finalize
-            cell_blocks(g.nx, g.ny, g.nz)};                                                       // cells
+            cells_pass_blocks(g.nx, g.ny, g.nz)};                                                 // cells
         for (int p = 0; p < kPassCount; ++p) {
             const uint64_t next = uint64_t(prefix[p][s]) + blocks[p];
             if (next >= (uint64_t(1) << 31)) {
@@ -232,7 +232,7 @@ int build_group(vkhr_batch* b, const vkhr_batch_style* styles, uint32_t first, u
         a.status = w + l.status;
         a.minmax = w + l.minmax;
         a.density = reinterpret_cast<uint8_t*>(w + l.raw);
-        a.entries = reinterpret_cast<uint2*>(w + l.entries);
+        a.entries = reinterpret_cast<BinEntry*>(w + l.entries);
     }
     BatchDevice& d = *out;
     std::memset(&d, 0, sizeof(d));

@@ -1,10 +1,10 @@
 // Batched volume build of many styles: the passes of the one-call build
 // (build.hpp) -- init, bin count, scan, bin fill, bricks, the workspace
 // finalize, cells -- each one launch over the concatenated workgroups of
-// all styles, then the brick distance field of every style.  Compiled with
-// -ffp-contract=off like the per-style path: the same DDA, brick and cells
-// bodies, so counts, tangents, densities and cells are bit-identical to
-// per-style launches.
+// all styles, then the occupancy bitmask and brick distance field of every
+// style.  Compiled with -ffp-contract=off like the per-style path: the same
+// DDA, brick and cells bodies, so counts, tangents, densities and cells are
+// bit-identical to per-style launches.
 #include "batch.hpp"
 
 namespace vkhr {
@@ -86,7 +86,7 @@ batch_overflow_finalize_kernel(BatchDevice d, const BatchVolStyle* __restrict__ 
 
 // The corner-packed cells with the normalize applied as the uint8 view
 // loads (the style's map from its min/max slots, built per workgroup),
-// writing the normalized volume; one cells_block per workgroup.
+// writing the normalized volume; one workgroup of cells_pass each.
 __global__ void __launch_bounds__(256)
 batch_cells_kernel(BatchDevice d, const BatchVolStyle* __restrict__ styles) {
     __shared__ uint8_t lut[256];
@@ -96,9 +96,19 @@ batch_cells_kernel(BatchDevice d, const BatchVolStyle* __restrict__ styles) {
     unsigned long long* cells = st.cells;
     uint8_t* density = st.density;
     if (build_lut(a.minmax, true, lut))
-        cells_block<1>(a.density, a.g.nx, a.g.ny, a.g.nz, w.block, cells, a.occupancy, nullptr, density);
+        cells_pass<1>(a.density, a.g.nx, a.g.ny, a.g.nz, w.block, cells, a.occupancy, nullptr, density);
     else
-        cells_block<2>(a.density, a.g.nx, a.g.ny, a.g.nz, w.block, cells, a.occupancy, lut, density);
+        cells_pass<2>(a.density, a.g.nx, a.g.ny, a.g.nz, w.block, cells, a.occupancy, lut, density);
+}
+
+// The bitmask of every style's occupancy from its brick flags: one
+// workgroup per style.
+__global__ void __launch_bounds__(256)
+batch_pack_bits_kernel(const BatchVolStyle* __restrict__ styles) {
+    const BuildArgs& a = styles[blockIdx.x].a;
+    if (!a.occupancy) return;
+    const uint32_t words = uint32_t(occupancy_bit_words(a.g.nx, a.g.ny, a.g.nz));
+    for (uint32_t w = threadIdx.x; w < words; w += blockDim.x) pack_brick_bits(a.occupancy, a.g.nx, a.g.ny, a.g.nz, w);
 }
 
 // One pass of every style's brick distance field: a line chunk is four
@@ -127,6 +137,7 @@ void launch_batch_volume(const BatchDevice& d, hipStream_t stream) {
     batch_brick_kernel<<<d.blocks[kPassBricks], kBrickBlock, 0, stream>>>(d, st);
     batch_overflow_finalize_kernel<<<d.blocks[kPassFinalize], 256, 0, stream>>>(d, st);
     batch_cells_kernel<<<d.blocks[kPassCells], 256, 0, stream>>>(d, st);
+    batch_pack_bits_kernel<<<d.n_styles, 256, 0, stream>>>(st);
     for (int axis = 0; axis < 3; ++axis)
         if (d.n_line_chunks[axis])
             batch_distance_kernel<<<grid_for(d.n_line_chunks[axis], 1, 4096), 256, 0, stream>>>(d, st, axis);

@@ -23,14 +23,14 @@ __global__ void __launch_bounds__(kBuildBlock) bin_count_kernel(BuildArgs a) {
     bin_count_body(a, t, blockIdx.x);
 }
 
-__global__ void __launch_bounds__(kBuildBlock) bin_fill_kernel(BuildArgs a, size_t n_voxels) {
-    __shared__ BinTable t;
-    bin_fill_body(a, t, blockIdx.x, gridDim.x, n_voxels);
-}
-
 __global__ void __launch_bounds__(kScanThreads) scan_kernel(BuildArgs a) {
     __shared__ ScanLds l;
     scan_body(a, l);
+}
+
+__global__ void __launch_bounds__(kBuildBlock) bin_fill_kernel(BuildArgs a, size_t n_voxels) {
+    __shared__ BinTable t;
+    bin_fill_body(a, t, blockIdx.x, gridDim.x, n_voxels);
 }
 
 __global__ void __launch_bounds__(kBrickBlock) brick_kernel(BuildArgs a) {
@@ -44,19 +44,19 @@ __global__ void __launch_bounds__(256) overflow_finalize_kernel(BuildArgs a, siz
 
 // The corner-packed cells from the uint8 view with the normalize applied as
 // the texels load (the map from the min/max slots), writing the normalized
-// volume and its float view on the way (cells_block modes 1 and 2).
+// volume and its float view on the way (cells_pass modes 1 and 2).
 __global__ void __launch_bounds__(256)
 mapped_cells_kernel(const uint8_t* __restrict__ raw, uint32_t nx, uint32_t ny, uint32_t nz,
                     const uint32_t* __restrict__ minmax, int normalize, unsigned long long* __restrict__ cells,
                     uint32_t* __restrict__ occupancy, uint8_t* __restrict__ out_u8, float* __restrict__ out_f32) {
     __shared__ uint8_t lut[256];
     const bool identity = build_lut(minmax, normalize != 0, lut);
-    const uint32_t n_blocks = cell_blocks(nx, ny, nz);
+    const uint32_t n_blocks = cells_pass_blocks(nx, ny, nz);
     for (uint32_t b = blockIdx.x; b < n_blocks; b += gridDim.x) {
         if (identity)
-            cells_block<1>(raw, nx, ny, nz, b, cells, occupancy, nullptr, out_u8, out_f32);
+            cells_pass<1>(raw, nx, ny, nz, b, cells, occupancy, nullptr, out_u8, out_f32);
         else
-            cells_block<2>(raw, nx, ny, nz, b, cells, occupancy, lut, out_u8, out_f32);
+            cells_pass<2>(raw, nx, ny, nz, b, cells, occupancy, lut, out_u8, out_f32);
     }
 }
 
@@ -83,8 +83,10 @@ extern "C" int vkhr_volume_build_segments(const float* xyz, uint32_t n_vertices,
     VKHR_REQUIRE(!tangent_i8x4 || tangents, VKHR_ERR_INVALID, "tangent_i8x4 needs tangents");
     VKHR_REQUIRE(!occupancy || cells, VKHR_ERR_INVALID, "occupancy needs cells");
     VKHR_REQUIRE(reinterpret_cast<uintptr_t>(density_u8) % 4 == 0 && reinterpret_cast<uintptr_t>(density_f32) % 16 == 0 &&
-                     reinterpret_cast<uintptr_t>(workspace) % 16 == 0,
-                 VKHR_ERR_INVALID, "density_u8 must be 4-byte and density_f32, workspace 16-byte aligned");
+                     reinterpret_cast<uintptr_t>(workspace) % 16 == 0 && reinterpret_cast<uintptr_t>(counts) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(tangent_i8x4) % 16 == 0,
+                 VKHR_ERR_INVALID,
+                 "density_u8 must be 4-byte and counts, tangent_i8x4, density_f32, workspace 16-byte aligned");
     hipStream_t s = as_stream(stream);
     const BrickGrid bg = brick_grid(nx, ny, nz);
     const size_t n_bricks = size_t(bg.bx) * bg.by * bg.bz;  // <= 2^24 (make_grid bounds the voxels)
@@ -112,7 +114,7 @@ extern "C" int vkhr_volume_build_segments(const float* xyz, uint32_t n_vertices,
     a.cursor = w + lay.cursor;
     a.status = w + lay.status;
     a.minmax = w + lay.minmax;
-    a.entries = reinterpret_cast<uint2*>(w + lay.entries);
+    a.entries = reinterpret_cast<BinEntry*>(w + lay.entries);
     a.capacity = uint32_t(capacity);
     a.occupancy = occupancy;
     a.occupancy_words = occupancy ? occupancy_words(nx, ny, nz) : 0;
@@ -142,9 +144,9 @@ extern "C" int vkhr_volume_build_segments(const float* xyz, uint32_t n_vertices,
     if (!mapped)
         return finish_volume(density_u8, nx, ny, nz, normalize, density_f32, nullptr, nullptr,
                              normalize ? a.minmax : nullptr, s);
-    mapped_cells_kernel<<<grid_for(cell_blocks(nx, ny, nz), 1, 1u << 20), 256, 0, s>>>(
+    mapped_cells_kernel<<<grid_for(cells_pass_blocks(nx, ny, nz), 1, 1u << 20), 256, 0, s>>>(
         a.density, nx, ny, nz, a.minmax, normalize, reinterpret_cast<unsigned long long*>(cells), occupancy,
         density_u8, density_f32);
     VKHR_CHECK_LAUNCH();
-    return occupancy ? launch_distance_field(occupancy, nx, ny, nz, s) : VKHR_OK;
+    return occupancy ? finish_occupancy(occupancy, nx, ny, nz, s) : VKHR_OK;
 }

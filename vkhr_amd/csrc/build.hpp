@@ -4,8 +4,9 @@
 // build of many.
 //
 // Bricks of 8x8x8 voxels.  The index pairs are binned by the bricks their
-// samples visit -- a counting sort: count, scan, fill -- then one workgroup
-// per brick walks the DDA of every pair in its bin and adds the samples
+// samples visit -- a counting sort: count, scan, fill -- then one
+// workgroup per brick walks the DDA of every pair in its
+// bin (from the run's first sample) and adds the samples
 // that fall inside its brick into an LDS accumulator (count and fixed-point
 // tangent sums: integer adds, so the order of the adds does not matter).
 // The workgroup finalizes its brick straight from LDS: the saturating uint8
@@ -62,6 +63,17 @@ __device__ __forceinline__ uint32_t brick_index(const BrickGrid& b, uint3 v) {
     return (v.x >> 3) + b.bx * ((v.y >> 3) + b.by * (v.z >> 3));
 }
 
+// A bin entry: one pair's run through one brick -- the DDA at the run's
+// first sample (position, steps left before the sample is taken, the step)
+// and the pair's root vertex (its tangent).  The walk on from there makes
+// the same float adds as the walk from the pair's root.
+struct BinEntry {
+    float rx, ry, rz, steps;
+    float dx, dy, dz;
+    uint32_t root;
+};
+static_assert(sizeof(BinEntry) == 32, "two 16-byte words per entry");
+
 // Where the build keeps its state (32-bit words from one base).
 struct BuildLayout {
     size_t bins;        // bin sizes, one per brick
@@ -69,7 +81,7 @@ struct BuildLayout {
     size_t status;      // [0] total entries, [1] overflow flag
     size_t minmax;      // the min/max slots (kMinMaxBytes)
     size_t raw;         // the uint8 view before the normalize (with cells), n voxels
-    size_t entries;     // bin entries: the pairs' vertex indices (uint2)
+    size_t entries;     // bin entries (BinEntry)
     size_t words;
 };
 
@@ -82,14 +94,14 @@ __host__ __forceinline__ BuildLayout build_layout(size_t n_bricks, size_t n_voxe
     l.minmax = l.status + 64;
     l.raw = l.minmax + up(kMinMaxBytes / 4);
     l.entries = l.raw + (raw ? up((n_voxels + 3) / 4) : 0);
-    l.words = l.entries + up(2 * capacity);
+    l.words = l.entries + up(sizeof(BinEntry) / 4 * capacity);
     return l;
 }
 
-// Bin entries per style: 4 per pair plus 64 Ki, at most 2^28.
+// Bin entries per style: 4 per pair plus 64 Ki, at most 2^26 (2 GiB).
 __host__ __forceinline__ size_t build_capacity(uint32_t n_pairs) {
     const size_t c = 4 * size_t(n_pairs) + 65536;
-    return c < (size_t(1) << 28) ? c : (size_t(1) << 28);
+    return c < (size_t(1) << 26) ? c : (size_t(1) << 26);
 }
 
 // Pairs per bin workgroup: ranges of 256 .. 1024 (kPairsPerThread per
@@ -112,7 +124,7 @@ struct BuildArgs {
     uint32_t* cursor;
     uint32_t* status;
     uint32_t* minmax;
-    uint2* entries;
+    BinEntry* entries;
     uint32_t capacity;
     uint32_t* occupancy;            // zeroed by the init pass (nullable)
     size_t occupancy_words;
@@ -188,13 +200,17 @@ __device__ __forceinline__ void table_clear(BinTable& t) {
     }
 }
 
-// Calls run(pair's vertex indices, brick) for each brick run of each pair
-// of workgroup block's range (at most kPairsPerThread per thread: the index
-// loads of all of a thread's pairs are issued before the first walk).
+// The DDAs of one thread's pairs of workgroup block's range (at most
+// kPairsPerThread; the index and vertex loads of all of them are issued
+// before the first walk), kept so that a pass can walk them twice.
 constexpr uint32_t kPairsPerThread = 4;
 
-template <class Run>
-__device__ __forceinline__ void walk_runs(const BuildArgs& a, uint32_t block, Run run) {
+struct PairWalks {
+    SegmentDda w[kPairsPerThread];
+    uint32_t root[kPairsPerThread];  // vertex index of the pair's root (its tangent)
+};
+
+__device__ __forceinline__ void load_walks(const BuildArgs& a, uint32_t block, PairWalks& p) {
     const uint32_t r0 = block * a.pairs_per_block;
     const uint32_t r1 = min(a.n_pairs, r0 + a.pairs_per_block);
     uint2 pr[kPairsPerThread];
@@ -205,75 +221,37 @@ __device__ __forceinline__ void walk_runs(const BuildArgs& a, uint32_t block, Ru
     }
 #pragma unroll
     for (uint32_t j = 0; j < kPairsPerThread; ++j) {
-        SegmentDda w = segment_dda(a.g, a.xyz, a.n_vertices, pr[j].x, pr[j].y);  // kBuildNone: no samples
+        p.w[j] = segment_dda(a.g, a.xyz, a.n_vertices, pr[j].x, pr[j].y);  // kBuildNone: no samples
+        p.root[j] = pr[j].x;
+    }
+}
+
+// Calls run(j, DDA at the sample, steps before the sample, brick) at the
+// first sample of each brick run of each of the thread's pairs.
+template <class Run>
+__device__ __forceinline__ void walk_runs(const BuildArgs& a, const PairWalks& p, Run run) {
+#pragma unroll
+    for (uint32_t j = 0; j < kPairsPerThread; ++j) {
+        SegmentDda w = p.w[j];
         uint32_t last = kBuildNone;
         while (w.steps > 0.0f) {
+            const float before = w.steps;
             w.steps -= 1.0f;
             const uint32_t k = brick_index(a.b, voxel_coords(a.g, w.rx, w.ry, w.rz));
-            if (k != last) run(pr[j], k);
+            if (k != last) run(j, w, before, k);
             last = k;
             w.advance();
         }
     }
 }
 
-// The bin sizes: runs counted per brick in the table, then one add per
-// table entry.
-__device__ __forceinline__ void bin_count_body(const BuildArgs& a, BinTable& t, uint32_t block) {
-    table_clear(t);
-    __syncthreads();
-    walk_runs(a, block, [&](uint2, uint32_t k) {
-        const uint32_t slot = table_insert(t, k);
-        if (slot != kBuildNone)
-            atomicAdd(&t.count[slot], 1u);
-        else
-            atomicAdd(&a.bins[k], 1u);
-    });
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < kTableSlots; i += blockDim.x)
-        if (t.key[i] != kBuildNone) atomicAdd(&a.bins[t.key[i]], t.count[i]);
-}
-
-// The bin entries: the same counts in the table, one range reserved per
-// table entry in its brick's bin, then a second walk that places each run
-// in its range (table misses take a slot of their own).  With the overflow
-// flag set the workgroups zero the workspace instead.
-__device__ __forceinline__ void bin_fill_body(const BuildArgs& a, BinTable& t, uint32_t block, uint32_t blocks,
-                                              size_t n_voxels) {
-    if (a.status[1]) {
-        uint32_t* acc = a.acc ? reinterpret_cast<uint32_t*>(a.acc) : a.acc_counts;
-        const size_t words = overflow_words(a, n_voxels);
-        const size_t stride = size_t(blocks) * blockDim.x;
-        for (size_t i = size_t(block) * blockDim.x + threadIdx.x; i < words; i += stride) acc[i] = 0u;
-        return;
-    }
-    table_clear(t);
-    __syncthreads();
-    walk_runs(a, block, [&](uint2, uint32_t k) {
-        const uint32_t slot = table_insert(t, k);
-        if (slot != kBuildNone) atomicAdd(&t.count[slot], 1u);
-    });
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < kTableSlots; i += blockDim.x) {
-        if (t.key[i] != kBuildNone) t.base[i] = atomicAdd(&a.cursor[t.key[i]], t.count[i]);
-        t.count[i] = 0u;
-    }
-    __syncthreads();
-    walk_runs(a, block, [&](uint2 pair, uint32_t k) {
-        const uint32_t slot = table_find(t, k);
-        const uint32_t pos =
-            slot != kBuildNone ? t.base[slot] + atomicAdd(&t.count[slot], 1u) : atomicAdd(&a.cursor[k], 1u);
-        if (pos < a.capacity) a.entries[pos] = pair;
-    });
-}
-
-// Exclusive scan of the bin sizes into the bin starts (one workgroup of
-// kScanThreads, 16 sizes per thread per round), the total, and the
+// Exclusive scan of the bin sizes into the bin starts (one workgroup of at
+// most kScanThreads, 16 sizes per thread per round), the total, and the
 // overflow flag.
-constexpr uint32_t kScanThreads = 1024;
+constexpr uint32_t kScanThreads = 1024, kMaxScanWaves = kScanThreads / kWave;
 
 struct ScanLds {
-    unsigned long long wave_sums[kScanThreads / kWave];
+    unsigned long long wave_sums[kMaxScanWaves];
     unsigned long long carry;  // 64-bit: the total of absurd inputs may pass 2^32
 };
 
@@ -283,7 +261,7 @@ __device__ __forceinline__ void scan_body(const BuildArgs& a, ScanLds& l) {
     const uint32_t n = a.n_bricks;
     if (threadIdx.x == 0) l.carry = 0;
     __syncthreads();
-    for (uint32_t chunk = 0; chunk < n; chunk += kPer * kScanThreads) {
+    for (uint32_t chunk = 0; chunk < n; chunk += kPer * blockDim.x) {
         const uint32_t i0 = chunk + kPer * threadIdx.x;
         uint32_t v[kPer];
         if (i0 + kPer <= n) {  // the sections are 256-byte aligned and i0 a multiple of 16
@@ -329,13 +307,93 @@ __device__ __forceinline__ void scan_body(const BuildArgs& a, ScanLds& l) {
                 if (i0 + j < n) a.cursor[i0 + j] = out[j];
         }
         __syncthreads();
-        if (threadIdx.x == kScanThreads - 1) l.carry = before;
+        if (threadIdx.x == blockDim.x - 1) l.carry = before;
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         a.status[0] = uint32_t(l.carry < 0xffffffffull ? l.carry : 0xffffffffull);
         a.status[1] = l.carry > a.capacity ? 1u : 0u;
     }
+}
+
+// The bin sizes: runs counted per brick in the table, then one add per
+// table entry.  One walk per pair: each DDA is set up as it is walked
+// (the index loads of the thread's pairs issued first), which keeps fewer
+// registers live than PairWalks.
+__device__ __forceinline__ void bin_count_body(const BuildArgs& a, BinTable& t, uint32_t block) {
+    const uint32_t r0 = block * a.pairs_per_block;
+    const uint32_t r1 = min(a.n_pairs, r0 + a.pairs_per_block);
+    uint2 pr[kPairsPerThread];
+#pragma unroll
+    for (uint32_t j = 0; j < kPairsPerThread; ++j) {
+        const uint32_t s = r0 + threadIdx.x + j * blockDim.x;
+        pr[j] = s < r1 ? make_uint2(a.indices[2u * s], a.indices[2u * s + 1u]) : make_uint2(kBuildNone, kBuildNone);
+    }
+    table_clear(t);
+    __syncthreads();
+    const auto count = [&](uint32_t k) {
+        const uint32_t slot = table_insert(t, k);
+        if (slot != kBuildNone)
+            atomicAdd(&t.count[slot], 1u);
+        else
+            atomicAdd(&a.bins[k], 1u);
+    };
+#pragma unroll
+    for (uint32_t j = 0; j < kPairsPerThread; ++j) {
+        SegmentDda w = segment_dda(a.g, a.xyz, a.n_vertices, pr[j].x, pr[j].y);  // kBuildNone: no samples
+        uint32_t last = kBuildNone;
+        while (w.steps > 0.0f) {
+            w.steps -= 1.0f;
+            const uint32_t k = brick_index(a.b, voxel_coords(a.g, w.rx, w.ry, w.rz));
+            if (k != last) count(k);
+            last = k;
+            w.advance();
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kTableSlots; i += blockDim.x)
+        if (t.key[i] != kBuildNone) atomicAdd(&a.bins[t.key[i]], t.count[i]);
+}
+
+// The bin entries: the same counts in the table, one range reserved per
+// table entry in its brick's bin, then a second walk of the same DDAs that
+// places each run in its range (table misses take a slot of their own):
+// the run's first sample and the DDA state there, from which the brick
+// pass walks on with the same float adds.  With the overflow flag set the
+// workgroups zero the workspace instead.
+__device__ __forceinline__ void bin_fill_body(const BuildArgs& a, BinTable& t, uint32_t block, uint32_t blocks,
+                                              size_t n_voxels) {
+    if (a.status[1]) {
+        uint32_t* acc = a.acc ? reinterpret_cast<uint32_t*>(a.acc) : a.acc_counts;
+        const size_t words = overflow_words(a, n_voxels);
+        const size_t stride = size_t(blocks) * blockDim.x;
+        for (size_t i = size_t(block) * blockDim.x + threadIdx.x; i < words; i += stride) acc[i] = 0u;
+        return;
+    }
+    PairWalks p;
+    load_walks(a, block, p);
+    table_clear(t);
+    __syncthreads();
+    walk_runs(a, p, [&](uint32_t, const SegmentDda&, float, uint32_t k) {
+        const uint32_t slot = table_insert(t, k);
+        if (slot != kBuildNone) atomicAdd(&t.count[slot], 1u);
+    });
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kTableSlots; i += blockDim.x) {
+        if (t.key[i] != kBuildNone) t.base[i] = atomicAdd(&a.cursor[t.key[i]], t.count[i]);
+        t.count[i] = 0u;
+    }
+    __syncthreads();
+    walk_runs(a, p, [&](uint32_t j, const SegmentDda& w, float before, uint32_t k) {
+        const uint32_t slot = table_find(t, k);
+        const uint32_t pos =
+            slot != kBuildNone ? t.base[slot] + atomicAdd(&t.count[slot], 1u) : atomicAdd(&a.cursor[k], 1u);
+        if (pos < a.capacity) {
+            float4* e = reinterpret_cast<float4*>(a.entries + pos);
+            e[0] = make_float4(w.rx, w.ry, w.rz, before);
+            e[1] = make_float4(w.dx, w.dy, w.dz, __uint_as_float(p.root[j]));
+        }
+    });
 }
 
 // Brick of a workgroup: groups of 8 consecutive bricks along x go to the
@@ -389,58 +447,125 @@ __device__ __forceinline__ void brick_body(const BuildArgs& a, BrickLds& l, uint
     }
     const uint32_t brick = brick_of_block(a.b, block);
     if (brick == kBuildNone) return;
-    for (uint32_t v = threadIdx.x; v < kVox; v += kBrickBlock) {
-        l.count[v] = 0u;
-        l.sum[0][v] = l.sum[1][v] = l.sum[2][v] = 0;
-    }
     const uint32_t bxo = brick % a.b.bx, byo = (brick / a.b.bx) % a.b.by, bzo = brick / (a.b.bx * a.b.by);
-    __syncthreads();
-    const uint32_t end = a.cursor[brick], n = a.bins[brick];
+    const uint32_t end = a.cursor[brick], n = a.bins[brick];  // workgroup-uniform
     const bool with_tangents = a.tangent_i8x4 != nullptr;
-    for (uint32_t e = end - n + threadIdx.x; e < end; e += kBrickBlock) {
-        const uint2 pair = a.entries[e];
-        SegmentDda w = segment_dda(a.g, a.xyz, a.n_vertices, pair.x, pair.y);
-        int tx = 0, ty = 0, tz = 0;
-        if (with_tangents && w.steps > 0.0f) {
-            const float3 t = load_xyz(a.tangents, pair.x);
-            tx = int(tangent_fixed(t.x));
-            ty = int(tangent_fixed(t.y));
-            tz = int(tangent_fixed(t.z));
+    // With nx % 4 == 0 thread t writes voxels 4 (t & 1) .. +3 of brick row
+    // t >> 1 (y = row & 7, z = row >> 3), whole words at a time; else one
+    // voxel at a time.
+    static_assert(kBrickBlock * 4 == kVox, "one quad of voxels per thread");
+    const bool quads = (a.g.nx & 3u) == 0u;
+    const uint32_t t = threadIdx.x, row = t >> 1, q0 = 8u * row + 4u * (t & 1u);
+    const uint32_t qx = bxo * 8u + 4u * (t & 1u), qy = byo * 8u + (row & 7u), qz = bzo * 8u + (row >> 3);
+    const bool quad_in = qx < a.g.nx && qy < a.g.ny && qz < a.g.nz;  // the quad is whole when nx % 4 == 0
+    const size_t qv = size_t(qx) + size_t(a.g.nx) * (size_t(qy) + size_t(a.g.ny) * qz);
+    uint32_t lo = 255u, hi = 0u;
+    if (n == 0u) {
+        // an empty bin: every voxel of the brick is 0, and no LDS is needed
+        if (quads) {
+            if (quad_in) {
+                *reinterpret_cast<uint32_t*>(a.density + qv) = 0u;
+                if (a.counts) *reinterpret_cast<uint4*>(a.counts + qv) = make_uint4(0u, 0u, 0u, 0u);
+                if (with_tangents) *reinterpret_cast<uint4*>(a.tangent_i8x4 + 4 * qv) = make_uint4(0u, 0u, 0u, 0u);
+            }
+        } else {
+            for (uint32_t i = t; i < kVox; i += kBrickBlock) {
+                const uint32_t x = bxo * 8u + (i & 7u), y = byo * 8u + ((i >> 3) & 7u), z = bzo * 8u + (i >> 6);
+                if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz) continue;
+                const size_t v = size_t(x) + size_t(a.g.nx) * (size_t(y) + size_t(a.g.ny) * z);
+                a.density[v] = 0u;
+                if (a.counts) a.counts[v] = 0u;
+                if (with_tangents) reinterpret_cast<char4*>(a.tangent_i8x4)[v] = make_char4(0, 0, 0, 0);
+            }
         }
-        bool entered = false;
-        while (w.steps > 0.0f) {
-            w.steps -= 1.0f;
-            const uint3 v = voxel_coords(a.g, w.rx, w.ry, w.rz);
-            const bool inside = (v.x >> 3) == bxo && (v.y >> 3) == byo && (v.z >> 3) == bzo;
-            if (inside) {
+        if (t == 0)  // the brick holds a voxel of the volume (brick_of_block)
+            fold_minmax(a.minmax + (block % kMinMaxSlots) * kMinMaxStride, 0u, 0u);
+        return;
+    }
+    reinterpret_cast<uint4*>(l.count)[t] = make_uint4(0u, 0u, 0u, 0u);
+    reinterpret_cast<int4*>(l.sum[0])[t] = make_int4(0, 0, 0, 0);
+    reinterpret_cast<int4*>(l.sum[1])[t] = make_int4(0, 0, 0, 0);
+    reinterpret_cast<int4*>(l.sum[2])[t] = make_int4(0, 0, 0, 0);
+    __syncthreads();
+    // kEntriesInFlight entries per thread per round: their entry and tangent
+    // loads are issued before the first walk
+    constexpr uint32_t kEntriesInFlight = 2;
+    for (uint32_t e0 = end - n + t; e0 < end; e0 += kEntriesInFlight * kBrickBlock) {
+        SegmentDda w[kEntriesInFlight];
+        int tx[kEntriesInFlight], ty[kEntriesInFlight], tz[kEntriesInFlight];
+#pragma unroll
+        for (uint32_t u = 0; u < kEntriesInFlight; ++u) {
+            const uint32_t e = e0 + u * kBrickBlock;
+            float4 p0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p1 = p0;  // steps 0: no samples
+            if (e < end) {
+                const float4* en = reinterpret_cast<const float4*>(a.entries + e);
+                p0 = en[0];
+                p1 = en[1];
+            }
+            w[u] = {p0.x, p0.y, p0.z, p1.x, p1.y, p1.z, p0.w};
+            tx[u] = ty[u] = tz[u] = 0;
+            if (with_tangents && e < end) {
+                const float3 tan = load_xyz(a.tangents, __float_as_uint(p1.w));
+                tx[u] = int(tangent_fixed(tan.x));
+                ty[u] = int(tangent_fixed(tan.y));
+                tz[u] = int(tangent_fixed(tan.z));
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kEntriesInFlight; ++u) {
+            // the run starts inside the brick and, once out, never comes back
+            while (w[u].steps > 0.0f) {
+                w[u].steps -= 1.0f;
+                const uint3 v = voxel_coords(a.g, w[u].rx, w[u].ry, w[u].rz);
+                if ((v.x >> 3) != bxo || (v.y >> 3) != byo || (v.z >> 3) != bzo) break;
                 const uint32_t i = (v.x & 7u) + 8u * ((v.y & 7u) + 8u * (v.z & 7u));
                 atomicAdd(&l.count[i], 1u);
                 if (with_tangents) {  // wraps only past 32767 contributions, where the tangent is 0
-                    atomicAdd(reinterpret_cast<unsigned*>(&l.sum[0][i]), unsigned(tx));
-                    atomicAdd(reinterpret_cast<unsigned*>(&l.sum[1][i]), unsigned(ty));
-                    atomicAdd(reinterpret_cast<unsigned*>(&l.sum[2][i]), unsigned(tz));
+                    atomicAdd(reinterpret_cast<unsigned*>(&l.sum[0][i]), unsigned(tx[u]));
+                    atomicAdd(reinterpret_cast<unsigned*>(&l.sum[1][i]), unsigned(ty[u]));
+                    atomicAdd(reinterpret_cast<unsigned*>(&l.sum[2][i]), unsigned(tz[u]));
                 }
-            } else if (entered) {
-                break;  // the brick run is over: the pair never comes back
+                w[u].advance();
             }
-            entered = entered || inside;
-            w.advance();
         }
     }
     __syncthreads();
-    uint32_t lo = 255u, hi = 0u;
-    for (uint32_t i = threadIdx.x; i < kVox; i += kBrickBlock) {
-        const uint32_t x = bxo * 8u + (i & 7u), y = byo * 8u + ((i >> 3) & 7u), z = bzo * 8u + (i >> 6);
-        if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz) continue;
-        const size_t v = size_t(x) + size_t(a.g.nx) * (size_t(y) + size_t(a.g.ny) * z);
-        const uint32_t c = l.count[i];
-        const uint32_t d = c < 255u ? c : 255u;
-        a.density[v] = uint8_t(d);
-        if (a.counts) a.counts[v] = c;
-        if (with_tangents)
-            reinterpret_cast<char4*>(a.tangent_i8x4)[v] = quantize_tangent_sums(l.sum[0][i], l.sum[1][i], l.sum[2][i], c);
-        lo = d < lo ? d : lo;
-        hi = d > hi ? d : hi;
+    if (quads) {
+        if (quad_in) {
+            const uint4 c4 = *reinterpret_cast<const uint4*>(&l.count[q0]);
+            const uint32_t c[4] = {c4.x, c4.y, c4.z, c4.w};
+            uint32_t dens = 0u, tq[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (uint32_t m = 0; m < 4; ++m) {
+                const uint32_t d = c[m] < 255u ? c[m] : 255u;
+                dens |= d << (8u * m);
+                lo = d < lo ? d : lo;
+                hi = d > hi ? d : hi;
+                if (with_tangents && c[m]) {
+                    const char4 k = quantize_tangent_sums(l.sum[0][q0 + m], l.sum[1][q0 + m], l.sum[2][q0 + m], c[m]);
+                    tq[m] = uint32_t(uint8_t(k.x)) | (uint32_t(uint8_t(k.y)) << 8) | (uint32_t(uint8_t(k.z)) << 16) |
+                            (uint32_t(uint8_t(k.w)) << 24);
+                }
+            }
+            *reinterpret_cast<uint32_t*>(a.density + qv) = dens;
+            if (a.counts) *reinterpret_cast<uint4*>(a.counts + qv) = c4;
+            if (with_tangents) *reinterpret_cast<uint4*>(a.tangent_i8x4 + 4 * qv) = make_uint4(tq[0], tq[1], tq[2], tq[3]);
+        }
+    } else {
+        for (uint32_t i = t; i < kVox; i += kBrickBlock) {
+            const uint32_t x = bxo * 8u + (i & 7u), y = byo * 8u + ((i >> 3) & 7u), z = bzo * 8u + (i >> 6);
+            if (x >= a.g.nx || y >= a.g.ny || z >= a.g.nz) continue;
+            const size_t v = size_t(x) + size_t(a.g.nx) * (size_t(y) + size_t(a.g.ny) * z);
+            const uint32_t c = l.count[i];
+            const uint32_t d = c < 255u ? c : 255u;
+            a.density[v] = uint8_t(d);
+            if (a.counts) a.counts[v] = c;
+            if (with_tangents)
+                reinterpret_cast<char4*>(a.tangent_i8x4)[v] =
+                    quantize_tangent_sums(l.sum[0][i], l.sum[1][i], l.sum[2][i], c);
+            lo = d < lo ? d : lo;
+            hi = d > hi ? d : hi;
+        }
     }
     for (int o = kWave / 2; o > 0; o >>= 1) {
         const uint32_t p = __shfl_xor(lo, o, kWave), q = __shfl_xor(hi, o, kWave);

@@ -65,8 +65,15 @@ finalize_counts_kernel(const uint32_t* __restrict__ counts, const long long* __r
 __global__ void __launch_bounds__(256)
 cells_kernel(const uint8_t* __restrict__ density, uint32_t nx, uint32_t ny, uint32_t nz,
              unsigned long long* __restrict__ cells, uint32_t* __restrict__ occupancy) {
-    const uint32_t n_blocks = cell_blocks(nx, ny, nz);
-    for (uint32_t b = blockIdx.x; b < n_blocks; b += gridDim.x) cells_block(density, nx, ny, nz, b, cells, occupancy);
+    const uint32_t n_blocks = cells_pass_blocks(nx, ny, nz);
+    for (uint32_t b = blockIdx.x; b < n_blocks; b += gridDim.x) cells_pass(density, nx, ny, nz, b, cells, occupancy);
+}
+
+// The bitmask of the occupancy from the brick flags, one word per thread.
+__global__ void __launch_bounds__(256)
+pack_bits_kernel(uint32_t* __restrict__ occupancy, uint32_t nx, uint32_t ny, uint32_t nz) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < occupancy_bit_words(nx, ny, nz)) pack_brick_bits(occupancy, nx, ny, nz, w);
 }
 
 // One pass of the brick distance field: four lines per block (one per wave).
@@ -242,15 +249,17 @@ int finish_volume(uint8_t* density_u8, uint32_t nx, uint32_t ny, uint32_t nz, in
     }
     if (cells) {
         if (occupancy && (rc = fill_u32(occupancy, occupancy_words(nx, ny, nz), 0u, s))) return rc;
-        cells_kernel<<<grid_for(cell_blocks(nx, ny, nz), 1, 1u << 20), 256, 0, s>>>(
+        cells_kernel<<<grid_for(cells_pass_blocks(nx, ny, nz), 1, 1u << 20), 256, 0, s>>>(
             density_u8, nx, ny, nz, reinterpret_cast<unsigned long long*>(cells), occupancy);
         VKHR_CHECK_LAUNCH();
-        if (occupancy && (rc = launch_distance_field(occupancy, nx, ny, nz, s))) return rc;
+        if (occupancy && (rc = finish_occupancy(occupancy, nx, ny, nz, s))) return rc;
     }
     return VKHR_OK;
 }
 
-int launch_distance_field(uint32_t* occupancy, uint32_t nx, uint32_t ny, uint32_t nz, hipStream_t s) {
+int finish_occupancy(uint32_t* occupancy, uint32_t nx, uint32_t ny, uint32_t nz, hipStream_t s) {
+    pack_bits_kernel<<<grid_for(occupancy_bit_words(nx, ny, nz), 256), 256, 0, s>>>(occupancy, nx, ny, nz);
+    VKHR_CHECK_LAUNCH();
     if (!distance_field_fits(nx, ny, nz)) return VKHR_OK;
     for (int axis = 0; axis < 3; ++axis) {
         distance_kernel<<<grid_for(distance_lines(nx, ny, nz, axis), 256 / kWave), 256, 0, s>>>(occupancy, nx, ny,

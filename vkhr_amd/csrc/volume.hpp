@@ -157,9 +157,10 @@ constexpr size_t kMinMaxBytes = size_t(kMinMaxSlots) * kMinMaxStride * sizeof(ui
 int finish_volume(uint8_t* density_u8, uint32_t nx, uint32_t ny, uint32_t nz, int normalize, float* density_f32,
                   uint64_t* cells, uint32_t* occupancy, const uint32_t* mm, hipStream_t s);
 
-// The three passes of the brick distance field over an occupancy buffer
-// whose bitmask is set (nothing when a side is too long to transform).
-int launch_distance_field(uint32_t* occupancy, uint32_t nx, uint32_t ny, uint32_t nz, hipStream_t s);
+// The occupancy after the cells pass: the bitmask packed from the brick
+// flags, then the three passes of the brick distance field (none when a
+// side is too long to transform).
+int finish_occupancy(uint32_t* occupancy, uint32_t nx, uint32_t ny, uint32_t nz, hipStream_t s);
 
 // Folds (lo, hi) into a min/max slot.  A slot only moves toward the
 // result, so an atomic whose value a read of the slot already reaches (or
@@ -191,6 +192,38 @@ __device__ __forceinline__ uint32_t texel_density(uint32_t d, const DensityMap& 
     return m.normalize ? normalize_value(d, m.lo, m.scaling, m.flat) : d;
 }
 
+// The brick flags the cells pass sets: one byte per brick in the last
+// section of the occupancy buffer (zeroed before the pass), set to 1 with
+// a plain store for a brick holding a nonzero cell.  Plain stores, unlike
+// atomics on the bit words, do not queue at the memory side when the
+// waves of thousands of workgroups mark the same few bricks; pack_brick_bits
+// turns the flags into the bitmask afterwards, and the distance field's
+// passes then reuse the bytes.
+__host__ __device__ __forceinline__ uint8_t* brick_flags(uint32_t* occupancy, uint32_t nx, uint32_t ny, uint32_t nz) {
+    return reinterpret_cast<uint8_t*>(occupancy + occupancy_bit_words(nx, ny, nz) + distance_words(nx, ny, nz));
+}
+
+// Bit word w of the bitmask from the brick flags (one thread per word).
+__device__ __forceinline__ void pack_brick_bits(uint32_t* occupancy, uint32_t nx, uint32_t ny, uint32_t nz,
+                                                uint32_t w) {
+    const uint8_t* flags = brick_flags(occupancy, nx, ny, nz);
+    const uint32_t n = uint32_t(brick_count(nx, ny, nz));
+    uint32_t bits = 0u;
+    const uint32_t b0 = 32u * w;
+    if (b0 + 32u <= n) {  // the flags section is 4-byte aligned and b0 a multiple of 32
+        const uint32_t* f = reinterpret_cast<const uint32_t*>(flags + b0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t v = f[k];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bits |= ((v >> (8 * c)) & 0xffu) ? 1u << (4 * k + c) : 0u;
+        }
+    } else {
+        for (uint32_t b = b0; b < n; ++b) bits |= flags[b] ? 1u << (b - b0) : 0u;
+    }
+    occupancy[w] = bits;
+}
+
 // One block of the corner-packed layout -- cells (i, j, k), i in [-1, nx-1],
 // index (i+1) + (nx+1) * ((j+1) + (ny+1) * (k+1)), each the 8 texels of the
 // trilinear footprint whose lower corner is (i,j,k) as one uint64 (byte b =
@@ -200,8 +233,9 @@ __device__ __forceinline__ uint32_t texel_density(uint32_t d, const DensityMap& 
 // k+1 in [slab zc, slab zc + slab) walked upwards (slab = cell_slab), so each
 // level loads only its own texel slice and reuses the one below.  Lanes
 // read consecutive texels and store consecutive cells.  The bricks (8x8x8
-// cells) holding a nonzero cell are or-ed into occupancy (nullable; one word
-// atomic per wave and brick level).  All threads of the block call it.
+// cells) holding a nonzero cell are flagged in occupancy's brick flags
+// (nullable; one store per wave, brick and brick level).  All threads of
+// the block call it.
 // Cell levels per block: 32, halved (down to 4) while the volume would give
 // fewer than 2048 blocks, so small volumes still spread over the chip
 // instead of walking long columns on a few compute units.
@@ -301,17 +335,189 @@ __device__ __forceinline__ void cells_block(const uint8_t* __restrict__ density,
                 uint32_t m = 0u;
 #pragma unroll
                 for (int g = 0; g < 8; ++g) m |= ((nz_lanes >> (8 * g)) & 0xffull) ? (1u << g) : 0u;
-                if (lane == 0 && m) {
-                    // bricks 8 xb .. 8 xb + 7 of brick row (by, ck / 8): bits first .. first + 7
+                if (lane < 8u && ((m >> lane) & 1u)) {
+                    // bricks 8 xb .. 8 xb + 7 of brick row (by, ck / 8), one per lane
                     const uint32_t first = bricks_along(nx) * (by + bys * (ck >> kBrickShift)) + 8u * xb;
-                    const unsigned long long bits = (unsigned long long)(m) << (first & 31u);
-                    if (uint32_t(bits)) atomicOr(&occupancy[first >> 5], uint32_t(bits));
-                    if (uint32_t(bits >> 32)) atomicOr(&occupancy[(first >> 5) + 1], uint32_t(bits >> 32));
+                    brick_flags(occupancy, nx, ny, nz)[first + lane] = 1u;
                 }
                 any = 0u;
             }
         }
     }
+}
+
+// The same cells four per lane (quads: cell columns i+1 in [4q, 4q + 4) of
+// one cell row, with one 4-byte load per texel row and level instead of
+// eight byte loads), for rows of nx % 4 == 0 texels whose nx / 4 quads
+// divide the wave or fill whole waves.  Per level a lane loads the texel
+// words of rows j-1 and j at slice k (the slice below is kept from the
+// level before) and takes texel 4q-1 from its neighbour lane's word; the
+// last quad of a row also writes cell column nx.  A wave takes 64
+// consecutive (row, quad) pairs of a level; workgroups of 4 waves walk a
+// slab of levels, as cells_block, and flag the bricks as it does.  In
+// modes 1 and 2 the lane at (row j, level k) writes texel row (j-1, k-1)
+// of the final volume as one word.
+__host__ __device__ __forceinline__ bool cells_by_quads(uint32_t nx) {
+    const uint32_t q = nx / 4u;
+    return nx % 4u == 0u && q > 0u && (64u % q == 0u || q % 64u == 0u);
+}
+
+__host__ __device__ __forceinline__ uint32_t quad_waves(uint32_t nx, uint32_t ny) {
+    return ((ny + 1u) * (nx / 4u) + uint32_t(kWave) - 1u) / uint32_t(kWave);
+}
+
+__host__ __device__ __forceinline__ uint32_t quad_slab(uint32_t nx, uint32_t ny, uint32_t nz) {
+    const uint32_t plane = (quad_waves(nx, ny) + 3u) / 4u;
+    uint32_t slab = 32u;
+    while (slab > 4u && plane * ((nz + 1u + slab - 1u) / slab) < 2048u) slab >>= 1;
+    return slab;
+}
+
+// Workgroups of the cells pass (either form).
+__host__ __device__ __forceinline__ uint32_t cells_pass_blocks(uint32_t nx, uint32_t ny, uint32_t nz) {
+    if (!cells_by_quads(nx)) return cell_blocks(nx, ny, nz);
+    const uint32_t slab = quad_slab(nx, ny, nz);
+    return (quad_waves(nx, ny) + 3u) / 4u * ((nz + 1u + slab - 1u) / slab);
+}
+
+template <int kMode>
+__device__ __forceinline__ uint32_t map_word(uint32_t w, const uint8_t* lut) {
+    if constexpr (kMode != 2) return w;
+    return uint32_t(lut[w & 0xffu]) | (uint32_t(lut[(w >> 8) & 0xffu]) << 8) |
+           (uint32_t(lut[(w >> 16) & 0xffu]) << 16) | (uint32_t(lut[w >> 24]) << 24);
+}
+
+template <int kMode = 0>
+__device__ __forceinline__ void cells_quads(const uint8_t* __restrict__ density, uint32_t nx, uint32_t ny,
+                                            uint32_t nz, uint32_t block, unsigned long long* __restrict__ cells,
+                                            uint32_t* occupancy, const uint8_t* lut = nullptr,
+                                            uint8_t* __restrict__ out_u8 = nullptr,
+                                            float* __restrict__ out_f32 = nullptr) {
+    const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const uint32_t nq = nx / 4u;
+    const uint32_t waves = quad_waves(nx, ny), plane = (waves + 3u) / 4u;
+    const uint32_t chunk = (block % plane) * 4u + wave, zc = block / plane;
+    if (chunk >= waves) return;  // wave-uniform
+    const uint32_t slab = quad_slab(nx, ny, nz);
+    const uint32_t k0 = zc * slab, k1 = min(k0 + slab, nz + 1u);
+    const uint32_t rq = chunk * uint32_t(kWave) + lane;
+    const bool live = rq < (ny + 1u) * nq;
+    const uint32_t cj = rq / nq, q = rq % nq;  // cell row (texel rows cj-1, cj) and quad
+    const bool last = live && q + 1u == nq;   // also writes cell column nx
+    const size_t row_pitch = nx, slice_pitch = size_t(nx) * ny;
+    // texel words 4q .. 4q+3 of texel row y in slice z (0 outside), and
+    // texel 4q-1 of the same row
+    auto word = [&](int y, int z) -> uint32_t {
+        if (!live || y < 0 || z < 0 || y >= int(ny) || z >= int(nz)) return 0u;
+        const uint32_t w = *reinterpret_cast<const uint32_t*>(density + size_t(y) * row_pitch +
+                                                               size_t(z) * slice_pitch + 4u * q);
+        return map_word<kMode>(w, lut);
+    };
+    auto before = [&](uint32_t w, int y, int z) -> uint32_t {
+        const uint32_t from_left = uint32_t(__shfl_up(int(w), 1, kWave)) >> 24;
+        if (q == 0u) return 0u;
+        if (lane != 0u) return from_left;
+        // the wave starts inside a row (nx / 4 a multiple of 64)
+        if (!live || y < 0 || z < 0 || y >= int(ny) || z >= int(nz)) return 0u;
+        const uint32_t t = density[size_t(y) * row_pitch + size_t(z) * slice_pitch + 4u * q - 1u];
+        return kMode == 2 ? uint32_t(lut[t]) : t;
+    };
+    const int y0 = int(cj) - 1, y1 = int(cj);
+    uint32_t bw[2], bp[2];  // slice k0 - 1: words and texels 4q-1 of rows y0, y1
+    bw[0] = word(y0, int(k0) - 1);
+    bw[1] = word(y1, int(k0) - 1);
+    bp[0] = before(bw[0], y0, int(k0) - 1);
+    bp[1] = before(bw[1], y1, int(k0) - 1);
+    uint32_t any = 0u, any_last = 0u;
+    constexpr uint32_t kLevelBatch = 4;
+    for (uint32_t cg = k0; cg < k1; cg += kLevelBatch) {
+        uint32_t aw[kLevelBatch][2];
+#pragma unroll
+        for (uint32_t u = 0; u < kLevelBatch; ++u) {
+            aw[u][0] = cg + u < k1 ? word(y0, int(cg + u)) : 0u;
+            aw[u][1] = cg + u < k1 ? word(y1, int(cg + u)) : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kLevelBatch; ++u) {
+            const uint32_t ck = cg + u;
+            if (ck >= k1) break;  // wave-uniform
+            uint32_t ap[2];
+            ap[0] = before(aw[u][0], y0, int(ck));
+            ap[1] = before(aw[u][1], y1, int(ck));
+            // bytes 4q-1 .. 4q+3 of the four texel rows, as 40-bit strings
+            const unsigned long long s00 = bp[0] | ((unsigned long long)bw[0] << 8);
+            const unsigned long long s01 = bp[1] | ((unsigned long long)bw[1] << 8);
+            const unsigned long long s10 = ap[0] | ((unsigned long long)aw[u][0] << 8);
+            const unsigned long long s11 = ap[1] | ((unsigned long long)aw[u][1] << 8);
+            unsigned long long c[4];
+#pragma unroll
+            for (uint32_t m = 0; m < 4; ++m) {
+                const uint32_t sh = 8u * m;
+                const uint32_t lo = uint32_t((s00 >> sh) & 0xffffull) | (uint32_t((s01 >> sh) & 0xffffull) << 16);
+                const uint32_t hi = uint32_t((s10 >> sh) & 0xffffull) | (uint32_t((s11 >> sh) & 0xffffull) << 16);
+                c[m] = (unsigned long long)lo | ((unsigned long long)hi << 32);
+            }
+            if (live) {
+                unsigned long long* dst = cells + (size_t(cj) + size_t(ny + 1u) * ck) * (nx + 1u) + 4u * q;
+                dst[0] = c[0];
+                dst[1] = c[1];
+                dst[2] = c[2];
+                dst[3] = c[3];
+                any |= (c[0] | c[1] | c[2] | c[3]) != 0ull ? 1u : 0u;
+                if (last) {  // cell column nx: texels nx-1 and the zero border
+                    const uint32_t lo = (bw[0] >> 24) | ((bw[1] >> 24) << 16);
+                    const uint32_t hi = (aw[u][0] >> 24) | ((aw[u][1] >> 24) << 16);
+                    const unsigned long long e = (unsigned long long)lo | ((unsigned long long)hi << 32);
+                    dst[4] = e;
+                    any_last |= e != 0ull ? 1u : 0u;
+                }
+                if constexpr (kMode != 0) {
+                    // texel row (cj-1, ck-1) of the final volume, once per texel
+                    if (cj >= 1u && ck >= 1u && cj <= ny && ck <= nz) {
+                        const size_t t = size_t(cj - 1u) * row_pitch + size_t(ck - 1u) * slice_pitch + 4u * q;
+                        *reinterpret_cast<uint32_t*>(out_u8 + t) = bw[0];
+                        if (out_f32)
+                            *reinterpret_cast<float4*>(out_f32 + t) =
+                                make_float4(float(bw[0] & 0xffu) / 255.0f, float((bw[0] >> 8) & 0xffu) / 255.0f,
+                                            float((bw[0] >> 16) & 0xffu) / 255.0f, float(bw[0] >> 24) / 255.0f);
+                    }
+                }
+            }
+            bw[0] = aw[u][0];
+            bw[1] = aw[u][1];
+            bp[0] = ap[0];
+            bp[1] = ap[1];
+            if (occupancy && ((ck & 7u) == 7u || ck + 1u == k1)) {  // wave-uniform: the end of a brick level
+                // the two quads of a brick column share its bit
+                any |= uint32_t(__shfl_xor(int(any), 1, kWave));
+                if (live && any && (q & 1u) == 0u) {
+                    const uint32_t b = (q >> 1) + bricks_along(nx) * ((cj >> kBrickShift) +
+                                                                      bricks_along(ny) * (ck >> kBrickShift));
+                    brick_flags(occupancy, nx, ny, nz)[b] = 1u;
+                }
+                if (any_last) {
+                    const uint32_t b = (nx >> kBrickShift) + bricks_along(nx) * ((cj >> kBrickShift) +
+                                                                                 bricks_along(ny) * (ck >> kBrickShift));
+                    brick_flags(occupancy, nx, ny, nz)[b] = 1u;
+                }
+                any = 0u;
+                any_last = 0u;
+            }
+        }
+    }
+}
+
+// The cells pass of one workgroup in whichever form fits nx
+// (cells_pass_blocks workgroups).
+template <int kMode = 0>
+__device__ __forceinline__ void cells_pass(const uint8_t* __restrict__ density, uint32_t nx, uint32_t ny, uint32_t nz,
+                                           uint32_t block, unsigned long long* __restrict__ cells, uint32_t* occupancy,
+                                           const uint8_t* lut = nullptr, uint8_t* __restrict__ out_u8 = nullptr,
+                                           float* __restrict__ out_f32 = nullptr) {
+    if (cells_by_quads(nx))
+        cells_quads<kMode>(density, nx, ny, nz, block, cells, occupancy, lut, out_u8, out_f32);
+    else
+        cells_block<kMode>(density, nx, ny, nz, block, cells, occupancy, lut, out_u8, out_f32);
 }
 
 // The brick distance field of the occupancy buffer (occupancy_words): a
