@@ -231,6 +231,7 @@ int build_group(vkhr_batch* b, const vkhr_batch_style* styles, uint32_t first, u
         a.cursor = w + l.cursor;
         a.status = w + l.status;
         a.minmax = w + l.minmax;
+        a.brick_mm = w + l.brick_mm;
         a.density = reinterpret_cast<uint8_t*>(w + l.raw);
         a.entries = reinterpret_cast<BinEntry*>(w + l.entries);
     }

@@ -79,9 +79,10 @@ batch_brick_kernel(BatchDevice d, const BatchVolStyle* __restrict__ styles) {
 
 __global__ void __launch_bounds__(256)
 batch_overflow_finalize_kernel(BatchDevice d, const BatchVolStyle* __restrict__ styles) {
+    __shared__ uint32_t red[2][256 / kWave];
     const BlockOf w = block_of(d, kPassFinalize);
     const BuildArgs& a = styles[w.style].a;
-    overflow_finalize_body(a, size_t(a.g.nx) * a.g.ny * a.g.nz, w.block, w.blocks);
+    overflow_finalize_body(a, size_t(a.g.nx) * a.g.ny * a.g.nz, w.block, w.blocks, red);
 }
 
 // The corner-packed cells with the normalize applied as the uint8 view

@@ -39,7 +39,8 @@ __global__ void __launch_bounds__(kBrickBlock) brick_kernel(BuildArgs a) {
 }
 
 __global__ void __launch_bounds__(256) overflow_finalize_kernel(BuildArgs a, size_t n) {
-    overflow_finalize_body(a, n, blockIdx.x, gridDim.x);
+    __shared__ uint32_t red[2][256 / kWave];
+    overflow_finalize_body(a, n, blockIdx.x, gridDim.x, red);
 }
 
 // The corner-packed cells from the uint8 view with the normalize applied as
@@ -114,6 +115,7 @@ extern "C" int vkhr_volume_build_segments(const float* xyz, uint32_t n_vertices,
     a.cursor = w + lay.cursor;
     a.status = w + lay.status;
     a.minmax = w + lay.minmax;
+    a.brick_mm = w + lay.brick_mm;
     a.entries = reinterpret_cast<BinEntry*>(w + lay.entries);
     a.capacity = uint32_t(capacity);
     a.occupancy = occupancy;
@@ -138,7 +140,7 @@ extern "C" int vkhr_volume_build_segments(const float* xyz, uint32_t n_vertices,
     }
     brick_kernel<<<brick_blocks(bg), kBrickBlock, 0, s>>>(a);
     VKHR_CHECK_LAUNCH();
-    // the workspace path's finalize: every block returns at once unless the bins overflowed
+    // the min/max of the bricks into the slots, or the workspace path's finalize
     overflow_finalize_kernel<<<1024, 256, 0, s>>>(a, n);
     VKHR_CHECK_LAUNCH();
     if (!mapped)

@@ -80,6 +80,7 @@ struct BuildLayout {
     size_t cursor;      // bin starts, then (after the fill) bin ends
     size_t status;      // [0] total entries, [1] overflow flag
     size_t minmax;      // the min/max slots (kMinMaxBytes)
+    size_t brick_mm;    // per brick: the min/max of the uint8 view of its two waves' voxels (4 bytes)
     size_t raw;         // the uint8 view before the normalize (with cells), n voxels
     size_t entries;     // bin entries (BinEntry)
     size_t words;
@@ -92,7 +93,8 @@ __host__ __forceinline__ BuildLayout build_layout(size_t n_bricks, size_t n_voxe
     l.cursor = up(n_bricks);
     l.status = l.cursor + up(n_bricks);
     l.minmax = l.status + 64;
-    l.raw = l.minmax + up(kMinMaxBytes / 4);
+    l.brick_mm = l.minmax + up(kMinMaxBytes / 4);
+    l.raw = l.brick_mm + up(n_bricks);
     l.entries = l.raw + (raw ? up((n_voxels + 3) / 4) : 0);
     l.words = l.entries + up(sizeof(BinEntry) / 4 * capacity);
     return l;
@@ -124,6 +126,7 @@ struct BuildArgs {
     uint32_t* cursor;
     uint32_t* status;
     uint32_t* minmax;
+    uint32_t* brick_mm;             // bytes lo0, hi0, lo1, hi1 per brick (lo > hi: no voxel)
     BinEntry* entries;
     uint32_t capacity;
     uint32_t* occupancy;            // zeroed by the init pass (nullable)
@@ -427,7 +430,6 @@ __host__ __forceinline__ uint32_t brick_blocks(const BrickGrid& b) {
 struct BrickLds {
     uint32_t count[512];
     int sum[3][512];
-    uint32_t red[2][kBrickBlock / kWave];
 };
 
 __device__ __forceinline__ void brick_body(const BuildArgs& a, BrickLds& l, uint32_t block, uint32_t blocks) {
@@ -478,8 +480,7 @@ __device__ __forceinline__ void brick_body(const BuildArgs& a, BrickLds& l, uint
                 if (with_tangents) reinterpret_cast<char4*>(a.tangent_i8x4)[v] = make_char4(0, 0, 0, 0);
             }
         }
-        if (t == 0)  // the brick holds a voxel of the volume (brick_of_block)
-            fold_minmax(a.minmax + (block % kMinMaxSlots) * kMinMaxStride, 0u, 0u);
+        if (t == 0) a.brick_mm[brick] = 0u;  // the brick holds a voxel of the volume (brick_of_block)
         return;
     }
     reinterpret_cast<uint4*>(l.count)[t] = make_uint4(0u, 0u, 0u, 0u);
@@ -572,27 +573,49 @@ __device__ __forceinline__ void brick_body(const BuildArgs& a, BrickLds& l, uint
         lo = p < lo ? p : lo;
         hi = q > hi ? q : hi;
     }
-    const uint32_t lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-    if (lane == 0) {
-        l.red[0][wave] = lo;
-        l.red[1][wave] = hi;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (uint32_t w = 1; w < kBrickBlock / kWave; ++w) {
-            lo = l.red[0][w] < lo ? l.red[0][w] : lo;
-            hi = l.red[1][w] > hi ? l.red[1][w] : hi;
-        }
-        if (lo <= hi)  // the brick holds a voxel of the volume
-            fold_minmax(a.minmax + (block % kMinMaxSlots) * kMinMaxStride, lo, hi);
-    }
+    // each wave's min/max into its half of the brick's word (folded into
+    // the slots by the finalize pass, off this pass's critical path)
+    if (threadIdx.x % kWave == 0)
+        reinterpret_cast<uint16_t*>(a.brick_mm)[2u * brick + threadIdx.x / kWave] = uint16_t(lo | (hi << 8));
 }
 
-// The finalize of the workspace path (returns at once while the flag is
-// clear): the outputs of the brick pass from the workspace.
-__device__ __forceinline__ void overflow_finalize_body(const BuildArgs& a, size_t n, uint32_t block, uint32_t blocks) {
-    if (!a.status[1]) return;
+// The min/max of the brick pass's words folded into the slots (one fold
+// per workgroup), or, with the overflow flag set, the finalize of the
+// workspace path: the outputs of the brick pass from the workspace.
+__device__ __forceinline__ void overflow_finalize_body(const BuildArgs& a, size_t n, uint32_t block, uint32_t blocks,
+                                                       uint32_t (&red)[2][256 / kWave]) {
     uint32_t lo = 255u, hi = 0u;
+    if (!a.status[1]) {
+        if (size_t(block) * blockDim.x >= a.n_bricks) return;  // workgroup-uniform
+        for (size_t i = size_t(block) * blockDim.x + threadIdx.x; i < a.n_bricks; i += size_t(blocks) * blockDim.x) {
+            const uint32_t w = a.brick_mm[i];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t l0 = (w >> (16 * h)) & 0xffu, h0 = (w >> (16 * h + 8)) & 0xffu;
+                lo = l0 < lo ? l0 : lo;
+                hi = h0 > hi ? h0 : hi;
+            }
+        }
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const uint32_t p = __shfl_xor(lo, o, kWave), q = __shfl_xor(hi, o, kWave);
+            lo = p < lo ? p : lo;
+            hi = q > hi ? q : hi;
+        }
+        const uint32_t wave = threadIdx.x / kWave;
+        if (threadIdx.x % kWave == 0) {
+            red[0][wave] = lo;
+            red[1][wave] = hi;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (uint32_t w = 1; w < blockDim.x / kWave; ++w) {
+                lo = red[0][w] < lo ? red[0][w] : lo;
+                hi = red[1][w] > hi ? red[1][w] : hi;
+            }
+            if (lo <= hi) fold_minmax(a.minmax + (block % kMinMaxSlots) * kMinMaxStride, lo, hi);
+        }
+        return;
+    }
     const size_t stride = size_t(blocks) * blockDim.x;
     const size_t first = size_t(block) * blockDim.x + threadIdx.x;
     for (size_t i = first; i < n; i += stride) {
