@@ -64,15 +64,28 @@ __device__ __forceinline__ uint32_t brick_index(const BrickGrid& b, uint3 v) {
 }
 
 // A bin entry: one pair's run through one brick -- the DDA at the run's
-// first sample (position, steps left before the sample is taken, the step)
-// and the pair's root vertex (its tangent).  The walk on from there makes
-// the same float adds as the walk from the pair's root.
+// first sample (position and step; the walk on from there makes the same
+// float adds as the walk from the pair's root) and a 64-bit word holding
+// the run's sample limit (bits 0-9: the samples left, at most kMaxRun) and
+// the fixed-point tangent of the pair's root (three 18-bit fields, each
+// tangent_fixed + 65536), so the brick pass gathers nothing.  Runs longer
+// than kMaxRun samples (a clamped walk along the grid's edge can stay in
+// one brick) are split into entries of kMaxRun, by the count and the fill
+// pass alike.
 struct BinEntry {
-    float rx, ry, rz, steps;
-    float dx, dy, dz;
-    uint32_t root;
+    float rx, ry, rz, dx;
+    float dy, dz;
+    uint32_t word_lo, word_hi;
 };
 static_assert(sizeof(BinEntry) == 32, "two 16-byte words per entry");
+constexpr uint32_t kMaxRun = 1023;
+
+__device__ __forceinline__ unsigned long long pack_tangent(float3 t) {
+    const unsigned long long x = (unsigned long long)(tangent_fixed(t.x) + 65536);
+    const unsigned long long y = (unsigned long long)(tangent_fixed(t.y) + 65536);
+    const unsigned long long z = (unsigned long long)(tangent_fixed(t.z) + 65536);
+    return (x << 10) | (y << 28) | (z << 46);
+}
 
 // Where the build keeps its state (32-bit words from one base).
 struct BuildLayout {
@@ -175,6 +188,13 @@ struct BinTable {
 
 __device__ __forceinline__ uint32_t table_home(uint32_t k) { return (k * 2654435761u) >> (32 - kTableBits); }
 
+// An add to a table counter at workgroup scope (an LDS atomic): written
+// apart from the device-wide adds of the table misses, which the compiler
+// would otherwise merge with it into one flat atomic on a selected address.
+__device__ __forceinline__ uint32_t table_add(uint32_t* counter, uint32_t v) {
+    return __hip_atomic_fetch_add(counter, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __device__ __forceinline__ uint32_t table_insert(BinTable& t, uint32_t k) {
     const uint32_t h = table_home(k);
     for (uint32_t p = 0; p < kTableProbes; ++p) {
@@ -210,7 +230,7 @@ constexpr uint32_t kPairsPerThread = 4;
 
 struct PairWalks {
     SegmentDda w[kPairsPerThread];
-    uint32_t root[kPairsPerThread];  // vertex index of the pair's root (its tangent)
+    unsigned long long tangent[kPairsPerThread];  // pack_tangent of the pair's root (0 without tangents)
 };
 
 __device__ __forceinline__ void load_walks(const BuildArgs& a, uint32_t block, PairWalks& p) {
@@ -225,23 +245,28 @@ __device__ __forceinline__ void load_walks(const BuildArgs& a, uint32_t block, P
 #pragma unroll
     for (uint32_t j = 0; j < kPairsPerThread; ++j) {
         p.w[j] = segment_dda(a.g, a.xyz, a.n_vertices, pr[j].x, pr[j].y);  // kBuildNone: no samples
-        p.root[j] = pr[j].x;
+        p.tangent[j] = a.tangent_i8x4 && p.w[j].steps > 0.0f ? pack_tangent(load_vertex(a.tangents, pr[j].x)) : 0ull;
     }
 }
 
 // Calls run(j, DDA at the sample, steps before the sample, brick) at the
-// first sample of each brick run of each of the thread's pairs.
+// first sample of each brick run (split every kMaxRun samples) of each of
+// the thread's pairs.
 template <class Run>
 __device__ __forceinline__ void walk_runs(const BuildArgs& a, const PairWalks& p, Run run) {
 #pragma unroll
     for (uint32_t j = 0; j < kPairsPerThread; ++j) {
         SegmentDda w = p.w[j];
-        uint32_t last = kBuildNone;
+        uint32_t last = kBuildNone, len = 0;
         while (w.steps > 0.0f) {
             const float before = w.steps;
             w.steps -= 1.0f;
             const uint32_t k = brick_index(a.b, voxel_coords(a.g, w.rx, w.ry, w.rz));
-            if (k != last) run(j, w, before, k);
+            if (k != last || len == kMaxRun) {
+                run(j, w, before, k);
+                len = 0;
+            }
+            ++len;
             last = k;
             w.advance();
         }
@@ -334,21 +359,25 @@ __device__ __forceinline__ void bin_count_body(const BuildArgs& a, BinTable& t, 
     }
     table_clear(t);
     __syncthreads();
-    const auto count = [&](uint32_t k) {
+    const auto count = [&](uint32_t k) {  // one run (walk_runs' split)
         const uint32_t slot = table_insert(t, k);
         if (slot != kBuildNone)
-            atomicAdd(&t.count[slot], 1u);
+            table_add(&t.count[slot], 1u);
         else
             atomicAdd(&a.bins[k], 1u);
     };
 #pragma unroll
     for (uint32_t j = 0; j < kPairsPerThread; ++j) {
         SegmentDda w = segment_dda(a.g, a.xyz, a.n_vertices, pr[j].x, pr[j].y);  // kBuildNone: no samples
-        uint32_t last = kBuildNone;
+        uint32_t last = kBuildNone, len = 0;
         while (w.steps > 0.0f) {
             w.steps -= 1.0f;
             const uint32_t k = brick_index(a.b, voxel_coords(a.g, w.rx, w.ry, w.rz));
-            if (k != last) count(k);
+            if (k != last || len == kMaxRun) {
+                count(k);
+                len = 0;
+            }
+            ++len;
             last = k;
             w.advance();
         }
@@ -379,7 +408,7 @@ __device__ __forceinline__ void bin_fill_body(const BuildArgs& a, BinTable& t, u
     __syncthreads();
     walk_runs(a, p, [&](uint32_t, const SegmentDda&, float, uint32_t k) {
         const uint32_t slot = table_insert(t, k);
-        if (slot != kBuildNone) atomicAdd(&t.count[slot], 1u);
+        if (slot != kBuildNone) table_add(&t.count[slot], 1u);
     });
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < kTableSlots; i += blockDim.x) {
@@ -390,11 +419,15 @@ __device__ __forceinline__ void bin_fill_body(const BuildArgs& a, BinTable& t, u
     walk_runs(a, p, [&](uint32_t j, const SegmentDda& w, float before, uint32_t k) {
         const uint32_t slot = table_find(t, k);
         const uint32_t pos =
-            slot != kBuildNone ? t.base[slot] + atomicAdd(&t.count[slot], 1u) : atomicAdd(&a.cursor[k], 1u);
+            slot != kBuildNone ? t.base[slot] + table_add(&t.count[slot], 1u) : atomicAdd(&a.cursor[k], 1u);
         if (pos < a.capacity) {
+            // samples left (this one included): the walk's iterations, ceil(before)
+            const float left = ceilf(before);
+            const unsigned long long word =
+                p.tangent[j] | (left < float(kMaxRun) ? (unsigned long long)(left) : (unsigned long long)(kMaxRun));
             float4* e = reinterpret_cast<float4*>(a.entries + pos);
-            e[0] = make_float4(w.rx, w.ry, w.rz, before);
-            e[1] = make_float4(w.dx, w.dy, w.dz, __uint_as_float(p.root[j]));
+            e[0] = make_float4(w.rx, w.ry, w.rz, w.dx);
+            e[1] = make_float4(w.dy, w.dz, __uint_as_float(uint32_t(word)), __uint_as_float(uint32_t(word >> 32)));
         }
     });
 }
@@ -488,45 +521,41 @@ __device__ __forceinline__ void brick_body(const BuildArgs& a, BrickLds& l, uint
     reinterpret_cast<int4*>(l.sum[1])[t] = make_int4(0, 0, 0, 0);
     reinterpret_cast<int4*>(l.sum[2])[t] = make_int4(0, 0, 0, 0);
     __syncthreads();
-    // kEntriesInFlight entries per thread per round: their entry and tangent
-    // loads are issued before the first walk
+    // kEntriesInFlight entries per thread per round: their loads are issued
+    // before the first walk
     constexpr uint32_t kEntriesInFlight = 2;
     for (uint32_t e0 = end - n + t; e0 < end; e0 += kEntriesInFlight * kBrickBlock) {
-        SegmentDda w[kEntriesInFlight];
-        int tx[kEntriesInFlight], ty[kEntriesInFlight], tz[kEntriesInFlight];
+        float4 p0[kEntriesInFlight], p1[kEntriesInFlight];
 #pragma unroll
         for (uint32_t u = 0; u < kEntriesInFlight; ++u) {
             const uint32_t e = e0 + u * kBrickBlock;
-            float4 p0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p1 = p0;  // steps 0: no samples
+            p0[u] = p1[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // a sample limit of 0: no samples
             if (e < end) {
                 const float4* en = reinterpret_cast<const float4*>(a.entries + e);
-                p0 = en[0];
-                p1 = en[1];
-            }
-            w[u] = {p0.x, p0.y, p0.z, p1.x, p1.y, p1.z, p0.w};
-            tx[u] = ty[u] = tz[u] = 0;
-            if (with_tangents && e < end) {
-                const float3 tan = load_xyz(a.tangents, __float_as_uint(p1.w));
-                tx[u] = int(tangent_fixed(tan.x));
-                ty[u] = int(tangent_fixed(tan.y));
-                tz[u] = int(tangent_fixed(tan.z));
+                p0[u] = en[0];
+                p1[u] = en[1];
             }
         }
 #pragma unroll
         for (uint32_t u = 0; u < kEntriesInFlight; ++u) {
+            SegmentDda w = {p0[u].x, p0[u].y, p0[u].z, p0[u].w, p1[u].x, p1[u].y, 0.0f};
+            const unsigned long long word =
+                (unsigned long long)__float_as_uint(p1[u].z) | ((unsigned long long)__float_as_uint(p1[u].w) << 32);
+            const uint32_t limit = uint32_t(word & 1023ull);
+            const int tx = int((word >> 10) & 0x3ffffull) - 65536, ty = int((word >> 28) & 0x3ffffull) - 65536,
+                      tz = int((word >> 46) & 0x3ffffull) - 65536;
             // the run starts inside the brick and, once out, never comes back
-            while (w[u].steps > 0.0f) {
-                w[u].steps -= 1.0f;
-                const uint3 v = voxel_coords(a.g, w[u].rx, w[u].ry, w[u].rz);
+            for (uint32_t k = 0; k < limit; ++k) {
+                const uint3 v = voxel_coords(a.g, w.rx, w.ry, w.rz);
                 if ((v.x >> 3) != bxo || (v.y >> 3) != byo || (v.z >> 3) != bzo) break;
                 const uint32_t i = (v.x & 7u) + 8u * ((v.y & 7u) + 8u * (v.z & 7u));
                 atomicAdd(&l.count[i], 1u);
                 if (with_tangents) {  // wraps only past 32767 contributions, where the tangent is 0
-                    atomicAdd(reinterpret_cast<unsigned*>(&l.sum[0][i]), unsigned(tx[u]));
-                    atomicAdd(reinterpret_cast<unsigned*>(&l.sum[1][i]), unsigned(ty[u]));
-                    atomicAdd(reinterpret_cast<unsigned*>(&l.sum[2][i]), unsigned(tz[u]));
+                    atomicAdd(reinterpret_cast<unsigned*>(&l.sum[0][i]), unsigned(tx));
+                    atomicAdd(reinterpret_cast<unsigned*>(&l.sum[1][i]), unsigned(ty));
+                    atomicAdd(reinterpret_cast<unsigned*>(&l.sum[2][i]), unsigned(tz));
                 }
-                w[u].advance();
+                w.advance();
             }
         }
     }

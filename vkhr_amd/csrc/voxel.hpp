@@ -54,6 +54,15 @@ struct SegmentDda {
     }
 };
 
+// Vertex i of a packed float3 array as one 12-byte load (three dwords of
+// 4-byte alignment) rather than three.
+typedef float packed_float3 __attribute__((ext_vector_type(3), aligned(4)));
+
+__device__ __forceinline__ float3 load_vertex(const float* __restrict__ p, uint32_t i) {
+    const packed_float3 v = *reinterpret_cast<const packed_float3*>(p + 3u * i);
+    return make_float3(v.x, v.y, v.z);
+}
+
 // The set-up of the reference's DDA; a malformed pair, or a non-finite or
 // absurd length (vertices far outside the given box, which would never
 // finish), gives no samples (steps = 0).
@@ -61,8 +70,8 @@ __device__ __forceinline__ SegmentDda segment_dda(const VoxelGrid& g, const floa
                                                   uint32_t n_vertices, uint32_t ia, uint32_t ib) {
     SegmentDda w = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     if (!(ia < n_vertices && ib < n_vertices)) return w;
-    float3 a = load_xyz(xyz, ia);
-    float3 b = load_xyz(xyz, ib);
+    float3 a = load_vertex(xyz, ia);
+    float3 b = load_vertex(xyz, ib);
     // root / tip in voxel units (hair_style.cc:312-313)
     w.rx = (a.x - g.ox) / g.vx; w.ry = (a.y - g.oy) / g.vy; w.rz = (a.z - g.oz) / g.vz;
     float tx = (b.x - g.ox) / g.vx, ty = (b.y - g.oy) / g.vy, tz = (b.z - g.oz) / g.vz;
