@@ -180,6 +180,28 @@ def test_empty_and_degenerate_inputs(device):
         assert int(got["density"].sum()) == 0   # a constant volume normalizes to 0
 
 
+def test_runs_split_past_1023_samples(device):
+    """Vertices far outside the given box: the walk is clamped to the grid's
+    edge, so one pair stays in one brick for ~80,000 samples.  The count
+    and fill passes split such runs into bin entries of at most 1023
+    samples; the volume equals the two-call path's and the oracle's counts
+    (voxels past 32767 samples get tangent 0 on both paths)."""
+    res = (16, 16, 16)
+    b = dict(origin=np.zeros(3, np.float32), size=np.ones(3, np.float32), radius=np.float32(1.7),
+             volume=np.float32(1.0))
+    verts = np.array([[-5000.0, -5.0, 0.3], [5000.0, -5.0, 0.3], [0.2, 0.3, 0.4], [0.7, 0.6, 0.5],
+                      [0.5, 3000.0, 0.5], [0.5, -3000.0, 0.6]], np.float32)
+    tans = np.array([[1, 0, 0], [1, 0, 0], [0.6, 0.8, 0], [0.6, 0.8, 0], [0, -1, 0], [0, -1, 0]], np.float32)
+    idx = np.array([0, 1, 2, 3, 4, 5], np.uint32)
+    xyz, tan, di = D.to_dev(verts, device), D.to_dev(tans, device), D.to_dev(idx, device)
+    got = _build(xyz, b, res, di, tan, True)
+    _compare(got, _two_call(xyz, b, res, di, tan, True), "clamped runs")
+    ref = OV.voxelize(verts, b, res, indices=idx)
+    counts = got["counts"].cpu().numpy().view(np.uint32)
+    assert np.array_equal(counts, ref["counts"])
+    assert counts.max() > 32767
+
+
 def test_invalid_arguments_are_errors(device):
     buf = torch.zeros(1 << 16, dtype=torch.int64, device=device)
     p = capi.ptr(buf)
