@@ -1,11 +1,12 @@
 """Benchmark of the hybrid-hair hot path on one MI355X.
 
 Workload (N=1): 136,320 strands x 13 vertices (1,635,840 segments) ->
-segment voxelization into 256^3 counts packed with fixed-point tangent sums -> finalize
-(saturating view, normalize, tangent quantization, corner-packed cells) ->
-volume raymarch at 1920x1080, 512 steps, Kajiya-Kay + ADSM + AO.  Inputs
-(vertices, indices, tangents, camera) are resident in HBM before the timed
-region; a "step" is one pass voxelize -> finalize -> raymarch.
+segment voxelization into 256^3 with the tangent sums -> finalize
+(saturating view, normalize, tangent quantization, corner-packed cells,
+brick occupancy) in one call of the LDS-binned build -> volume raymarch at
+1920x1080, 512 steps, Kajiya-Kay + ADSM + AO.  Inputs (vertices, indices in
+the load-time spatial order, tangents, camera) are resident in HBM before
+the timed region; a "step" is one pass build -> raymarch.
 
 value = covered rays per second of the whole step (Mrays/s); the per-kernel
 throughputs (raymarch Mrays/s, segments voxelized/s) are reported beside it.
