@@ -723,15 +723,18 @@ def main():
     off = D.to_dev(style.strand_offsets, dev)
     tan = D.generate_tangents(xyz, off)
     bounds = D.aabb(xyz)
-    # load-time layout: the generated pairs in the Morton order of the segment
-    # midpoints (a permutation: every pass gives the same results on it); the
-    # first call also initialises the sort, so a warm call is timed
+    # load-time layouts (permutations of the pairs: every pass gives the same
+    # results on them): for the step's volume build the strands in the
+    # Morton order of their roots, each strand's pairs together; for the
+    # strand passes the pairs in the Morton order of their midpoints.  The
+    # first calls also initialise the sorts, so warm calls are timed.
     idx_generated = D.generate_indices(off)
-    idx = D.sort_segments(xyz, idx_generated, bounds)
+    idx = D.sort_strands(xyz, idx_generated, bounds)
+    idx_mid = D.sort_segments(xyz, idx_generated, bounds)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(3):
-        idx = D.sort_segments(xyz, idx_generated, bounds)
+        idx = D.sort_strands(xyz, idx_generated, bounds)
     torch.cuda.synchronize()
     t_sort = (time.perf_counter() - t0) / 3
     del idx_generated
@@ -888,13 +891,14 @@ def main():
         "voxelize_ms": t_vox * 1e3,
         "voxelize_atomics": voxelize_atomics,
         "voxelize_roofline": voxelize_roofline,
-        "load": {"sort_segments_ms": t_sort * 1e3,
-                 "note": "once per style at load, outside the step: the index pairs in the Morton order of the "
-                         "segment midpoints, which every pass here draws"},
+        "load": {"sort_strands_ms": t_sort * 1e3,
+                 "note": "once per style at load, outside the step: the index pairs strand by strand, the strands "
+                         "in the Morton order of their roots, which the step's build draws; the strand passes "
+                         "(raster, shadow, tracer) draw the pairs in the Morton order of their midpoints"},
         "roofline": raymarch_roofline(t_ray, ray_bytes),
     }
     if not args.no_extras:
-        t_sh, t_ra, t_re, n_frag = time_raster(dev, style, xyz, idx, tan, vol, res, scene, cam, light, bounds)
+        t_sh, t_ra, t_re, n_frag = time_raster(dev, style, xyz, idx_mid, tan, vol, res, scene, cam, light, bounds)
         # algorithmic work per fragment: 9 shadow taps x 4 B, 8 AO taps x 8
         # texels x 1 B, the 12-byte node; per segment the two vertices'
         # position, tangent and thickness; one head exchange per fragment
@@ -906,14 +910,14 @@ def main():
                             "hbm_frac": ra_bytes / (t_ra * 1e-3) / 1e9 / HBM_PEAK_GBS,
                             "head_exchanges_per_s": n_frag / (t_ra * 1e-3),
                             "atomic_frac": n_frag / (t_ra * 1e-3) / peak["u32_uniform"]}
-        frames = time_reference_frames(dev, style, xyz, idx, tan, vol, res, bounds)
+        frames = time_reference_frames(dev, style, xyz, idx_mid, tan, vol, res, bounds)
         result["reference_setup_720p"] = {
             "workload": "136,320 strands x 13 vertices, 1280x720, 1024^2 3x3 PCF shadow map, 256^3, 512 steps",
             "rasterizer_frame_ms": frames["rasterizer"], "raymarcher_frame_ms": frames["raymarcher"],
             "published": "ponytail rasterized 7 ms per frame on a Radeon Pro WX 9100 (readme.md:8); the "
                          "raymarcher about 2x faster near (docs/vkhr.tex:403) -- other hardware, not a ratio"}
-        result["raytrace"] = time_trace(dev, style, xyz, idx, tan, scene, bounds)
-        result["ao_ground_truth"] = ao_ground_truth(dev, style, xyz, idx, tan, bounds, res)
+        result["raytrace"] = time_trace(dev, style, xyz, idx_mid, tan, scene, bounds)
+        result["ao_ground_truth"] = ao_ground_truth(dev, style, xyz, idx_mid, tan, bounds, res)
         t_f, t_b = time_adjoint(dev)
         # (t_fwd + t_bwd) / t_fwd: the forward and the backward of one
         # gradient evaluation against the forward alone (target <= 3)

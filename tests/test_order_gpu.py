@@ -126,3 +126,58 @@ def test_passes_are_unchanged_on_the_sorted_order(device):
     assert np.array_equal(frags[0][0], frags[1][0])
     assert np.array_equal(frags[0][1], frags[1][1]) and len(frags[0][1]) > 100000
     assert np.array_equal(frags[0][2], frags[1][2])
+
+
+def _strand_runs(pairs):
+    """Start of each pair's run: a pair continues the previous one when its
+    first index is the previous pair's second."""
+    starts = np.ones(len(pairs), bool)
+    starts[1:] = pairs[1:, 0] != pairs[:-1, 1]
+    return np.maximum.accumulate(np.where(starts, np.arange(len(pairs)), 0))
+
+
+def test_strand_order_is_runs_in_morton_order_of_their_roots(device):
+    """vkhr_sort_strands: a permutation of the pairs in which every strand
+    run of the input stays whole and in order, the runs ordered by the
+    documented code of their root vertex (non-decreasing; equal codes in
+    input order); malformed pairs last; the build's volume is unchanged."""
+    s = synth.strands_10k()
+    xyz = D.to_dev(s.vertices, device)
+    off = D.to_dev(s.strand_offsets, device)
+    idx = D.generate_indices(off)
+    b = D.aabb(xyz)
+    out = D.sort_strands(xyz, idx, b).cpu().numpy().view(np.uint32).reshape(-1, 2)
+    pairs = idx.cpu().numpy().view(np.uint32).reshape(-1, 2)
+    assert np.array_equal(_keys(out), _keys(pairs))
+    first = _strand_runs(pairs)
+    roots = pairs[first, 0]
+    codes = _codes(s.vertices, np.stack([roots, roots], axis=1), b)   # the midpoint of (r, r) is r
+    expect = pairs[np.argsort(codes, kind="stable")]
+    assert np.array_equal(out, expect)
+    # the runs of the output are the strands of the input, whole
+    assert len(np.unique(_strand_runs(out))) == len(np.unique(first))
+    # the build's outputs do not depend on the order
+    tan = D.generate_tangents(xyz, off)
+    res = (64, 64, 64)
+    a = D.build_volume(xyz, b, res, idx, tan, normalize=True, want_counts=True)
+    c = D.build_volume(xyz, b, res, D.sort_strands(xyz, idx, b), tan, normalize=True, want_counts=True)
+    for k in ("counts", "density", "tangents", "cells", "occupancy"):
+        assert torch.equal(a[k], c[k]), k
+
+
+def test_strand_order_of_malformed_and_broken_runs(device):
+    s = synth.hemisphere_style(40, 5, seed=4)
+    xyz = D.to_dev(s.vertices, device)
+    idx = D.generate_indices(D.to_dev(s.strand_offsets, device)).clone()
+    n_v = s.n_vertices
+    idx[0] = n_v + 3            # the first strand's root is malformed: its run sorts last
+    idx[8] = n_v                # a malformed second index breaks its strand into two runs
+    b = D.aabb(xyz)
+    out = D.sort_strands(xyz, idx, b).cpu().numpy().view(np.uint32).reshape(-1, 2)
+    pairs = idx.cpu().numpy().view(np.uint32).reshape(-1, 2)
+    assert np.array_equal(_keys(out), _keys(pairs))
+    first = _strand_runs(pairs)
+    roots = pairs[first, 0]
+    codes = np.where(roots < n_v, _codes(s.vertices, np.stack([np.minimum(roots, n_v - 1)] * 2, axis=1), b),
+                     np.uint64(1 << 30))
+    assert np.array_equal(out, pairs[np.argsort(codes, kind="stable")])

@@ -34,9 +34,10 @@ class StyleBuffers:
     def __init__(self, style: synth.Style, n: int, width: int, height: int, device, azimuth: float = 0.0,
                  steps: float = 512.0, with_counts: bool = True, morton: bool = True):
         """with_counts=False writes no separate uint32 count volume.
-        morton: the index pairs in the Morton order of their midpoints
-        (vkhr_sort_segments, once at load), the order the binned volume
-        build runs fastest on; the volumes do not depend on the order."""
+        morton: the index pairs strand by strand with the strands in the
+        Morton order of their roots (vkhr_sort_strands, once at load), the
+        order the binned volume build runs fastest on; the volumes do not
+        depend on the order."""
         self.n = n
         self.xyz = D.to_dev(style.vertices, device)
         off = D.to_dev(style.strand_offsets, device)
@@ -44,7 +45,7 @@ class StyleBuffers:
         self.bounds = D.aabb(self.xyz)
         self.idx = D.generate_indices(off)
         if morton and self.idx.numel() >= 2:
-            self.idx = D.sort_segments(self.xyz, self.idx, self.bounds)
+            self.idx = D.sort_strands(self.xyz, self.idx, self.bounds)
         nv = n ** 3
         self.counts = torch.empty(nv, dtype=torch.int32, device=device) if with_counts else None
         self.acc = torch.empty(2 * nv, dtype=torch.int64, device=device)

@@ -124,6 +124,7 @@ SIGNATURES = {
     "vkhr_aabb_compute": [P, U32, C.POINTER(AABB), P],
     "vkhr_generate_indices": [P, U32, P, P],
     "vkhr_sort_segments": [P, U32, P, U32, C.POINTER(AABB), P, P],
+    "vkhr_sort_strands": [P, U32, P, U32, C.POINTER(AABB), P, P],
     "vkhr_generate_tangents": [P, P, U32, P, P],
     "vkhr_voxelize_vertices": [P, U32, P, C.POINTER(AABB), U32, U32, U32, P, P, P],
     "vkhr_voxelize_segments": [P, U32, P, U32, P, C.POINTER(AABB), U32, U32, U32, P, P, P],

@@ -46,6 +46,16 @@ def sort_segments(xyz: torch.Tensor, indices: torch.Tensor, bounds: dict) -> tor
     return out
 
 
+def sort_strands(xyz: torch.Tensor, indices: torch.Tensor, bounds: dict) -> torch.Tensor:
+    """The index pairs strand by strand, the strands in the Morton order of
+    their root vertex (a permutation of the pairs, made once at load)."""
+    out = torch.empty_like(indices)
+    b = capi.make_aabb(bounds)
+    call("vkhr_sort_strands", ptr(xyz), xyz.shape[0], ptr(indices), indices.shape[0], C.byref(b), ptr(out),
+         stream_of())
+    return out
+
+
 def generate_tangents(xyz: torch.Tensor, offsets: torch.Tensor) -> torch.Tensor:
     out = torch.zeros_like(xyz)
     call("vkhr_generate_tangents", ptr(xyz), ptr(offsets), offsets.shape[0] - 1, ptr(out), stream_of())
