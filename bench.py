@@ -903,13 +903,18 @@ def main():
         # texels x 1 B, the 12-byte node; per segment the two vertices'
         # position, tangent and thickness; one head exchange per fragment
         ra_bytes = (36 + 64 + 12) * n_frag + 2 * (12 + 12 + 4) * n_seg
+        re_bytes = (4 + 16) * W * H + 12 * n_frag
         result["raster"] = {"workload": "1,635,840 segments, 1920x1080, 1024^2 shadow map, k=16, 2^25-node PPLL",
                             "shadow_ms": t_sh, "raster_ms": t_ra, "resolve_ms": t_re, "fragments": n_frag,
                             "fragments_per_s": n_frag / (t_ra * 1e-3),
                             "algorithmic_bytes": ra_bytes, "achieved_gbs": ra_bytes / (t_ra * 1e-3) / 1e9,
                             "hbm_frac": ra_bytes / (t_ra * 1e-3) / 1e9 / HBM_PEAK_GBS,
                             "head_exchanges_per_s": n_frag / (t_ra * 1e-3),
-                            "atomic_frac": n_frag / (t_ra * 1e-3) / peak["u32_uniform"]}
+                            "atomic_frac": n_frag / (t_ra * 1e-3) / peak["u32_uniform"],
+                            # the resolve: per pixel a 4-byte head and a 16-byte colour out, per
+                            # node 12 bytes, each node a dependent load of the pixel's list
+                            "resolve_bytes": re_bytes, "resolve_achieved_gbs": re_bytes / (t_re * 1e-3) / 1e9,
+                            "resolve_hbm_frac": re_bytes / (t_re * 1e-3) / 1e9 / HBM_PEAK_GBS}
         frames = time_reference_frames(dev, style, xyz, idx_mid, tan, vol, res, bounds)
         result["reference_setup_720p"] = {
             "workload": "136,320 strands x 13 vertices, 1280x720, 1024^2 3x3 PCF shadow map, 256^3, 512 steps",
