@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--batched-only", action="store_true",
                     help="only time the 32-style batched leg against its per-style sequences (for a kernel "
                          "profile of that leg), print its JSON object and exit")
+    ap.add_argument("--strand-passes-only", action="store_true",
+                    help="only time the strand passes (shadow map, colour pass, resolve) at the bench's workload "
+                         "(for a kernel profile of them), print their JSON object and exit")
     return ap.parse_args()
 
 
@@ -168,6 +171,47 @@ def time_raster(dev, style, xyz, idx, tan, vol, res, scene, cam, light, bounds, 
         torch.cuda.synchronize()
         times.append(e0.elapsed_time(e1) / reps)
     return times[0], times[1], times[2], min(ppll.used(), ppll.node_count)
+
+
+def raster_result(t_sh, t_ra, t_re, n_frag, n_seg, W, H, atomic_peak_per_s) -> dict:
+    """The strand passes' JSON object (times from time_raster)."""
+    # algorithmic work per fragment: 9 shadow taps x 4 B, 8 AO taps x 8
+    # texels x 1 B, the 12-byte node; per segment the two vertices'
+    # position, tangent and thickness; one head exchange per fragment
+    ra_bytes = (36 + 64 + 12) * n_frag + 2 * (12 + 12 + 4) * n_seg
+    re_bytes = (4 + 16) * W * H + 12 * n_frag
+    out = {"workload": f"{n_seg:,} segments, {W}x{H}, 1024^2 shadow map, k=16, 2^25-node PPLL",
+           "shadow_ms": t_sh, "raster_ms": t_ra, "resolve_ms": t_re, "fragments": n_frag,
+           "fragments_per_s": n_frag / (t_ra * 1e-3),
+           "algorithmic_bytes": ra_bytes, "achieved_gbs": ra_bytes / (t_ra * 1e-3) / 1e9,
+           "hbm_frac": ra_bytes / (t_ra * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "head_exchanges_per_s": n_frag / (t_ra * 1e-3),
+           # the resolve: per pixel a 4-byte head and a 16-byte colour out, per
+           # node 12 bytes, each node a dependent load of the pixel's list
+           "resolve_bytes": re_bytes, "resolve_achieved_gbs": re_bytes / (t_re * 1e-3) / 1e9,
+           "resolve_hbm_frac": re_bytes / (t_re * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if atomic_peak_per_s:
+        out["atomic_frac"] = n_frag / (t_ra * 1e-3) / atomic_peak_per_s
+    return out
+
+
+def strand_passes_only(dev) -> dict:
+    """The strand passes alone at the bench's workload (136k strands, the
+    256^3 volume for the LAO, 1080p), for a kernel profile of them."""
+    from vkhr_amd import params, synth
+    from vkhr_amd import device as D
+    style = synth.strands_136k()
+    res = (256, 256, 256)
+    xyz = D.to_dev(style.vertices, dev)
+    off = D.to_dev(style.strand_offsets, dev)
+    tan = D.generate_tangents(xyz, off)
+    bounds = D.aabb(xyz)
+    idx_mid = D.sort_segments(xyz, D.generate_indices(off), bounds)
+    vol = D.build_volume(xyz, bounds, res, idx_mid, tan, normalize=True, want_cells=True)
+    scene = synth.make_scene(bounds, 1920, 1080)
+    cam, light = params.camera(scene), params.light(scene)
+    t_sh, t_ra, t_re, n_frag = time_raster(dev, style, xyz, idx_mid, tan, vol, res, scene, cam, light, bounds)
+    return raster_result(t_sh, t_ra, t_re, n_frag, idx_mid.shape[0] // 2, scene.width, scene.height, None)
 
 
 def time_trace(dev, style, xyz, idx, tan, scene, bounds, frames=4):
@@ -708,6 +752,12 @@ def main():
         torch.cuda.set_device(dev)
         print(json.dumps({"batched": batched_result(*time_batch(dev))}))
         return
+    if args.strand_passes_only:
+        import torch
+        dev = torch.device("cuda:0")
+        torch.cuda.set_device(dev)
+        print(json.dumps({"raster": strand_passes_only(dev)}))
+        return
     import numpy as np
     import torch
 
@@ -899,22 +949,7 @@ def main():
     }
     if not args.no_extras:
         t_sh, t_ra, t_re, n_frag = time_raster(dev, style, xyz, idx_mid, tan, vol, res, scene, cam, light, bounds)
-        # algorithmic work per fragment: 9 shadow taps x 4 B, 8 AO taps x 8
-        # texels x 1 B, the 12-byte node; per segment the two vertices'
-        # position, tangent and thickness; one head exchange per fragment
-        ra_bytes = (36 + 64 + 12) * n_frag + 2 * (12 + 12 + 4) * n_seg
-        re_bytes = (4 + 16) * W * H + 12 * n_frag
-        result["raster"] = {"workload": "1,635,840 segments, 1920x1080, 1024^2 shadow map, k=16, 2^25-node PPLL",
-                            "shadow_ms": t_sh, "raster_ms": t_ra, "resolve_ms": t_re, "fragments": n_frag,
-                            "fragments_per_s": n_frag / (t_ra * 1e-3),
-                            "algorithmic_bytes": ra_bytes, "achieved_gbs": ra_bytes / (t_ra * 1e-3) / 1e9,
-                            "hbm_frac": ra_bytes / (t_ra * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                            "head_exchanges_per_s": n_frag / (t_ra * 1e-3),
-                            "atomic_frac": n_frag / (t_ra * 1e-3) / peak["u32_uniform"],
-                            # the resolve: per pixel a 4-byte head and a 16-byte colour out, per
-                            # node 12 bytes, each node a dependent load of the pixel's list
-                            "resolve_bytes": re_bytes, "resolve_achieved_gbs": re_bytes / (t_re * 1e-3) / 1e9,
-                            "resolve_hbm_frac": re_bytes / (t_re * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        result["raster"] = raster_result(t_sh, t_ra, t_re, n_frag, n_seg, W, H, peak["u32_uniform"])
         frames = time_reference_frames(dev, style, xyz, idx_mid, tan, vol, res, bounds)
         result["reference_setup_720p"] = {
             "workload": "136,320 strands x 13 vertices, 1280x720, 1024^2 3x3 PCF shadow map, 256^3, 512 steps",
