@@ -219,6 +219,51 @@ def test_pool_overflow_drops_without_out_of_bounds(c0, device):
     ppll.resolve()
 
 
+def _list_nodes(ppll):
+    """Node ids reachable from the heads (each list walked to its end)."""
+    heads = ppll.heads.cpu().numpy().view(np.uint32)
+    nxt = ppll.nodes.cpu().numpy().view(np.uint32).reshape(-1, 3)[:, 2]
+    cur = heads[heads != NULL].astype(np.int64)
+    seen = []
+    while len(cur):
+        assert (cur < ppll.node_count).all(), "a list links a node past the pool"
+        seen.append(cur)
+        cur = nxt[cur].astype(np.int64)
+        cur = cur[cur != NULL]
+    return np.concatenate(seen) if seen else np.zeros(0, np.int64)
+
+
+def test_pool_between_linked_and_reserved_counts(c0, device):
+    # The colour pass reserves a node id for every covered candidate, and
+    # leaves the ids of fragments its shading discards unlinked, so the
+    # counter runs ahead of the linked nodes.  With a pool between the two
+    # counts, fragments whose id falls past its end are dropped though
+    # enough ids were left unused below it.  Pinned here: the lists hold
+    # exactly the captured nodes below the pool end, each once; every one of
+    # them is a fragment of the unbounded run; the drops are no more than
+    # the ids past the end; the resolve reads the lists as they are.
+    big, cap_big = c0.raster(1 << 22)
+    reserved = big.used()
+    linked_big = int((cap_big["segment"][:reserved] != -1).sum())
+    assert reserved > linked_big, "no discarded candidate at this view"
+    key = lambda seg, pix: seg.astype(np.uint64) << np.uint64(32) | pix.astype(np.uint64)
+    sel = cap_big["segment"][:reserved].cpu().numpy() != -1
+    frags_big = key(cap_big["segment"][:reserved].cpu().numpy()[sel].view(np.uint32),
+                    cap_big["pixel"][:reserved].cpu().numpy()[sel].view(np.uint32))
+    n = linked_big + (reserved - linked_big) // 2
+    small, cap = c0.raster(n)
+    assert small.used() == reserved   # the same reservations, whatever the pool
+    seg = cap["segment"].cpu().numpy()
+    linked = np.flatnonzero(seg != -1)
+    listed = _list_nodes(small)
+    assert len(np.unique(listed)) == len(listed), "a node linked twice"
+    assert np.array_equal(np.sort(listed), linked), "the lists and the captured nodes differ"
+    frags = key(seg[linked].view(np.uint32), cap["pixel"].cpu().numpy()[linked].view(np.uint32))
+    assert np.isin(frags, frags_big).all()
+    assert 0 < len(linked) <= n and linked_big - len(linked) <= reserved - n
+    _resolve_parity(small)
+
+
 def test_size_limits_are_errors(c0, device):
     """Shadow maps beyond 16384 texels a side and volumes of 2^32 cells or
     more are refused before any launch (the taps use 32-bit offsets)."""
