@@ -2,7 +2,8 @@
 strands log-uniform, 8-32 vertices per strand, 64^3/128^3/256^3 volumes by
 strand-count tertile) voxelized, finalized and raymarched at 960x540 by one
 graph-captured launch; every output bit-identical to the per-style
-sequence.  Also: graph replays reproduce the first launch, and the tuning
+sequence, and one style per volume size against the oracle (voxelization and
+image).  Also: graph replays reproduce the first launch, and the tuning
 table's shapes are valid."""
 import numpy as np
 import pytest
@@ -85,6 +86,48 @@ def test_batch_matches_per_style_one_call_build(batch32):
         for k, mine in [("counts", b.counts), ("density", b.density), ("tangents", b.tangents), ("cells", b.cells),
                         ("occupancy", b.occupancy), ("rgba", b.rgba)]:
             assert torch.equal(mine, out[k]), (k, b.n)
+
+
+def test_batch_volumes_and_images_match_the_oracle(batch32):
+    """The batch's outputs against the oracle directly, at the batch's own
+    shapes, for the first style of each volume size (64^3, 128^3, 256^3):
+    its counts equal the oracle's voxelization of the style, its uint8
+    density the oracle's normalize of them, its tangents the reference's
+    quantization within 1 LSB (no voxel of these styles saturates); and at
+    960x540, 512 steps, the per-style raymarch of the batch's volume (with
+    the depth and iso-decision outputs the parity check reads) equals the
+    batch's image bit for bit, which the float64 oracle, run on the batch's
+    volumes, checks at 6,000 random pixels plus 2,000 covered ones."""
+    from oracle import voxel as OV
+    from tests.parity import compare_raymarch
+    from vkhr_amd import device as D
+    styles, _ = synth.batch_styles()
+    batch = Batch(batch32, W, H, tuning=load_tuning("batch"))
+    batch.launch()
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(11)
+    for n in (64, 128, 256):
+        i = next(k for k, x in enumerate(batch32) if x.n == n)
+        b, st = batch32[i], styles[i]
+        res = (n, n, n)
+        ref = OV.voxelize(st.vertices, OV.aabb(st.vertices), res, indices=st.indices(),
+                          tangents=OV.generate_tangents(st.vertices, st.strand_offsets), want_sums=True)
+        assert int(ref["counts"].max()) <= 255
+        assert np.array_equal(b.counts.cpu().numpy().view(np.uint32), ref["counts"]), n
+        assert np.array_equal(b.density.cpu().numpy(), OV.normalize(ref["dens_u8"])), n
+        q_ref = OV.quantize_tangents(ref["sums_ref"], ref["dens_u8"].astype(np.uint32)).reshape(-1, 4)
+        q = b.tangents.cpu().numpy().reshape(-1, 4)
+        assert np.abs(q.astype(np.int32) - q_ref.astype(np.int32)).max() <= 1, n
+        vol = dict(density=b.density, tangents=b.tangents, cells=b.cells, occupancy=b.occupancy)
+        g = D.raymarch(vol, res, b.cam, b.light, b.sp, b.rp, W, H, want_depth=True, want_info=True)
+        assert torch.equal(g["rgba"], b.rgba), n
+        g = {k: (v.cpu().numpy() if v is not None else None) for k, v in g.items()}
+        covered = np.nonzero(g["info"].reshape(-1, 4)[:, 0] == 1)[0]
+        assert len(covered) > 2000, (n, len(covered))
+        pix = np.unique(np.concatenate([rng.integers(0, W * H, 6000), rng.choice(covered, 2000, replace=False)]))
+        vol_np = dict(density=b.density.cpu().numpy(), tangents=b.tangents.cpu().numpy())
+        rep = compare_raymarch(g, vol_np, res, b.cam, b.light, b.sp, b.rp, W, H, pixels=pix)
+        print(n, len(st.strand_offsets) - 1, "strands", rep)
 
 
 def _long_segment_style(n_pairs, seed):
