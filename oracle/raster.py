@@ -26,9 +26,10 @@ def shadow_depth(xyz, indices, light_vp_cm, line_width, size):
 
 
 def raster(xyz, tangents, thickness, indices, cam, light, strand, render, density=None, density_f32=None,
-           res=(1, 1, 1), shadow_map=None, depth_buffer=None, pairs=None):
+           res=(1, 1, 1), shadow_map=None, depth_buffer=None, pairs=None, capacity=1 << 16):
     """Returns dict(segment, pixel, rgba (n,4), depth, kept) of every fragment
-    of the listed index pairs (all when None)."""
+    of the listed index pairs (all when None).  capacity: the fragment count
+    to allocate for first (a second pass runs when it is exceeded)."""
     xyz = np.ascontiguousarray(xyz, np.float32)
     tan = np.ascontiguousarray(tangents, np.float32)
     th = np.ascontiguousarray(thickness, np.float32)
@@ -40,7 +41,7 @@ def raster(xyz, tangents, thickness, indices, cam, light, strand, render, densit
     sh = (0, 0) if shadow_map is None else (shadow_map.shape[1], shadow_map.shape[0])
     db = None if depth_buffer is None else np.ascontiguousarray(depth_buffer, np.float32)
     lib.oracle_raster.restype = C.c_uint64
-    cap = 1 << 16
+    cap = max(int(capacity), 1)
     while True:
         seg = np.zeros(cap, np.uint32)
         pix = np.zeros(cap, np.uint32)

@@ -8,7 +8,7 @@ two stages the blend-order-sensitive k-buffer calls for:
 Also: the shadow-map depth pass, PPLL bookkeeping, pool overflow without
 out-of-bounds writes, the hybrid LoD (strand coverage x (1 - lod), raymarch
 coverage x lod, one shared list), and the full 1080p / 1024^2 / 32M-node
-configuration on a sampled subset of segments."""
+configuration on every segment."""
 import numpy as np
 import pytest
 import torch
@@ -65,7 +65,7 @@ class Scene:
         return ppll, cap
 
 
-def _fragment_parity(s: Scene, ppll, cap, pairs=None):
+def _fragment_parity(s: Scene, ppll, cap, pairs=None, capacity=1 << 16):
     n = min(ppll.used(), ppll.node_count)
     seg = cap["segment"][:n].cpu().numpy().view(np.uint32)
     pix = cap["pixel"][:n].cpu().numpy().view(np.uint32)
@@ -73,7 +73,7 @@ def _fragment_parity(s: Scene, ppll, cap, pairs=None):
     nodes = ppll.nodes.cpu().numpy().view(np.uint32).reshape(-1, 3)[:n]
     ni = s.np_inputs()
     o = ORs.raster(ni["xyz"], ni["tan"], ni["th"], ni["idx"], s.cam, s.light, s.sp, s.rp, density=ni["dens"],
-                   res=s.res, shadow_map=ni["smap"], pairs=pairs)
+                   res=s.res, shadow_map=ni["smap"], pairs=pairs, capacity=capacity)
     kept = o["kept"]
     o_key = o["segment"][kept].astype(np.uint64) << np.uint64(32) | o["pixel"][kept].astype(np.uint64)
     # nodes a block reserved for fragments its shading discarded are never
@@ -398,15 +398,15 @@ def test_hybrid_lod_scales_each_pass(c0, device):
     print("hybrid resolve", _resolve_parity(ph))
 
 
-def test_136k_strands_full_scale_sampled(device):
-    """136,320 strands at 1920x1080, 1024^2 shadow map, k=16 and a 32M-node
-    pool; the oracle checks 4,000 random segments and the whole resolve."""
+def test_136k_strands_full_scale(device):
+    """136,320 strands (1,635,840 segments) at 1920x1080, 1024^2 shadow map,
+    k=16 and a 32M-node pool: the oracle checks every fragment of every
+    segment (16.7 M) and the whole resolve."""
     s = Scene(synth.strands_136k(), (256, 256, 256), 1920, 1080, device)
     ppll, cap = s.raster(1 << 25)
     assert ppll.used() < ppll.node_count
-    rng = np.random.default_rng(5)
-    pairs = rng.choice(s.idx.shape[0] // 2, 4000, replace=False)
-    rep = _fragment_parity(s, ppll, cap, pairs=pairs)
+    rep = _fragment_parity(s, ppll, cap, capacity=ppll.used())
+    assert rep["fragments"] > 16_000_000
     print("136k strands raster", rep, "nodes", ppll.used())
     err = _resolve_parity(ppll)
     print("136k strands resolve", err)
