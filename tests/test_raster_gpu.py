@@ -202,6 +202,53 @@ def test_resolve_of_crafted_lists(device):
     assert err <= RESOLVE_TOL, err
 
 
+def _crafted_ppll(device, w, h, lengths, rng, past_pool=()):
+    """Host-written lists of the given lengths per pixel (random colours,
+    depths in (0, 1] with ties); a pixel of past_pool links, midway, a node
+    index one past the pool."""
+    heads = np.full(w * h, NULL, np.uint32)
+    parts, n_nodes = [], 0
+    for p in range(w * h):
+        n = int(lengths[p])
+        if n == 0:
+            continue
+        ids = n_nodes + np.arange(n, dtype=np.uint32)
+        colour = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32) | np.uint32(0x60000000)
+        depth = (rng.integers(1, 64, n) / 64).astype(np.float32)
+        nxt = np.concatenate([[NULL], ids[:-1]]).astype(np.uint32)   # the newest node (last id) is the head
+        parts.append(np.stack([colour, depth.view(np.uint32), nxt], axis=1))
+        heads[p] = ids[-1]
+        n_nodes += n
+    nodes = np.concatenate(parts).reshape(-1) if parts else np.zeros(3, np.uint32)
+    count = max(n_nodes, 1)
+    for p in past_pool:
+        if heads[p] != NULL and lengths[p] > 4:
+            nodes[3 * (int(heads[p]) - 3) + 2] = count   # the fourth node links one past the pool
+    ppll = D.PPLL(w, h, count, device)
+    ppll.heads.copy_(torch.from_numpy(heads.view(np.int32)))
+    ppll.nodes.copy_(torch.from_numpy(nodes[:3 * count].view(np.int32)))
+    return ppll, heads, nodes[:3 * count]
+
+
+def test_resolve_long_lists_pool_ends_and_odd_sizes(device):
+    """Lists at and past the resolve's 16 + 1024 node cap (the nodes beyond
+    it are never read), lists that link a node index past the pool (they end
+    there: the walk reads one node ahead, so this is its bound), and images
+    that are not multiples of the 16x16 workgroup tile, against the oracle."""
+    rng = np.random.default_rng(12)
+    for w, h in ((17, 5), (1, 1), (33, 19)):
+        lengths = rng.choice([0, 1, 3, 15, 16, 17, 40, 1039, 1040, 1041, 1100], size=w * h,
+                             p=[0.2, 0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.05, 0.05, 0.05, 0.05])
+        if w * h == 1:
+            lengths[:] = 1100
+        past = rng.choice(w * h, size=w * h // 8, replace=False)
+        ppll, heads, nodes = _crafted_ppll(device, w, h, lengths, rng, past_pool=past)
+        out = ppll.resolve().cpu().numpy().astype(np.float64)
+        want = ORs.resolve(heads, nodes, w, h)
+        err = float(np.abs(out - want).max())
+        assert err <= RESOLVE_TOL, (w, h, err)
+
+
 def test_pool_overflow_drops_without_out_of_bounds(c0, device):
     n = 5000
     guard = 64
