@@ -495,7 +495,9 @@ int vkhr_raster_strands(const float* xyz, const float* tangents, const float* th
  * depth and blended unsorted, then the k-buffer blended back to front, all
  * onto the background (float4 W*H, nullable: opaque white, the cleared
  * target).  out (float4 W*H) receives the blended colour before the rgba8
- * store; pixels without nodes receive the background. */
+ * store; pixels without nodes receive the background.  Launch: 16x16-pixel
+ * workgroups that each reserve 64 KB of LDS, so at most two share a CU
+ * (fewer lists walked at once keep their node lines in L2). */
 int vkhr_ppll_resolve(const vkhr_ppll* ppll, const float* background, float* out, vkhr_stream stream);
 
 /* Raymarch fragments into the PPLL (the raymarch pass's ppll_node_data /
