@@ -287,7 +287,8 @@ __device__ __forceinline__ void adjoint_tile(const AdjointArgs& a, int x0, int y
 
 // blockIdx.z: 0 the primary rays, LAO and parameters, 1 the shadow rays.
 __global__ void __launch_bounds__(256)
-adjoint_kernel(AdjointArgs a) {
+adjoint_kernel(AdjointArgs) {
+    const AdjointArgs& a = kernel_args<AdjointArgs>();
     __shared__ float red[4][256 / kWave];
     __shared__ ScatterLds scatter[256 / kWave];
     adjoint_tile(a, blockIdx.x * 16, blockIdx.y * 16, red, scatter, blockIdx.z == 0 ? kWalkPrimary : kWalkShadow);

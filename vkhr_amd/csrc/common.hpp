@@ -98,4 +98,16 @@ __device__ __forceinline__ float3 load_xyz(const float* __restrict__ p, uint32_t
     return make_float3(p[3u * i + 0u], p[3u * i + 1u], p[3u * i + 2u]);
 }
 
+// The launch's argument block T read in place from the kernarg segment
+// (the explicit arguments start at its offset 0, so T must be the kernel's
+// first parameter).  Read through this reference, each field is loaded
+// from constant memory where it is used; taken from the by-value
+// parameter, the compiler holds the block's fields in scalar registers
+// across the kernel's loops, and a large block spills them into vector
+// register lanes.
+template <class T>
+__device__ __forceinline__ const T& kernel_args() {
+    return *(const T*)__builtin_amdgcn_kernarg_segment_ptr();
+}
+
 }  // namespace vkhr

@@ -273,13 +273,39 @@ __device__ __forceinline__ bool pcf3x3_separable(const RasterArgs& a, double u0,
     return true;
 }
 
+__device__ __forceinline__ void normalize3f(float v[3]) {
+    const float l = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const float s = l > 0.0f ? 1.0f / l : 0.0f;
+    v[0] *= s; v[1] *= s; v[2] *= s;
+}
+
+// Kajiya-Kay (kajiya-kay.glsl:7-24) in float32 for the colour pass: a
+// continuous term, so float32 stays within the fragment tolerance and
+// frees the registers and issue slots float64 takes; sqrt of a negative
+// rounding residue is 0.  The colours are the parameter blocks' floats.
+__device__ __forceinline__ void kajiya_kay_f32(const double diffuse[3], const double specular[3], double p,
+                                               const float t[3], const float l[3], const float e[3], double out[3]) {
+    const float ctl = t[0] * l[0] + t[1] * l[1] + t[2] * l[2];
+    const float cte = t[0] * e[0] + t[1] * e[1] + t[2] * e[2];
+    const float stl = sqrtf(fmaxf(1.0f - ctl * ctl, 0.0f)), ste = sqrtf(fmaxf(1.0f - cte * cte, 0.0f));
+    const float base = fmaxf(ctl * cte + stl * ste, 0.0f);
+    const float spec = powf(base, float(p));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = double(float(diffuse[c]) * stl + float(specular[c]) * spec);
+}
+
 // Per-wave LDS of the colour pass: the shading data of the wave's 64
 // segments (structure of arrays, so lanes reading different segments hit
-// different banks) and a ring of queued fragments.
+// different banks) and a ring of queued fragments.  Positions, clip w and
+// depth stay float64 (they decide coverage and the shadow test); the
+// tangents and thickness feed continuous terms only and are float32, which
+// brings a workgroup to 37 KB, four to a CU.
 constexpr int kQueue = 128;          // >= 2 x 64: a fill round never overflows
-enum SegField { kPa = 0, kPb = 3, kTa = 6, kTb = 9, kTha = 12, kThb, kWa, kWb, kAz, kBz, kSegFields };
+enum SegField { kPa = 0, kPb = 3, kWa = 6, kWb, kAz, kBz, kSegFields };
+enum SegFieldF32 { kTa = 0, kTb = 3, kTha = 6, kThb, kSegFieldsF32 };
 struct WaveShared {
     double seg[kSegFields][kWave];
+    float segf[kSegFieldsF32][kWave];
     uint32_t segment[kWave];
     double qt[kQueue];
     uint32_t qpix[kQueue];
@@ -367,13 +393,13 @@ __device__ void colour_segments(const RasterArgs& a, WaveShared& W, uint32_t s, 
         double h0 = a.thickness[ia], h1 = a.thickness[ib];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            W.seg[kTa + c][lane] = t0[c] + L.s0 * (t1[c] - t0[c]);
-            W.seg[kTb + c][lane] = t0[c] + L.s1 * (t1[c] - t0[c]);
+            W.segf[kTa + c][lane] = float(t0[c] + L.s0 * (t1[c] - t0[c]));
+            W.segf[kTb + c][lane] = float(t0[c] + L.s1 * (t1[c] - t0[c]));
             W.seg[kPa + c][lane] = pa[c] + L.s0 * (pb[c] - pa[c]);
             W.seg[kPb + c][lane] = pa[c] + L.s1 * (pb[c] - pa[c]);
         }
-        W.seg[kTha][lane] = h0 + L.s0 * (h1 - h0);
-        W.seg[kThb][lane] = h0 + L.s1 * (h1 - h0);
+        W.segf[kTha][lane] = float(h0 + L.s0 * (h1 - h0));
+        W.segf[kThb][lane] = float(h0 + L.s1 * (h1 - h0));
         W.seg[kWa][lane] = L.wa;
         W.seg[kWb][lane] = L.wb;
         W.seg[kAz][lane] = L.az;
@@ -444,15 +470,18 @@ __device__ void colour_segments(const RasterArgs& a, WaveShared& W, uint32_t s, 
             const double wa = W.seg[kWa][src], wb = W.seg[kWb][src];
             double iw = (1.0 - t) / wa + t / wb;
             double sp = (t / wb) / iw;
-            double P[3], T[3];
+            const float spf = float(sp);
+            double P[3];
+            float T[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const double p0 = W.seg[kPa + c][src], t0 = W.seg[kTa + c][src];
+                const double p0 = W.seg[kPa + c][src];
+                const float t0 = W.segf[kTa + c][src];
                 P[c] = p0 + sp * (W.seg[kPb + c][src] - p0);
-                T[c] = t0 + sp * (W.seg[kTb + c][src] - t0);
+                T[c] = t0 + spf * (W.segf[kTb + c][src] - t0);
             }
-            const double tha = W.seg[kTha][src];
-            double thick = tha + sp * (W.seg[kThb][src] - tha);
+            const double tha = W.segf[kTha][src];
+            double thick = tha + sp * (double(W.segf[kThb][src]) - tha);
             // gpaa (gpaa.glsl:33-51)
             double cl[4];
             xform4(a.vp, P[0], P[1], P[2], cl);
@@ -464,12 +493,15 @@ __device__ void colour_segments(const RasterArgs& a, WaveShared& W, uint32_t s, 
             if (frag) {
                 coverage *= 1.0 - a.lod;
                 coverage *= thick * (1.0 / 0.042);
-                double e[3] = {P[0] - a.eye[0], P[1] - a.eye[1], P[2] - a.eye[2]};
-                double l[3] = {a.light_origin[0] - P[0], a.light_origin[1] - P[1], a.light_origin[2] - P[2]};
-                normalize3(e);
-                normalize3(l);
                 double shading[3] = {1.0, 1.0, 1.0};
-                if (a.shading_model == 0) kajiya_kay<true>(a.hair_color, a.intensity, a.shininess, T, l, e, shading);
+                if (a.shading_model == 0) {
+                    float e[3] = {float(P[0] - a.eye[0]), float(P[1] - a.eye[1]), float(P[2] - a.eye[2])};
+                    float l[3] = {float(a.light_origin[0] - P[0]), float(a.light_origin[1] - P[1]),
+                                  float(a.light_origin[2] - P[2])};
+                    normalize3f(e);
+                    normalize3f(l);
+                    kajiya_kay_f32(a.hair_color, a.intensity, a.shininess, T, l, e, shading);
+                }
                 double occlusion = 1.0;
                 if (a.adsm_on == 1 && a.shading_model != 3) {
                     double ls[4];
@@ -513,7 +545,8 @@ __device__ void colour_segments(const RasterArgs& a, WaveShared& W, uint32_t s, 
 
 
 // The colour pass: one segment per lane, the whole target.
-__global__ void __launch_bounds__(kRasterBlock) raster_kernel(RasterArgs a) {
+__global__ void __launch_bounds__(kRasterBlock) raster_kernel(RasterArgs) {
+    const RasterArgs& a = kernel_args<RasterArgs>();
     __shared__ WaveShared shared[kRasterBlock / kWave];
     __shared__ uint32_t block_sums[3];  // fragments counted, reservation base, LDS allocation cursor
     if (threadIdx.x < 3) block_sums[threadIdx.x] = 0u;

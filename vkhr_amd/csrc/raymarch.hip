@@ -41,15 +41,15 @@ __device__ __forceinline__ void raymarch_block(const RaymarchArgs& a) {
 // steps, 256^3).
 template <bool kF32>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
-raymarch_kernel(RaymarchArgs a) {
-    raymarch_block<kF32>(a);
+raymarch_kernel(RaymarchArgs) {
+    raymarch_block<kF32>(kernel_args<RaymarchArgs>());
 }
 
 // The float density volume (the differentiable forward), at the register
 // budget the compiler picks.
 __global__ void __launch_bounds__(256)
-raymarch_f32_kernel(RaymarchArgs a) {
-    raymarch_block<true>(a);
+raymarch_f32_kernel(RaymarchArgs) {
+    raymarch_block<true>(kernel_args<RaymarchArgs>());
 }
 
 void matmul_cm(const float* x, const float* y, double* out) {  // x * y, column-major

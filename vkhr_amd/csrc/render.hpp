@@ -349,8 +349,6 @@ __device__ __forceinline__ double local_ambient_occlusion_cells_f32pow(const Den
 }
 
 // Kajiya-Kay (kajiya-kay.glsl:7-24); sqrt of a negative rounding residue is 0.
-// kF32Pow: the specular power in float32 (the strand rasterizer).
-template <bool kF32Pow = false>
 __device__ __forceinline__ void kajiya_kay(const double diffuse[3], const double specular[3], double p,
                                            const double t[3], const double l[3], const double e[3], double out[3],
                                            double* base_out = nullptr) {
@@ -359,7 +357,7 @@ __device__ __forceinline__ void kajiya_kay(const double diffuse[3], const double
     double stl = sqrt(fmax(1.0 - ctl * ctl, 0.0)), ste = sqrt(fmax(1.0 - cte * cte, 0.0));
     double base = fmax(ctl * cte + stl * ste, 0.0);
     if (base_out) *base_out = base;
-    double spec = kF32Pow ? double(powf(float(base), float(p))) : pow(base, p);
+    double spec = pow(base, p);
 #pragma unroll
     for (int c = 0; c < 3; ++c) out[c] = diffuse[c] * stl + specular[c] * spec;
 }
