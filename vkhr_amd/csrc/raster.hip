@@ -318,8 +318,42 @@ __device__ __forceinline__ void wave_lds_fence() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// The shadow pass keeps its block's depths per texel in an LDS table: the
+// block's 256 segments are neighbours in the load-time order and cover the
+// same texels many times over, while a device-scope atomicMin on the map
+// is a memory-side atomic per fragment.  A fragment whose texel finds no
+// slot within kDepthProbes probes takes the global atomic at once; the
+// table's minima are flushed with one global atomicMin per texel at the
+// end.  The minimum does not depend on the order: the map is the same bit
+// for bit.
+constexpr uint32_t kDepthSlotBits = 10, kDepthSlots = 1u << kDepthSlotBits, kDepthProbes = 8;
+constexpr uint32_t kDepthEmpty = 0xffffffffu;   // no texel index (maps are at most 16384^2)
+
+struct DepthTable {
+    uint32_t key[kDepthSlots];
+    uint32_t bits[kDepthSlots];
+};
+
+// The slot holding texel k (claimed when free), or kDepthEmpty.
+__device__ __forceinline__ uint32_t depth_slot(DepthTable& t, uint32_t k) {
+    const uint32_t h = (k * 2654435761u) >> (32 - kDepthSlotBits);
+    for (uint32_t i = 0; i < kDepthProbes; ++i) {
+        const uint32_t slot = (h + i) & (kDepthSlots - 1u);
+        const uint32_t prev = atomicCAS(&t.key[slot], kDepthEmpty, k);
+        if (prev == kDepthEmpty || prev == k) return slot;
+    }
+    return kDepthEmpty;
+}
+
 __global__ void __launch_bounds__(kRasterBlock)
-shadow_raster_kernel(RasterArgs a) {
+shadow_raster_kernel(RasterArgs) {
+    const RasterArgs& a = kernel_args<RasterArgs>();
+    __shared__ DepthTable table;
+    for (uint32_t i = threadIdx.x; i < kDepthSlots; i += blockDim.x) {
+        table.key[i] = kDepthEmpty;
+        table.bits[i] = kDepthEmpty;
+    }
+    __syncthreads();
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     LineSetup L;
     L.valid = false;
@@ -336,8 +370,16 @@ shadow_raster_kernel(RasterArgs a) {
     double t;
     while (next_fragment(L, w, px, py, t)) {
         const double z = L.az + t * (L.bz - L.az);
-        atomicMin(&a.depth_bits[uint32_t(py) * uint32_t(a.width) + uint32_t(px)], __float_as_uint(float(z)));
+        const uint32_t texel = uint32_t(py) * uint32_t(a.width) + uint32_t(px), bits = __float_as_uint(float(z));
+        const uint32_t slot = depth_slot(table, texel);
+        if (slot != kDepthEmpty)
+            __hip_atomic_fetch_min(&table.bits[slot], bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+            atomicMin(&a.depth_bits[texel], bits);
     }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kDepthSlots; i += blockDim.x)
+        if (table.key[i] != kDepthEmpty) atomicMin(&a.depth_bits[table.key[i]], table.bits[i]);
 }
 
 // The PPLL insert of the colour pass (ppll.glsl:23-48).  Node ids come
